@@ -1,0 +1,278 @@
+/*
+ * xpbd_hip.h -- C-ABI of the MI355X (gfx950) XPBD soft-tissue solver.
+ *
+ * Plain C types only: pointers to host arrays, sizes and int status codes.
+ * Every call returns XPBD_OK (0) or a negative XPBD_E_* code; the message of
+ * the last failure on a handle is xpbd_last_error(handle) (NULL handle: the
+ * last failure of xpbd_create / xpbd_batch_create on this thread).
+ *
+ * Threading: one handle owns one HIP stream on the device that was current
+ * when it was created.  Calls on one handle are not re-entrant; different
+ * handles may be driven from different threads (the reference drives its
+ * solver from a single physics thread, src/simulation/Simulation.cpp:331-336).
+ *
+ * Each entry point names the reference interface it replaces.  The reference
+ * is C++ (MAVYWAVY902/xpbd-tissue-sim); paths are relative to its root.
+ */
+#ifndef XPBD_HIP_H
+#define XPBD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XPBD_OK 0
+#define XPBD_E_INVALID_ARG (-1)  /* null pointer, bad size, bad enum */
+#define XPBD_E_HIP (-2)          /* HIP runtime failure (message has the HIP error) */
+#define XPBD_E_NONFINITE (-3)    /* NaN or Inf in an input array */
+#define XPBD_E_DEGENERATE (-4)   /* zero-volume element or zero-mass vertex */
+#define XPBD_E_UNSUPPORTED (-5)  /* option not available in this build */
+#define XPBD_E_NOMEM (-6)        /* host or device allocation failed */
+#define XPBD_E_STATE (-7)        /* call not valid in the handle's current state */
+#define XPBD_E_IO (-8)           /* file could not be read or written */
+
+/* Solver types.  XPBD_SOLVER_JACOBI fills the reference's PARALLEL_JACOBI slot
+ * (include/common/XPBDEnumTypes.hpp:6-11, "Parallel-Jacobi" in
+ * include/config/simobject/XPBDMeshObjectConfig.hpp:26-34);
+ * XPBD_SOLVER_GS_COLORED is the new "GPU-Gauss-Seidel" value beside it. */
+#define XPBD_SOLVER_JACOBI 0
+#define XPBD_SOLVER_GS_COLORED 1
+
+/* Signed-distance field kinds (src/geometry/SphereSDF.cpp, BoxSDF.cpp). */
+#define XPBD_SDF_NONE 0
+#define XPBD_SDF_SPHERE 1
+#define XPBD_SDF_BOX 2
+
+typedef struct xpbd_handle xpbd_handle;
+typedef struct xpbd_handle xpbd_batch;
+typedef struct xpbd_mlp xpbd_mlp;
+
+/* One tetrahedral object.  Replaces what XPBDMeshObject_::setup() reads from
+ * the mesh file and config (src/simobject/XPBDMeshObject.cpp:102-160):
+ * rest positions, elements, per-element material (the reference maps element
+ * classes to ElasticMaterial, XPBDMeshObject.cpp:287-300), fixed vertices
+ * (XPBDMeshObjectBase.hpp fixVertex) and the surface faces used by collision. */
+typedef struct xpbd_mesh_desc {
+    int32_t num_vertices;
+    int32_t num_tets;
+    const double* x0;        /* 3*V rest positions, xyz per vertex */
+    const double* v0;        /* 3*V initial velocities, or NULL for zero */
+    const int32_t* tets;     /* 4*T vertex indices */
+    const double* E;         /* T Young's moduli [Pa] */
+    const double* nu;        /* T Poisson ratios */
+    const double* rho;       /* T densities [kg/m^3] */
+    const uint8_t* pinned;   /* V flags (nonzero = fixed), or NULL */
+    int32_t num_faces;       /* surface triangles for SDF contact, 0 = none */
+    const int32_t* faces;    /* 3*F vertex indices */
+} xpbd_mesh_desc;
+
+/* Solver / step parameters: time-step, g-accel, num-solver-iters and
+ * solver-type of the reference config (SimulationConfig.hpp:185-195,
+ * XPBDMeshObjectConfig.hpp:26-125). */
+typedef struct xpbd_params {
+    double dt;                /* time step [s] */
+    double g;                 /* gravity magnitude, acts along -z [m/s^2] */
+    int32_t iters;            /* solver iterations per step */
+    int32_t solver;           /* XPBD_SOLVER_* */
+    int32_t deterministic;    /* Jacobi only: 1 = fixed-order accumulation */
+    int32_t collision_every;  /* re-detect contacts every k steps; 0 = never */
+} xpbd_params;
+
+/* ---- object life cycle ------------------------------------------------- */
+
+/* Build the solver for one object.  Rest-state precompute (Dm^-1, volume,
+ * lumped masses, Lame parameters, compliances) runs in FP64 on the host and is
+ * stored in FP32 on the device.  Replaces XPBDMeshObject_::setup +
+ * _calculatePerVertexQuantities + _createElasticConstraints
+ * (src/simobject/XPBDMeshObject.cpp:102-160, 277-331, 352-417) together with
+ * the XPBDSolver constructor (include/solver/xpbd_solver/XPBDSolver.hpp:47-61). */
+int xpbd_create(const xpbd_mesh_desc* mesh, const xpbd_params* params, xpbd_handle** out);
+
+/* Release all host and device resources of a handle (NULL is a no-op). */
+int xpbd_destroy(xpbd_handle* h);
+
+/* Message of the last failure on h (or of the last failed create on this
+ * thread when h is NULL); never NULL, "" when there was none. */
+const char* xpbd_last_error(const xpbd_handle* h);
+
+/* Advance nsteps time steps on the device.  One step replaces
+ * Simulation::_timeStep for this object (src/simulation/Simulation.cpp:197-276):
+ * optional contact re-detection, XPBDMeshObject_::update (predict, solve with
+ * lambda reset, pin reset; XPBDMeshObject.cpp:426-505), velocityUpdate
+ * (XPBDMeshObject.cpp:507-535).  The solve replaces XPBDSolver::solve
+ * (XPBDSolver.hpp:66-97) with XPBDGaussSeidelSolver / XPBDJacobiSolver
+ * iteration semantics (XPBDGaussSeidelSolver.hpp:29-80,
+ * XPBDJacobiSolver.hpp:79-114).  Steps are replayed from a captured HIP graph. */
+int xpbd_step(xpbd_handle* h, int32_t nsteps);
+
+/* Change the number of solver iterations (reference: num-solver-iters). */
+int xpbd_set_iterations(xpbd_handle* h, int32_t iters);
+
+/* Block until all work queued on the handle's stream has finished. */
+int xpbd_synchronize(xpbd_handle* h);
+
+/* ---- state ---------------------------------------------------------------- */
+
+/* Current positions (3*V, xyz per vertex) in double / float.  Replaces reading
+ * Geometry::Mesh::vertices() after the solve (the reference's device path
+ * copies back at XPBDParallelJacobiSolver.hpp:111-113). */
+int xpbd_get_positions(xpbd_handle* h, double* out);
+int xpbd_get_positions_f32(xpbd_handle* h, float* out);
+/* Current velocities (3*V) (XPBDMeshObjectBase.hpp vertexVelocity). */
+int xpbd_get_velocities(xpbd_handle* h, double* out);
+/* Overwrite positions and/or velocities (either may be NULL). */
+int xpbd_set_state(xpbd_handle* h, const double* x, const double* v);
+/* Lumped vertex masses (V) as computed at set-up (XPBDMeshObject.cpp:277-313). */
+int xpbd_get_masses(xpbd_handle* h, double* out);
+/* Lagrange multipliers (2*T, (lambda_D, lambda_H) per tet, in the caller's tet
+ * order) left by the last solve (CombinedConstraintProjector::lambda()). */
+int xpbd_get_lambdas(xpbd_handle* h, double* out);
+
+/* ---- diagnostics: FP64 on the device ------------------------------------- */
+
+/* Constraint residual r = C + alpha/dt^2 * lambda of every elastic constraint
+ * after the last solve; ||r||_2 into *norm_out and, if vec_out != NULL, the
+ * 2*T entries (Dev, Hyd per tet, caller's tet order).  Replaces
+ * XPBDSolver::_calculateConstraintResidual + constraintResidual()
+ * (XPBDSolver.hpp:130-135, 327-391). */
+int xpbd_constraint_residual(xpbd_handle* h, double* norm_out, double* vec_out);
+
+/* Residual policy (reference "residual-policy", XPBDMeshObjectConfig.hpp:45-53):
+ * 0 = never (default), 1 = every step: ||r||_2 is evaluated on the device at the
+ * end of every solve, before the pin reset, where XPBDSolver::solve computes it
+ * for EVERY_SUBSTEP (XPBDSolver.hpp:91-96).  xpbd_last_step_residual returns
+ * the value of the last step (XPBD_E_STATE when the policy is off). */
+int xpbd_set_residual_policy(xpbd_handle* h, int32_t policy);
+int xpbd_last_step_residual(xpbd_handle* h, double* out);
+
+/* Total strain energy sum(C^2 / alpha) over all deviatoric and hydrostatic
+ * constraints (XPBDMeshObject_::totalStrainEnergy, XPBDMeshObject.cpp:537-549). */
+int xpbd_strain_energy(xpbd_handle* h, double* out);
+
+/* ---- solver ordering (parity checks) ----------------------------------- */
+
+/* Number of colours of the Gauss-Seidel tet colouring (1 for Jacobi). */
+int xpbd_num_colors(xpbd_handle* h, int32_t* out);
+/* Order in which the solver visits tets (T entries, caller's tet indices).
+ * Colour-GS equals sequential Gauss-Seidel in exactly this order. */
+int xpbd_get_tet_order(xpbd_handle* h, int32_t* out);
+/* Rank of every surface face in the contact visiting order (F entries). */
+int xpbd_get_face_rank(xpbd_handle* h, int32_t* out);
+
+/* ---- contacts ------------------------------------------------------------- */
+
+/* Kinematic SDF collider for the object: pose = (px, py, pz, qx, qy, qz, qw),
+ * params: sphere radius in params[0], box edge lengths in params[0..2].
+ * Replaces the fixed RigidSphere / RigidBox colliders whose SDFs are queried by
+ * CollisionScene::_collideObjectPair (src/collision/CollisionScene.cpp:171-214).
+ * Resets the collider clock: the pose at step n after this call is
+ * pose + lin_vel * n * dt (see xpbd_set_sdf_velocity). */
+int xpbd_set_sdf(xpbd_handle* h, int32_t kind, const double* pose7, const double* params3);
+/* Linear velocity of the collider (m/s), applied from the last xpbd_set_sdf. */
+int xpbd_set_sdf_velocity(xpbd_handle* h, const double* vel3);
+/* Run contact detection now on the current positions (the detection step of
+ * Simulation::_timeStep, Simulation.cpp:202-241) and replace the contact set;
+ * clearCollisionConstraints + addStaticCollisionConstraint
+ * (XPBDMeshObject.cpp:180-251). */
+int xpbd_detect_contacts(xpbd_handle* h);
+/* Copy out the current contact set (at most cap entries), in solver order:
+ * face index, surface point p (3), normal n (3), barycentrics (3). */
+int xpbd_get_contacts(xpbd_handle* h, int32_t cap, int32_t* count, int32_t* face, double* p,
+                      double* n, double* bary);
+
+/* ---- timing --------------------------------------------------------------- */
+
+/* Kernel timing mode: when on, xpbd_step launches without the HIP graph and
+ * brackets every projection-kernel launch with HIP events on the handle's
+ * stream; xpbd_kernel_stats returns the summed milliseconds and the number of
+ * solver iterations timed since the last reset. */
+int xpbd_set_kernel_timing(xpbd_handle* h, int32_t on);
+int xpbd_kernel_stats(xpbd_handle* h, double* proj_ms, int64_t* iterations, int64_t* launches);
+
+/* ---- batched objects -------------------------------------------------------- */
+
+/* Many independent objects advanced together in one captured substep: every
+ * launch covers all instances (element and vertex arrays are concatenated
+ * with per-instance offsets; each instance keeps its own SDF collider).
+ * The reference steps objects one after another (Simulation.cpp:246-255). */
+int xpbd_batch_create(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* params,
+                      xpbd_batch** out);
+int xpbd_batch_num_instances(xpbd_batch* b, int32_t* out);
+int xpbd_batch_set_sdf(xpbd_batch* b, int32_t instance, int32_t kind, const double* pose7,
+                       const double* params3);
+int xpbd_batch_get_positions(xpbd_batch* b, int32_t instance, double* out);
+/* Time the kernel variants and launch shapes on this batch and keep the
+ * fastest; results are cached in cache_path (may be NULL) keyed by shape. */
+int xpbd_autotune(xpbd_batch* b, const char* cache_path);
+/* Batch handles are also plain handles: xpbd_step, xpbd_destroy, ... apply. */
+
+/* ---- host-side mesh utilities (no GPU needed) ------------------------------ */
+
+/* Structured grid of nx*ny*nz cells split 6 tets per cell (conforming);
+ * x: 3*(nx+1)*(ny+1)*(nz+1), tets: 24*nx*ny*nz.  Vertex (i,j,k) has index
+ * i + (nx+1)*(j + (ny+1)*k). */
+int xpbd_mesh_grid(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,
+                   const double* origin3, double* x, int32_t* tets);
+/* Boundary triangles (faces of exactly one tet); faces must hold 12*T ints.
+ * Replaces the surface-face search of TetMesh (src/geometry/TetMesh.cpp:123-141). */
+int xpbd_mesh_surface(int32_t num_vertices, int32_t num_tets, const int32_t* tets,
+                      int32_t* num_faces, int32_t* faces);
+/* Greedy conflict colouring (items of k vertices; no shared vertex within a
+ * colour).  Returns the colour count in *num_colors. */
+int xpbd_color_items(int32_t num_vertices, int32_t num_items, int32_t verts_per_item,
+                     const int32_t* items, int32_t* colors, int32_t* num_colors);
+/* GMSH 4.1 ASCII reader (src/utils/MeshUtils.cpp:92-218): first call with
+ * NULL arrays to get the counts, then again with arrays of that size. */
+int xpbd_read_msh(const char* path, int32_t* num_vertices, int32_t* num_tets, int32_t* num_tris,
+                  double* x, int32_t* tets, int32_t* tris);
+
+/* ---- learned warm start ----------------------------------------------------- */
+
+#define XPBD_MLP_IN 24
+#define XPBD_MLP_HIDDEN 128
+#define XPBD_MLP_OUT 3
+
+/* A 24 -> 128 -> 128 -> 3 per-vertex MLP with SiLU, BF16 weights and
+ * activations, FP32 accumulation on MFMA.  No reference counterpart: the
+ * learned warm start is new in this solver.  Parameter layout (float32,
+ * 20,099 values): W1[128][24], b1[128], W2[128][128], b2[128], W3[3][128], b3[3]. */
+int xpbd_mlp_create(xpbd_mlp** out);
+int xpbd_mlp_destroy(xpbd_mlp* m);
+int xpbd_mlp_num_params(int32_t* out);
+int xpbd_mlp_set_params(xpbd_mlp* m, const float* params);
+int xpbd_mlp_get_params(xpbd_mlp* m, float* params);
+/* Feature / target scaling: y_model = scale_out * MLP((x - mean_in) / std_in). */
+int xpbd_mlp_set_scaling(xpbd_mlp* m, const float* mean_in24, const float* std_in24, float scale_out);
+/* Forward on n host rows (n x 24) -> out (n x 3). */
+int xpbd_mlp_forward(xpbd_mlp* m, int32_t n, const float* x, float* out);
+/* Mean-squared-error loss and parameter gradient on n rows (host arrays);
+ * grad receives 20,099 floats.  Used by the gradient checks. */
+int xpbd_mlp_loss_grad(xpbd_mlp* m, int32_t n, const float* x, const float* y, double* loss,
+                       float* grad);
+/* Adam training on device-resident samples: epochs passes of minibatches. */
+int xpbd_mlp_train(xpbd_mlp* m, int32_t n, const float* x, const float* y, int32_t batch,
+                   int32_t epochs, float lr, uint64_t seed, double* final_loss);
+
+/* Record training pairs from the plain solver on h: for nsteps steps, the 24
+ * features of every free vertex after the prediction and the solver's
+ * position correction (x_solved - x_predicted).  Arrays hold cap rows. */
+int xpbd_collect_training(xpbd_handle* h, int32_t nsteps, int32_t cap, float* x, float* y,
+                          int32_t* rows);
+/* Features of every vertex for the current state (V x 24), for inspection. */
+int xpbd_features(xpbd_handle* h, float* out);
+/* Attach (or detach with NULL) a network to the handle for learned steps. */
+int xpbd_attach_mlp(xpbd_handle* h, xpbd_mlp* m);
+/* One learned step: predict, apply the network's correction, iters solver
+ * iterations.  If the residual exceeds fallback_ratio * ref_residual the step
+ * is redone from the saved state with the handle's full iteration count;
+ * *fell_back reports it and *residual the final ||r||_2. */
+int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_residual, double fallback_ratio,
+                      int32_t* fell_back, double* residual);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XPBD_HIP_H */

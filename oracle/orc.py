@@ -1,0 +1,234 @@
+"""ctypes wrapper of liboracle.so, the FP64 CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  It is the checker, never the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run `make -C {_HERE}`")
+lib = C.CDLL(LIB_PATH)
+
+GS, JACOBI = 0, 1
+SDF_NONE, SDF_SPHERE, SDF_BOX = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _sig(name, args, res=None):
+    f = getattr(lib, name)
+    f.argtypes = args
+    f.restype = res
+    return f
+
+
+_sig("orc_lame", [C.c_double, C.c_double, _dp, _dp])
+_sig("orc_rest_tet", [_dp, _dp, _dp, _dp, _dp, _dp], C.c_int)
+_sig("orc_eval_tet", [_dp, _dp, C.c_double, _dp, _dp])
+_sig("orc_deformation_gradient", [_dp, _dp, _dp])
+_sig("orc_project_tet", [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp])
+_sig("orc_sdf_eval", [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp])
+_sig("orc_frank_wolfe", [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp])
+_sig("orc_barycentric", [_dp, _dp, _dp, _dp, _dp])
+_sig("orc_create", [C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp, _dp, _u8p, C.c_double, C.c_double, C.c_int,
+                    C.c_int], C.c_void_p)
+_sig("orc_destroy", [C.c_void_p])
+_sig("orc_set_order", [C.c_void_p, _ip])
+_sig("orc_set_mode", [C.c_void_p, C.c_int, C.c_int])
+_sig("orc_set_faces", [C.c_void_p, C.c_int, _ip])
+_sig("orc_set_sdf", [C.c_void_p, C.c_int, _dp, _dp, _dp])
+_sig("orc_set_state", [C.c_void_p, _dp, _dp])
+_sig("orc_detect", [C.c_void_p], C.c_int)
+_sig("orc_num_contacts", [C.c_void_p], C.c_int)
+_sig("orc_get_contacts", [C.c_void_p, _ip, _dp, _dp, _dp])
+_sig("orc_step", [C.c_void_p, C.c_int])
+_sig("orc_predict", [C.c_void_p])
+_sig("orc_get_x", [C.c_void_p, _dp])
+_sig("orc_get_xprev", [C.c_void_p, _dp])
+_sig("orc_get_v", [C.c_void_p, _dp])
+_sig("orc_get_lambda", [C.c_void_p, _dp])
+_sig("orc_get_masses", [C.c_void_p, _dp])
+_sig("orc_get_rest", [C.c_void_p, _dp, _dp])
+_sig("orc_get_compliance", [C.c_void_p, _dp, _dp, _dp])
+_sig("orc_residual", [C.c_void_p, _dp], C.c_double)
+_sig("orc_energy", [C.c_void_p], C.c_double)
+_sig("orc_set_contact_order", [C.c_void_p, _ip])
+_sig("orc_last_step_residual", [C.c_void_p], C.c_double)
+
+
+def p(a, ct=C.c_double):
+    if a is None:
+        return C.cast(None, C.POINTER(ct))
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _d(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def lame(E, nu):
+    mu, lam = C.c_double(), C.c_double()
+    lib.orc_lame(E, nu, C.byref(mu), C.byref(lam))
+    return mu.value, lam.value
+
+
+def rest_tet(X):
+    X = _d(X).reshape(4, 3)
+    Q = np.empty(9)
+    vol = C.c_double()
+    rc = lib.orc_rest_tet(p(X[0].copy()), p(X[1].copy()), p(X[2].copy()), p(X[3].copy()), p(Q), C.byref(vol))
+    return Q.reshape(3, 3).T.copy(), vol.value, rc  # Q as a 3x3 (row, col) matrix
+
+
+def _qcm(Q):
+    """3x3 matrix -> column-major 9 vector."""
+    return _d(np.asarray(Q).T.reshape(9))
+
+
+def eval_tet(x12, Q, gamma):
+    C2 = np.empty(2)
+    g = np.empty(24)
+    lib.orc_eval_tet(p(_d(x12).reshape(12)), p(_qcm(Q)), gamma, p(C2), p(g))
+    return C2, g
+
+
+def deformation_gradient(x12, Q):
+    F = np.empty(9)
+    lib.orc_deformation_gradient(p(_d(x12).reshape(12)), p(_qcm(Q)), p(F))
+    return F.reshape(3, 3).T.copy()
+
+
+def project_tet(x12, w4, Q, aD, aH, gamma, dt, lam2):
+    lam = _d(lam2).copy()
+    dx = np.empty(12)
+    lib.orc_project_tet(p(_d(x12).reshape(12)), p(_d(w4)), p(_qcm(Q)), aD, aH, gamma, dt, p(lam), p(dx))
+    return dx, lam
+
+
+def sdf_eval(kind, pos, quat, prm, x):
+    d = C.c_double()
+    g = np.empty(3)
+    lib.orc_sdf_eval(kind, p(_d(pos)), p(_d(quat)), p(_d(prm)), p(_d(x)), C.byref(d), p(g))
+    return d.value, g
+
+
+def frank_wolfe(kind, pos, quat, prm, p1, p2, p3):
+    x = np.empty(3)
+    lib.orc_frank_wolfe(kind, p(_d(pos)), p(_d(quat)), p(_d(prm)), p(_d(p1)), p(_d(p2)), p(_d(p3)), p(x))
+    return x
+
+
+def barycentric(pt, a, b, c):
+    out = np.empty(3)
+    lib.orc_barycentric(p(_d(pt)), p(_d(a)), p(_d(b)), p(_d(c)), p(out))
+    return out
+
+
+class Sim:
+    """FP64 reference-semantics simulation of one object."""
+
+    def __init__(self, mesh, dt=1e-3, g=9.81, iters=10, mode="gs"):
+        self.nv, self.nt = len(mesh.x0), len(mesh.tets)
+        self._x0 = _d(mesh.x0)
+        self._v0 = None if mesh.v0 is None else _d(mesh.v0)
+        self._t = np.ascontiguousarray(mesh.tets, np.int32)
+        self._E, self._nu, self._rho = _d(mesh.E), _d(mesh.nu), _d(mesh.rho)
+        self._pin = None if mesh.pinned is None else np.ascontiguousarray(mesh.pinned, np.uint8)
+        self.s = lib.orc_create(self.nv, self.nt, p(self._x0), p(self._v0), p(self._t, C.c_int32), p(self._E),
+                                p(self._nu), p(self._rho), p(self._pin, C.c_uint8), dt, g, iters,
+                                GS if mode == "gs" else JACOBI)
+        self.nf = 0
+        if mesh.faces is not None and len(mesh.faces):
+            self._f = np.ascontiguousarray(mesh.faces, np.int32)
+            self.nf = len(self._f)
+            lib.orc_set_faces(self.s, self.nf, p(self._f, C.c_int32))
+
+    def __del__(self):
+        if getattr(self, "s", None):
+            lib.orc_destroy(self.s)
+            self.s = None
+
+    def set_order(self, order):
+        o = np.ascontiguousarray(order, np.int32)
+        lib.orc_set_order(self.s, p(o, C.c_int32))
+
+    def set_contact_order(self, face_rank):
+        r = np.ascontiguousarray(face_rank, np.int32)
+        lib.orc_set_contact_order(self.s, p(r, C.c_int32))
+
+    def set_mode(self, mode, iters):
+        lib.orc_set_mode(self.s, GS if mode == "gs" else JACOBI, iters)
+
+    def set_sdf(self, kind, pos, quat=(0, 0, 0, 1), params=(0, 0, 0)):
+        lib.orc_set_sdf(self.s, kind, p(_d(pos)), p(_d(quat)), p(_d(params)))
+
+    def set_state(self, x=None, v=None):
+        lib.orc_set_state(self.s, p(None if x is None else _d(x)), p(None if v is None else _d(v)))
+
+    def detect(self):
+        return lib.orc_detect(self.s)
+
+    def contacts(self):
+        n = lib.orc_num_contacts(self.s)
+        face = np.empty(n, np.int32)
+        pp, nn, bb = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3))
+        lib.orc_get_contacts(self.s, p(face, C.c_int32), p(pp), p(nn), p(bb))
+        return face, pp, nn, bb
+
+    def step(self, n=1, detect=False):
+        for _ in range(n):
+            lib.orc_step(self.s, 1 if detect else 0)
+
+    def predict(self):
+        lib.orc_predict(self.s)
+
+    def _get(self, fn, shape):
+        out = np.empty(shape)
+        fn(self.s, p(out))
+        return out
+
+    def x(self):
+        return self._get(lib.orc_get_x, (self.nv, 3))
+
+    def xprev(self):
+        return self._get(lib.orc_get_xprev, (self.nv, 3))
+
+    def v(self):
+        return self._get(lib.orc_get_v, (self.nv, 3))
+
+    def lambdas(self):
+        return self._get(lib.orc_get_lambda, (self.nt, 2))
+
+    def masses(self):
+        return self._get(lib.orc_get_masses, (self.nv,))
+
+    def rest(self):
+        Q = np.empty((self.nt, 9))
+        vol = np.empty(self.nt)
+        lib.orc_get_rest(self.s, p(Q), p(vol))
+        return Q, vol
+
+    def compliance(self):
+        aD, aH, g = np.empty(self.nt), np.empty(self.nt), np.empty(self.nt)
+        lib.orc_get_compliance(self.s, p(aD), p(aH), p(g))
+        return aD, aH, g
+
+    def residual(self, vec=False):
+        out = np.empty((self.nt, 2)) if vec else None
+        n = lib.orc_residual(self.s, p(out))
+        return (n, out) if vec else n
+
+    def energy(self):
+        return lib.orc_energy(self.s)
+
+    def last_step_residual(self):
+        return lib.orc_last_step_residual(self.s)

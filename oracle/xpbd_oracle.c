@@ -1,0 +1,797 @@
+/*
+ * xpbd_oracle.c -- CPU FP64 oracle for the XPBD Neo-Hookean solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load liboracle.so.  The shipped solver (the HIP library) never links it.
+ *
+ * It is a clean-room restatement, in plain C and double precision, of the
+ * reference CPU solver (MAVYWAVY902/xpbd-tissue-sim), which cannot be built
+ * here (Eigen, yaml-cpp, gmsh, Embree, ... are absent).  Each function names
+ * the reference file:line it follows.  Loop orders that decide floating-point
+ * rounding (Gauss-Seidel order, Jacobi buffer order, accumulation order of the
+ * 2x2 system) follow the reference so that the oracle reproduces it in FP64.
+ *
+ * Pinning: every function here is checked by tests/test_oracle.py against
+ * closed forms, invariants, finite differences, an independent float64
+ * autograd restatement, or brute force (see the test docstrings).  Functions
+ * whose bit-level rounding the reference decides through Eigen (the 3x3
+ * inverse in the rest-state precompute) are pinned by Q*Dm = I instead.
+ *
+ * Layout conventions (match the reference's Eigen storage):
+ *   positions  x[3*i + k]            (Eigen 3xN column-major = AoS xyz)
+ *   tets       tets[4*e + j]
+ *   Q          Q[9*e + 3*c + r] = Q(r, c)   (column-major 3x3, like Mat3r)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_API __attribute__((visibility("default")))
+
+enum { ORC_GS = 0, ORC_JACOBI = 1 };
+enum { ORC_SDF_NONE = 0, ORC_SDF_SPHERE = 1, ORC_SDF_BOX = 2 };
+
+/* ------------------------------------------------------------------------ */
+/* a1  material: include/simobject/ElasticMaterial.hpp:22-33                 */
+/* ------------------------------------------------------------------------ */
+ORC_API void orc_lame(double E, double nu, double* mu, double* lam)
+{
+    *mu = E / (2 * (1 + nu));
+    *lam = (E * nu) / ((1 + nu) * (1 - 2 * nu));
+}
+
+/* ------------------------------------------------------------------------ */
+/* a2  rest state: include/solver/constraint/ElementConstraint.hpp:36-50     */
+/*     Dm = [X0-X3, X1-X3, X2-X3] (columns), Q = Dm^-1, V = |det Dm| / 6.    */
+/*     The inverse is the closed-form adjugate / determinant.                */
+/* ------------------------------------------------------------------------ */
+static double det3(const double* M) /* column-major */
+{
+    return M[0] * (M[4] * M[8] - M[7] * M[5])
+         - M[3] * (M[1] * M[8] - M[7] * M[2])
+         + M[6] * (M[1] * M[5] - M[4] * M[2]);
+}
+
+ORC_API int orc_rest_tet(const double* X0, const double* X1, const double* X2,
+                         const double* X3, double* Q, double* vol)
+{
+    double D[9];
+    for (int k = 0; k < 3; k++) {
+        D[0 + k] = X0[k] - X3[k];
+        D[3 + k] = X1[k] - X3[k];
+        D[6 + k] = X2[k] - X3[k];
+    }
+    const double det = det3(D);
+    /* inverse(r, c) = cofactor(c, r) / det ; D(r, c) = D[3c + r] */
+#define DM(r, c) D[3 * (c) + (r)]
+    double inv[9];
+    inv[3 * 0 + 0] = (DM(1, 1) * DM(2, 2) - DM(1, 2) * DM(2, 1)) / det;
+    inv[3 * 1 + 0] = (DM(0, 2) * DM(2, 1) - DM(0, 1) * DM(2, 2)) / det;
+    inv[3 * 2 + 0] = (DM(0, 1) * DM(1, 2) - DM(0, 2) * DM(1, 1)) / det;
+    inv[3 * 0 + 1] = (DM(1, 2) * DM(2, 0) - DM(1, 0) * DM(2, 2)) / det;
+    inv[3 * 1 + 1] = (DM(0, 0) * DM(2, 2) - DM(0, 2) * DM(2, 0)) / det;
+    inv[3 * 2 + 1] = (DM(0, 2) * DM(1, 0) - DM(0, 0) * DM(1, 2)) / det;
+    inv[3 * 0 + 2] = (DM(1, 0) * DM(2, 1) - DM(1, 1) * DM(2, 0)) / det;
+    inv[3 * 1 + 2] = (DM(0, 1) * DM(2, 0) - DM(0, 0) * DM(2, 1)) / det;
+    inv[3 * 2 + 2] = (DM(0, 0) * DM(1, 1) - DM(0, 1) * DM(1, 0)) / det;
+#undef DM
+    memcpy(Q, inv, sizeof(inv));
+    *vol = fabs(det / 6);
+    return det == 0.0 ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a5  F = X Q : include/solver/constraint/ElementConstraint.hpp:78-119       */
+/* ------------------------------------------------------------------------ */
+static void compute_F(const double* p0, const double* p1, const double* p2,
+                      const double* p3, const double* Q, double* F)
+{
+    double X[9];
+    for (int k = 0; k < 3; k++) {
+        X[0 + k] = p0[k] - p3[k];
+        X[3 + k] = p1[k] - p3[k];
+        X[6 + k] = p2[k] - p3[k];
+    }
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++)
+            F[3 * c + r] = X[0 + r] * Q[3 * c + 0] + X[3 + r] * Q[3 * c + 1]
+                         + X[6 + r] * Q[3 * c + 2];
+}
+
+/* a3  deviatoric: include/solver/constraint/DeviatoricConstraint.hpp:112-152 */
+static double dev_eval(const double* F)
+{
+    double s = 0;
+    for (int i = 0; i < 9; i++) s += F[i] * F[i];
+    return sqrt(s);
+}
+
+static void dev_grad(const double* F, const double* Q, double C, double* g)
+{
+    const double inv_C = 1.0 / C;
+    /* column i of F Q^T (vertex i), i = 0..2 : sum_c F(r,c) Q(i,c) */
+    for (int i = 0; i < 3; i++)
+        for (int r = 0; r < 3; r++)
+            g[3 * i + r] = inv_C * (F[r] * Q[i] + F[3 + r] * Q[3 + i] + F[6 + r] * Q[6 + i]);
+    for (int r = 0; r < 3; r++) g[9 + r] = -g[r] - g[3 + r] - g[6 + r];
+}
+
+/* a4  hydrostatic: include/solver/constraint/HydrostaticConstraint.hpp:123-229 */
+static double det_F(const double* F)
+{
+    return F[0] * F[4] * F[8] - F[0] * F[7] * F[5] - F[3] * F[1] * F[8]
+         + F[3] * F[7] * F[2] + F[6] * F[1] * F[5] - F[6] * F[4] * F[2];
+}
+
+static double hyd_eval(const double* F, double gamma)
+{
+    const double J = det_F(F);
+    if (J >= 1) return -gamma + log(J);
+    const double a = J - 1;
+    return -gamma + a - a * a / 2.0 + a * a * a / 3.0;
+}
+
+static void cross3(const double* a, const double* b, double* c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static void hyd_grad(const double* F, const double* Q, double* g)
+{
+    double Fc[9];
+    cross3(F + 3, F + 6, Fc);     /* f1 x f2 */
+    cross3(F + 6, F, Fc + 3);     /* f2 x f0 */
+    cross3(F, F + 3, Fc + 6);     /* f0 x f1 */
+    const double J = det_F(F);
+    double fac;
+    if (J >= 1.0) fac = 1 / J;
+    else fac = 1 - (J - 1) + (J - 1) * (J - 1);
+    for (int i = 0; i < 3; i++)
+        for (int r = 0; r < 3; r++)
+            g[3 * i + r] = fac * (Fc[r] * Q[i] + Fc[3 + r] * Q[3 + i] + Fc[6 + r] * Q[6 + i]);
+    for (int r = 0; r < 3; r++) g[9 + r] = -g[r] - g[3 + r] - g[6 + r];
+}
+
+/* Evaluate both element constraints and their 12-vector gradients.
+ * C[0] = C_D, C[1] = C_H; grad[0..11] = dC_D, grad[12..23] = dC_H. */
+ORC_API void orc_eval_tet(const double* x12, const double* Q, double gamma,
+                          double* C, double* grad24)
+{
+    double F[9];
+    compute_F(x12, x12 + 3, x12 + 6, x12 + 9, Q, F);
+    C[0] = dev_eval(F);
+    dev_grad(F, Q, C[0], grad24);
+    C[1] = hyd_eval(F, gamma);
+    hyd_grad(F, Q, grad24 + 12);
+}
+
+ORC_API void orc_deformation_gradient(const double* x12, const double* Q, double* F)
+{
+    compute_F(x12, x12 + 3, x12 + 6, x12 + 9, Q, F);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a7  combined 2x2 projection (2nd order):                                  */
+/*     src/solver/xpbd_projector/CombinedNeohookeanConstraintProjector.cpp:98-179 */
+/*     Inputs are the four vertex positions (p[k] points at 3 doubles), the   */
+/*     four inverse masses, Q, compliances alpha_D/alpha_H (not yet divided   */
+/*     by dt^2), gamma, dt and the multipliers lam[2] (updated in place).     */
+/*     dx[12] receives the coordinate updates in vertex-major order.          */
+/* ------------------------------------------------------------------------ */
+static void project_combined(const double* const p[4], const double* w,
+                             const double* Q, double aD, double aH, double gamma,
+                             double dt, double* lam, double* dx)
+{
+    double F[9], C[2], g[24];
+    compute_F(p[0], p[1], p[2], p[3], Q, F);
+    C[0] = dev_eval(F);
+    dev_grad(F, Q, C[0], g);
+    C[1] = hyd_eval(F, gamma);
+    hyd_grad(F, Q, g + 12);
+
+    const double at[2] = { aD / (dt * dt), aH / (dt * dt) };
+    /* A(ci, cj) accumulated from alpha_tilde on the diagonal, vertex by vertex */
+    double A00 = at[0], A01 = 0, A11 = at[1];
+    for (int i = 0; i < 4; i++)
+        A00 += w[i] * (g[3 * i] * g[3 * i] + g[3 * i + 1] * g[3 * i + 1] + g[3 * i + 2] * g[3 * i + 2]);
+    for (int i = 0; i < 4; i++)
+        A01 += w[i] * (g[3 * i] * g[12 + 3 * i] + g[3 * i + 1] * g[12 + 3 * i + 1] + g[3 * i + 2] * g[12 + 3 * i + 2]);
+    for (int i = 0; i < 4; i++)
+        A11 += w[i] * (g[12 + 3 * i] * g[12 + 3 * i] + g[12 + 3 * i + 1] * g[12 + 3 * i + 1] + g[12 + 3 * i + 2] * g[12 + 3 * i + 2]);
+
+    const double b0 = -C[0] - at[0] * lam[0];
+    const double b1 = -C[1] - at[1] * lam[1];
+    const double det = A00 * A11 - A01 * A01;
+    const double d0 = (b0 * A11 - b1 * A01) / det;
+    const double d1 = (b1 * A00 - b0 * A01) / det;
+    lam[0] += d0;
+    lam[1] += d1;
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 3; k++)
+            dx[3 * i + k] = w[i] * (g[3 * i + k] * d0 + g[12 + 3 * i + k] * d1);
+}
+
+ORC_API void orc_project_tet(const double* x12, const double* w4, const double* Q,
+                             double aD, double aH, double gamma, double dt,
+                             double* lam2, double* dx12)
+{
+    const double* p[4] = { x12, x12 + 3, x12 + 6, x12 + 9 };
+    project_combined(p, w4, Q, aD, aH, gamma, dt, lam2, dx12);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a18 static SDF collision, single-constraint projector with alpha = 0:     */
+/*     include/solver/xpbd_projector/ConstraintProjector.hpp:111-172 and     */
+/*     src/solver/constraint/StaticDeformableCollisionConstraint.cpp:20-55   */
+/*     Returns 0 and leaves dx untouched when the inequality is inactive.    */
+/* ------------------------------------------------------------------------ */
+static int project_contact(const double* const p[3], const double* w,
+                           const double* cp, const double* cn, const double* bary,
+                           double* lam, double* dx9)
+{
+    double a[3];
+    for (int k = 0; k < 3; k++)
+        a[k] = bary[0] * p[0][k] + bary[1] * p[1][k] + bary[2] * p[2][k];
+    const double C = cn[0] * (a[0] - cp[0]) + cn[1] * (a[1] - cp[1]) + cn[2] * (a[2] - cp[2]);
+    if (C > 0) return 0;
+    double g[9];
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) g[3 * i + k] = bary[i] * cn[k];
+    double L = 0; /* alpha_tilde = 0 */
+    for (int i = 0; i < 3; i++)
+        L += w[i] * (g[3 * i] * g[3 * i] + g[3 * i + 1] * g[3 * i + 1] + g[3 * i + 2] * g[3 * i + 2]);
+    const double dl = (-C - 0.0 * (*lam)) / L;
+    *lam += dl;
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) dx9[3 * i + k] = w[i] * g[3 * i + k] * dl;
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a19 SDFs: src/geometry/SphereSDF.cpp:16-26, src/geometry/BoxSDF.cpp:19-84 */
+/*     Quaternions are (x, y, z, w) as in src/utils/GeometryUtils.cpp:92-102  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int kind;
+    double pos[3];
+    double quat[4];
+    double prm[3]; /* sphere: radius in prm[0]; box: full size */
+} orc_sdf;
+
+static void quat_rotate(const double* v, const double* q, double* out)
+{
+    /* 2 (u.v) u + (2 w^2 - 1) v + 2 w (u x v),  GeometryUtils.cpp:92-97 */
+    const double* u = q;
+    const double uv = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+    double c[3];
+    cross3(u, v, c);
+    const double s = 2 * q[3] * q[3] - 1;
+    for (int k = 0; k < 3; k++) out[k] = 2 * uv * u[k] + s * v[k] + 2 * q[3] * c[k];
+}
+
+static void quat_inverse(const double* q, double* qi)
+{
+    const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    qi[0] = -q[0] / n2; qi[1] = -q[1] / n2; qi[2] = -q[2] / n2; qi[3] = q[3] / n2;
+}
+
+static double sdf_eval(const orc_sdf* s, const double* x)
+{
+    if (s->kind == ORC_SDF_SPHERE) {
+        const double d0 = x[0] - s->pos[0], d1 = x[1] - s->pos[1], d2 = x[2] - s->pos[2];
+        return sqrt(d0 * d0 + d1 * d1 + d2 * d2) - s->prm[0];
+    }
+    double rel[3] = { x[0] - s->pos[0], x[1] - s->pos[1], x[2] - s->pos[2] }, qi[4], xb[3];
+    quat_inverse(s->quat, qi);
+    quat_rotate(rel, qi, xb);
+    double q[3], qc[3];
+    for (int k = 0; k < 3; k++) {
+        q[k] = fabs(xb[k]) - 0.5 * s->prm[k];
+        qc[k] = q[k] > 0.0 ? q[k] : 0.0;
+    }
+    double mq = q[0];
+    if (q[1] > mq) mq = q[1];
+    if (q[2] > mq) mq = q[2];
+    const double outside = sqrt(qc[0] * qc[0] + qc[1] * qc[1] + qc[2] * qc[2]);
+    const double inside = mq < 0.0 ? mq : 0.0;
+    return outside + inside;
+}
+
+static void sdf_grad(const orc_sdf* s, const double* x, double* g)
+{
+    if (s->kind == ORC_SDF_SPHERE) {
+        double d[3] = { x[0] - s->pos[0], x[1] - s->pos[1], x[2] - s->pos[2] };
+        const double n = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (int k = 0; k < 3; k++) g[k] = n > 0 ? d[k] / n : d[k];
+        return;
+    }
+    double rel[3] = { x[0] - s->pos[0], x[1] - s->pos[1], x[2] - s->pos[2] }, qi[4], xb[3];
+    quat_inverse(s->quat, qi);
+    quat_rotate(rel, qi, xb);
+    double q[3], qc[3], sg[3];
+    for (int k = 0; k < 3; k++) {
+        q[k] = fabs(xb[k]) - s->prm[k] / 2.0;
+        qc[k] = q[k] > 0.0 ? q[k] : 0.0;
+        sg[k] = xb[k] > 0 ? 1.0 : -1.0;
+    }
+    double mq = q[0];
+    if (q[1] > mq) mq = q[1];
+    if (q[2] > mq) mq = q[2];
+    const double min_dist = mq < 0.0 ? mq : 0.0;
+    double go[3] = { sg[0] * qc[0], sg[1] * qc[1], sg[2] * qc[2] };
+    const double n = sqrt(go[0] * go[0] + go[1] * go[1] + go[2] * go[2]);
+    if (n > 0) for (int k = 0; k < 3; k++) go[k] /= n;  /* zero vector stays zero */
+    double gb[3];
+    for (int k = 0; k < 3; k++) gb[k] = go[k] + sg[k] * (q[k] == min_dist ? 1.0 : 0.0);
+    quat_rotate(gb, s->quat, g);
+}
+
+ORC_API void orc_sdf_eval(int kind, const double* pos, const double* quat, const double* prm,
+                          const double* x, double* d, double* grad)
+{
+    orc_sdf s;
+    s.kind = kind;
+    memcpy(s.pos, pos, sizeof(s.pos));
+    if (quat) memcpy(s.quat, quat, sizeof(s.quat));
+    else { s.quat[0] = s.quat[1] = s.quat[2] = 0; s.quat[3] = 1; }
+    memcpy(s.prm, prm, sizeof(s.prm));
+    *d = sdf_eval(&s, x);
+    if (grad) sdf_grad(&s, x, grad);
+}
+
+/* Frank-Wolfe closest point on a triangle: src/collision/CollisionScene.cpp:253-283 */
+static void frank_wolfe(const orc_sdf* s, const double* p1, const double* p2,
+                        const double* p3, double* x)
+{
+    const double d1 = sdf_eval(s, p1), d2 = sdf_eval(s, p2), d3 = sdf_eval(s, p3);
+    const double* start;
+    if (d1 <= d2 && d1 <= d3) start = p1;
+    else if (d2 <= d1 && d2 <= d3) start = p2;
+    else start = p3;
+    memcpy(x, start, 3 * sizeof(double));
+    for (int i = 0; i < 32; i++) {
+        const double alpha = 2.0 / (i + 3);
+        double g[3];
+        sdf_grad(s, x, g);
+        const double s1 = p1[0] * g[0] + p1[1] * g[1] + p1[2] * g[2];
+        const double s2 = p2[0] * g[0] + p2[1] * g[1] + p2[2] * g[2];
+        const double s3 = p3[0] * g[0] + p3[1] * g[1] + p3[2] * g[2];
+        const double* sv;
+        if (s1 < s2 && s1 < s3) sv = p1;
+        else if (s2 < s1 && s2 < s3) sv = p2;
+        else sv = p3;
+        for (int k = 0; k < 3; k++) x[k] = x[k] + alpha * (sv[k] - x[k]);
+    }
+}
+
+/* src/utils/GeometryUtils.cpp:6-24 */
+static void barycentric(const double* p, const double* a, const double* b,
+                        const double* c, double* uvw)
+{
+    double v0[3], v1[3], v2[3];
+    for (int k = 0; k < 3; k++) { v0[k] = b[k] - a[k]; v1[k] = c[k] - a[k]; v2[k] = p[k] - a[k]; }
+#define DOT(u, v) (u[0] * v[0] + u[1] * v[1] + u[2] * v[2])
+    const double d00 = DOT(v0, v0), d01 = DOT(v0, v1), d11 = DOT(v1, v1);
+    const double d20 = DOT(v2, v0), d21 = DOT(v2, v1);
+#undef DOT
+    const double denom = d00 * d11 - d01 * d01;
+    const double v = (d11 * d20 - d01 * d21) / denom;
+    const double w = (d00 * d21 - d01 * d20) / denom;
+    uvw[0] = 1 - v - w;
+    uvw[1] = v;
+    uvw[2] = w;
+}
+
+ORC_API void orc_frank_wolfe(int kind, const double* pos, const double* quat, const double* prm,
+                             const double* p1, const double* p2, const double* p3, double* x)
+{
+    orc_sdf s;
+    s.kind = kind;
+    memcpy(s.pos, pos, sizeof(s.pos));
+    if (quat) memcpy(s.quat, quat, sizeof(s.quat));
+    else { s.quat[0] = s.quat[1] = s.quat[2] = 0; s.quat[3] = 1; }
+    memcpy(s.prm, prm, sizeof(s.prm));
+    frank_wolfe(&s, p1, p2, p3, x);
+}
+
+ORC_API void orc_barycentric(const double* p, const double* a, const double* b,
+                             const double* c, double* uvw)
+{
+    barycentric(p, a, b, c, uvw);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Whole-object simulation state (XPBDMeshObject_ + XPBDSolver + Simulation) */
+/* ------------------------------------------------------------------------ */
+typedef struct orc_sim {
+    int nv, nt, nf, mode, iters;
+    double dt, g;
+    double *x, *xp, *v, *m, *w;
+    unsigned char* pin;
+    int* tets;
+    double *Q, *vol, *aD, *aH, *gam, *lam;
+    int* order;            /* GS visiting order of tets (identity = reference order) */
+    int* faces;            /* surface triangles, 3 per face */
+    orc_sdf sdf[1];
+    int nsdf;
+    /* contacts: static collision constraints in insertion order */
+    int nc, ccap;
+    int* c_face;
+    double *c_p, *c_n, *c_bary, *c_lam;
+    int* face_rank;        /* optional visiting rank of faces for contact projection */
+    int* corder;           /* contact visiting order (indices into the contact list) */
+    double last_res;       /* ||r||_2 at the end of the last solve (before pin reset) */
+    double* jbuf;          /* Jacobi coordinate-update buffer (values) */
+    int* jidx;             /* Jacobi coordinate-update buffer (coordinate index) */
+} orc_sim;
+
+/* XPBDMeshObject.cpp:102-160 (setup) + 277-331 (masses) + 352-417 (constraints) */
+ORC_API orc_sim* orc_create(int nv, int nt, const double* x0, const double* v0,
+                            const int* tets, const double* E, const double* nu,
+                            const double* rho, const unsigned char* pinned,
+                            double dt, double g, int iters, int mode)
+{
+    orc_sim* s = (orc_sim*)calloc(1, sizeof(orc_sim));
+    s->nv = nv; s->nt = nt; s->dt = dt; s->g = g; s->iters = iters; s->mode = mode;
+    s->x = (double*)malloc(sizeof(double) * 3 * nv);
+    s->xp = (double*)malloc(sizeof(double) * 3 * nv);
+    s->v = (double*)calloc(3 * (size_t)nv, sizeof(double));
+    s->m = (double*)calloc(nv, sizeof(double));
+    s->w = (double*)malloc(sizeof(double) * nv);
+    s->pin = (unsigned char*)calloc(nv, 1);
+    s->tets = (int*)malloc(sizeof(int) * 4 * (size_t)nt);
+    s->Q = (double*)malloc(sizeof(double) * 9 * (size_t)nt);
+    s->vol = (double*)malloc(sizeof(double) * nt);
+    s->aD = (double*)malloc(sizeof(double) * nt);
+    s->aH = (double*)malloc(sizeof(double) * nt);
+    s->gam = (double*)malloc(sizeof(double) * nt);
+    s->lam = (double*)calloc(2 * (size_t)nt, sizeof(double));
+    s->order = (int*)malloc(sizeof(int) * nt);
+    s->jbuf = (double*)malloc(sizeof(double) * 12 * (size_t)nt);
+    s->jidx = (int*)malloc(sizeof(int) * 12 * (size_t)nt);
+    memcpy(s->x, x0, sizeof(double) * 3 * nv);
+    memcpy(s->xp, x0, sizeof(double) * 3 * nv);
+    if (v0) memcpy(s->v, v0, sizeof(double) * 3 * nv);
+    if (pinned) memcpy(s->pin, pinned, nv);
+    memcpy(s->tets, tets, sizeof(int) * 4 * (size_t)nt);
+    int bad = 0;
+    for (int e = 0; e < nt; e++) {
+        const int* t = tets + 4 * e;
+        if (orc_rest_tet(x0 + 3 * t[0], x0 + 3 * t[1], x0 + 3 * t[2], x0 + 3 * t[3],
+                         s->Q + 9 * e, s->vol + e)) bad = 1;
+        s->order[e] = e;
+    }
+    /* lumped masses, element order: XPBDMeshObject.cpp:288-313 */
+    for (int e = 0; e < nt; e++) {
+        const double em = s->vol[e] * rho[e];
+        for (int j = 0; j < 4; j++) s->m[tets[4 * e + j]] += em / 4.0;
+    }
+    for (int i = 0; i < nv; i++) s->w[i] = 1.0 / s->m[i];  /* PositionReference.hpp:30-33 */
+    /* DeviatoricConstraint.cpp:13, HydrostaticConstraint.cpp:15-16 */
+    for (int e = 0; e < nt; e++) {
+        double mu, lam;
+        orc_lame(E[e], nu[e], &mu, &lam);
+        s->aD[e] = 1 / (mu * s->vol[e]);
+        s->aH[e] = 1 / (lam * s->vol[e]);
+        s->gam[e] = mu / lam;
+    }
+    (void)bad;
+    return s;
+}
+
+ORC_API void orc_destroy(orc_sim* s)
+{
+    if (!s) return;
+    free(s->x); free(s->xp); free(s->v); free(s->m); free(s->w); free(s->pin);
+    free(s->tets); free(s->Q); free(s->vol); free(s->aD); free(s->aH); free(s->gam);
+    free(s->lam); free(s->order); free(s->faces); free(s->jbuf); free(s->jidx);
+    free(s->c_face); free(s->c_p); free(s->c_n); free(s->c_bary); free(s->c_lam);
+    free(s->face_rank); free(s->corder);
+    free(s);
+}
+
+ORC_API void orc_set_order(orc_sim* s, const int* order)
+{
+    memcpy(s->order, order, sizeof(int) * s->nt);
+}
+
+ORC_API void orc_set_mode(orc_sim* s, int mode, int iters)
+{
+    s->mode = mode;
+    s->iters = iters;
+}
+
+ORC_API void orc_set_faces(orc_sim* s, int nf, const int* faces)
+{
+    free(s->faces);
+    s->nf = nf;
+    s->faces = (int*)malloc(sizeof(int) * 3 * (size_t)(nf > 0 ? nf : 1));
+    memcpy(s->faces, faces, sizeof(int) * 3 * (size_t)nf);
+}
+
+ORC_API void orc_set_sdf(orc_sim* s, int kind, const double* pos, const double* quat,
+                         const double* prm)
+{
+    s->nsdf = kind == ORC_SDF_NONE ? 0 : 1;
+    s->sdf[0].kind = kind;
+    memcpy(s->sdf[0].pos, pos, 3 * sizeof(double));
+    if (quat) memcpy(s->sdf[0].quat, quat, 4 * sizeof(double));
+    else { s->sdf[0].quat[0] = s->sdf[0].quat[1] = s->sdf[0].quat[2] = 0; s->sdf[0].quat[3] = 1; }
+    memcpy(s->sdf[0].prm, prm, 3 * sizeof(double));
+}
+
+ORC_API void orc_set_state(orc_sim* s, const double* x, const double* v)
+{
+    if (x) memcpy(s->x, x, sizeof(double) * 3 * s->nv);
+    if (v) memcpy(s->v, v, sizeof(double) * 3 * s->nv);
+}
+
+static void add_contact(orc_sim* s, int face, const double* p, const double* n, const double* bary)
+{
+    if (s->nc == s->ccap) {
+        s->ccap = s->ccap ? 2 * s->ccap : 64;
+        s->c_face = (int*)realloc(s->c_face, sizeof(int) * s->ccap);
+        s->c_p = (double*)realloc(s->c_p, sizeof(double) * 3 * s->ccap);
+        s->c_n = (double*)realloc(s->c_n, sizeof(double) * 3 * s->ccap);
+        s->c_bary = (double*)realloc(s->c_bary, sizeof(double) * 3 * s->ccap);
+        s->c_lam = (double*)realloc(s->c_lam, sizeof(double) * s->ccap);
+    }
+    const int c = s->nc++;
+    s->c_face[c] = face;
+    memcpy(s->c_p + 3 * c, p, 3 * sizeof(double));
+    memcpy(s->c_n + 3 * c, n, 3 * sizeof(double));
+    memcpy(s->c_bary + 3 * c, bary, 3 * sizeof(double));
+    s->c_lam[c] = 0;
+}
+
+static void order_contacts(orc_sim* s);
+
+/* Face-vs-SDF detection, face order: src/collision/CollisionScene.cpp:171-214.
+ * Replaces the current contact set (XPBDMeshObject.cpp:233-251 clears first). */
+ORC_API int orc_detect(orc_sim* s)
+{
+    s->nc = 0;
+    if (!s->nsdf) return 0;
+    const orc_sdf* sd = &s->sdf[0];
+    for (int f = 0; f < s->nf; f++) {
+        const double* p1 = s->x + 3 * s->faces[3 * f];
+        const double* p2 = s->x + 3 * s->faces[3 * f + 1];
+        const double* p3 = s->x + 3 * s->faces[3 * f + 2];
+        double e12 = 0, e13 = 0, e23 = 0, cen[3];
+        for (int k = 0; k < 3; k++) {
+            e12 += (p2[k] - p1[k]) * (p2[k] - p1[k]);
+            e13 += (p3[k] - p1[k]) * (p3[k] - p1[k]);
+            e23 += (p3[k] - p2[k]) * (p3[k] - p2[k]);
+            cen[k] = (p1[k] + p2[k] + p3[k]) / 3;
+        }
+        double me = e12;
+        if (e13 > me) me = e13;
+        if (e23 > me) me = e23;
+        const double cd = sdf_eval(sd, cen);
+        if (cd * cd > me) continue;
+        double x[3];
+        frank_wolfe(sd, p1, p2, p3, x);
+        const double dist = sdf_eval(sd, x);
+        if (dist <= 1e-4) {
+            double uvw[3], gr[3], sx[3];
+            barycentric(x, p1, p2, p3, uvw);
+            sdf_grad(sd, x, gr);
+            for (int k = 0; k < 3; k++) sx[k] = x[k] - gr[k] * dist;
+            add_contact(s, f, sx, gr, uvw);
+        }
+    }
+    order_contacts(s);
+    return s->nc;
+}
+
+/* Contacts are projected in insertion (face) order, as the reference does
+ * (VariadicVectorContainer insertion order).  A face rank, when set, permutes
+ * that order: contacts are visited by increasing rank of their face, which is
+ * how the colour-ordered device solver sequences them. */
+static const int* g_sort_rank;
+static const int* g_sort_face;
+static int cmp_contact(const void* a, const void* b)
+{
+    const int ra = g_sort_rank[g_sort_face[*(const int*)a]];
+    const int rb = g_sort_rank[g_sort_face[*(const int*)b]];
+    return (ra > rb) - (ra < rb);
+}
+
+static void order_contacts(orc_sim* s)
+{
+    free(s->corder);
+    s->corder = (int*)malloc(sizeof(int) * (size_t)(s->nc > 0 ? s->nc : 1));
+    for (int c = 0; c < s->nc; c++) s->corder[c] = c;
+    if (s->face_rank && s->nc > 1) {
+        g_sort_rank = s->face_rank;
+        g_sort_face = s->c_face;
+        qsort(s->corder, s->nc, sizeof(int), cmp_contact);
+    }
+}
+
+ORC_API void orc_set_contact_order(orc_sim* s, const int* face_rank)
+{
+    free(s->face_rank);
+    s->face_rank = NULL;
+    if (face_rank) {
+        s->face_rank = (int*)malloc(sizeof(int) * (size_t)(s->nf > 0 ? s->nf : 1));
+        memcpy(s->face_rank, face_rank, sizeof(int) * s->nf);
+    }
+    order_contacts(s);
+}
+
+ORC_API int orc_num_contacts(const orc_sim* s) { return s->nc; }
+ORC_API double orc_last_step_residual(const orc_sim* s) { return s->last_res; }
+
+ORC_API void orc_get_contacts(const orc_sim* s, int* face, double* p, double* n, double* bary)
+{
+    if (face) memcpy(face, s->c_face, sizeof(int) * s->nc);
+    if (p) memcpy(p, s->c_p, sizeof(double) * 3 * s->nc);
+    if (n) memcpy(n, s->c_n, sizeof(double) * 3 * s->nc);
+    if (bary) memcpy(bary, s->c_bary, sizeof(double) * 3 * s->nc);
+}
+
+/* One solver iteration.
+ * GS:     include/solver/xpbd_solver/XPBDGaussSeidelSolver.hpp:29-80
+ *         (all combined projectors in s->order, then contacts in insertion order,
+ *          each projector's 12 (or 9) updates applied immediately in coordinate order)
+ * Jacobi: include/solver/xpbd_solver/XPBDJacobiSolver.hpp:79-114
+ *         (all projectors read the same positions, updates buffered in projector
+ *          order and summed into x afterwards, no relaxation) */
+static void iterate(orc_sim* s)
+{
+    const int nt = s->nt;
+    if (s->mode == ORC_GS) {
+        for (int oi = 0; oi < nt; oi++) {
+            const int e = s->order[oi];
+            const int* t = s->tets + 4 * e;
+            const double* p[4] = { s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3] };
+            const double w[4] = { s->w[t[0]], s->w[t[1]], s->w[t[2]], s->w[t[3]] };
+            double dx[12];
+            project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->dt, s->lam + 2 * e, dx);
+            for (int i = 0; i < 4; i++)
+                for (int k = 0; k < 3; k++) s->x[3 * t[i] + k] += dx[3 * i + k];
+        }
+        for (int ci = 0; ci < s->nc; ci++) {
+            const int c = s->corder[ci];
+            const int* f = s->faces + 3 * s->c_face[c];
+            const double* p[3] = { s->x + 3 * f[0], s->x + 3 * f[1], s->x + 3 * f[2] };
+            const double w[3] = { s->w[f[0]], s->w[f[1]], s->w[f[2]] };
+            double dx[9];
+            if (project_contact(p, w, s->c_p + 3 * c, s->c_n + 3 * c, s->c_bary + 3 * c, s->c_lam + c, dx))
+                for (int i = 0; i < 3; i++)
+                    for (int k = 0; k < 3; k++) s->x[3 * f[i] + k] += dx[3 * i + k];
+        }
+        return;
+    }
+    /* Jacobi */
+    size_t nu = 0;
+    double* cbuf = NULL;
+    int* cidx = NULL;
+    for (int e = 0; e < nt; e++) {
+        const int* t = s->tets + 4 * e;
+        const double* p[4] = { s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3] };
+        const double w[4] = { s->w[t[0]], s->w[t[1]], s->w[t[2]], s->w[t[3]] };
+        project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->dt, s->lam + 2 * e, s->jbuf + nu);
+        for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 3; k++) s->jidx[nu + 3 * i + k] = 3 * t[i] + k;
+        nu += 12;
+    }
+    int ncu = 0;
+    if (s->nc) {
+        cbuf = (double*)malloc(sizeof(double) * 9 * s->nc);
+        cidx = (int*)malloc(sizeof(int) * 9 * s->nc);
+        for (int c = 0; c < s->nc; c++) {
+            const int* f = s->faces + 3 * s->c_face[c];
+            const double* p[3] = { s->x + 3 * f[0], s->x + 3 * f[1], s->x + 3 * f[2] };
+            const double w[3] = { s->w[f[0]], s->w[f[1]], s->w[f[2]] };
+            if (project_contact(p, w, s->c_p + 3 * c, s->c_n + 3 * c, s->c_bary + 3 * c, s->c_lam + c, cbuf + ncu)) {
+                for (int i = 0; i < 3; i++)
+                    for (int k = 0; k < 3; k++) cidx[ncu + 3 * i + k] = 3 * f[i] + k;
+                ncu += 9;
+            }
+        }
+    }
+    for (size_t u = 0; u < nu; u++) s->x[s->jidx[u]] += s->jbuf[u];
+    for (int u = 0; u < ncu; u++) s->x[cidx[u]] += cbuf[u];
+    free(cbuf);
+    free(cidx);
+}
+
+/* The per-object part of one time step, with contacts optionally re-detected
+ * first (Simulation.cpp:202-241 runs detection before update()):
+ *   XPBDMeshObject.cpp:426-444  update: x_prev = x, predict, project, pin reset
+ *   XPBDMeshObject.cpp:459-469  predict (2nd order): x += dt v ; z -= g dt^2
+ *   XPBDSolver.hpp:66-97        solve: lambda = 0, iters x iterate
+ *   XPBDMeshObject.cpp:497-503  pinned vertices reset to x_prev
+ *   XPBDMeshObject.cpp:507-535  v = (x - x_prev) / dt                       */
+ORC_API double orc_residual(const orc_sim* s, double* vec);
+
+ORC_API void orc_step(orc_sim* s, int detect)
+{
+    if (detect) orc_detect(s);
+    const int nv = s->nv;
+    const double dt = s->dt;
+    memcpy(s->xp, s->x, sizeof(double) * 3 * nv);
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
+    const double dz = -s->g * dt * dt;
+    for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
+    memset(s->lam, 0, sizeof(double) * 2 * (size_t)s->nt);
+    for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
+    for (int it = 0; it < s->iters; it++) iterate(s);
+    /* EVERY_SUBSTEP residual is taken here, before the pin reset (XPBDSolver.hpp:91-96) */
+    s->last_res = orc_residual(s, NULL);
+    for (int i = 0; i < nv; i++)
+        if (s->pin[i])
+            for (int k = 0; k < 3; k++) s->x[3 * i + k] = s->xp[3 * i + k];
+    for (int i = 0; i < 3 * nv; i++) s->v[i] = (s->x[i] - s->xp[i]) / dt;
+}
+
+/* Prediction half of a step only (used to build learned-warm-start features). */
+ORC_API void orc_predict(orc_sim* s)
+{
+    const double dt = s->dt;
+    memcpy(s->xp, s->x, sizeof(double) * 3 * s->nv);
+    for (int i = 0; i < 3 * s->nv; i++) s->x[i] += dt * s->v[i];
+    const double dz = -s->g * dt * dt;
+    for (int i = 0; i < s->nv; i++) s->x[3 * i + 2] += dz;
+}
+
+ORC_API void orc_get_x(const orc_sim* s, double* out) { memcpy(out, s->x, sizeof(double) * 3 * s->nv); }
+ORC_API void orc_get_xprev(const orc_sim* s, double* out) { memcpy(out, s->xp, sizeof(double) * 3 * s->nv); }
+ORC_API void orc_get_v(const orc_sim* s, double* out) { memcpy(out, s->v, sizeof(double) * 3 * s->nv); }
+ORC_API void orc_get_lambda(const orc_sim* s, double* out) { memcpy(out, s->lam, sizeof(double) * 2 * s->nt); }
+ORC_API void orc_get_masses(const orc_sim* s, double* out) { memcpy(out, s->m, sizeof(double) * s->nv); }
+ORC_API void orc_get_rest(const orc_sim* s, double* Q, double* vol)
+{
+    if (Q) memcpy(Q, s->Q, sizeof(double) * 9 * s->nt);
+    if (vol) memcpy(vol, s->vol, sizeof(double) * s->nt);
+}
+ORC_API void orc_get_compliance(const orc_sim* s, double* aD, double* aH, double* gam)
+{
+    if (aD) memcpy(aD, s->aD, sizeof(double) * s->nt);
+    if (aH) memcpy(aH, s->aH, sizeof(double) * s->nt);
+    if (gam) memcpy(gam, s->gam, sizeof(double) * s->nt);
+}
+
+/* a11 constraint residual, projector order (Dev, Hyd per tet):
+ *     include/solver/xpbd_solver/XPBDSolver.hpp:327-391.  Inequality (contact)
+ *     constraints are skipped there, so only elastic entries appear.
+ *     Returns ||r||_2; vec (2*nt, optional) receives r. */
+ORC_API double orc_residual(const orc_sim* s, double* vec)
+{
+    const double dt2 = s->dt * s->dt;
+    double acc = 0;
+    for (int e = 0; e < s->nt; e++) {
+        const int* t = s->tets + 4 * e;
+        double F[9];
+        compute_F(s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3], s->Q + 9 * e, F);
+        const double C1 = dev_eval(F), C2 = hyd_eval(F, s->gam[e]);
+        const double r1 = C1 + s->aD[e] / dt2 * s->lam[2 * e];
+        const double r2 = C2 + s->aH[e] / dt2 * s->lam[2 * e + 1];
+        if (vec) { vec[2 * e] = r1; vec[2 * e + 1] = r2; }
+        acc += r1 * r1 + r2 * r2;
+    }
+    return sqrt(acc);
+}
+
+/* a12 total strain energy: XPBDMeshObject.cpp:537-549 (all Dev, then all Hyd,
+ *     sum of C^2 / alpha, no 1/2 factor). */
+ORC_API double orc_energy(const orc_sim* s)
+{
+    double E = 0;
+    for (int pass = 0; pass < 2; pass++)
+        for (int e = 0; e < s->nt; e++) {
+            const int* t = s->tets + 4 * e;
+            double F[9];
+            compute_F(s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3], s->Q + 9 * e, F);
+            if (pass == 0) { const double C = dev_eval(F); E += C * C / s->aD[e]; }
+            else { const double C = hyd_eval(F, s->gam[e]); E += C * C / s->aH[e]; }
+        }
+    return E;
+}

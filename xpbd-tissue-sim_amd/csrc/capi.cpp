@@ -1,0 +1,1009 @@
+// C-ABI of the XPBD solver: set-up (FP64 on the host), device buffers, the
+// graph-captured time step, diagnostics and the batched path.
+#include "xpbd_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "capi_internal.h"
+#include "kernels.h"
+#include "mesh_tools.h"
+
+using namespace xpbd;
+
+namespace {
+thread_local std::string g_create_error;
+}
+
+namespace xpbd {
+
+int set_err(xpbd_handle* h, int code, const std::string& msg)
+{
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int hip_err(xpbd_handle* h, hipError_t e, const char* what)
+{
+    return set_err(h, XPBD_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+static int dalloc(xpbd_handle* h, T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+    if (e != hipSuccess) return set_err(h, XPBD_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    h->allocs.push_back(*p);
+    return XPBD_OK;
+}
+
+static void release_graphs(xpbd_handle* h)
+{
+    for (int i = 0; i < 2; i++) {
+        if (h->graph_exec[i]) hipGraphExecDestroy(h->graph_exec[i]);
+        h->graph_exec[i] = nullptr;
+    }
+}
+
+void invalidate_graphs(xpbd_handle* h) { release_graphs(h); }
+
+TetArrays tet_arrays(xpbd_handle* h)
+{
+    TetArrays ta;
+    ta.tet = h->d_tet;
+    ta.Q = h->d_Q;
+    ta.qstride = h->nt;
+    ta.at = h->d_at;
+    ta.gamma = h->d_gamma;
+    ta.lam = h->d_lam;
+    return ta;
+}
+
+bool contacts_enabled(const xpbd_handle* h) { return h->nf > 0 && h->sdf_active; }
+
+}  // namespace xpbd
+
+// ---------------------------------------------------------------------------
+// set-up
+// ---------------------------------------------------------------------------
+static bool finite_all(const double* p, size_t n)
+{
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+static int validate_desc(const xpbd_mesh_desc* m)
+{
+    if (!m || m->num_vertices <= 0 || m->num_tets <= 0 || !m->x0 || !m->tets || !m->E || !m->nu || !m->rho)
+        return set_err(nullptr, XPBD_E_INVALID_ARG, "mesh: missing arrays or non-positive sizes");
+    if (m->num_faces < 0 || (m->num_faces > 0 && !m->faces))
+        return set_err(nullptr, XPBD_E_INVALID_ARG, "mesh: bad face list");
+    const size_t V = m->num_vertices, T = m->num_tets;
+    for (size_t i = 0; i < 4 * T; i++)
+        if (m->tets[i] < 0 || m->tets[i] >= m->num_vertices)
+            return set_err(nullptr, XPBD_E_INVALID_ARG, "mesh: tet vertex index out of range");
+    for (size_t i = 0; i < 3 * (size_t)m->num_faces; i++)
+        if (m->faces[i] < 0 || m->faces[i] >= m->num_vertices)
+            return set_err(nullptr, XPBD_E_INVALID_ARG, "mesh: face vertex index out of range");
+    if (!finite_all(m->x0, 3 * V) || (m->v0 && !finite_all(m->v0, 3 * V)) || !finite_all(m->E, T) ||
+        !finite_all(m->nu, T) || !finite_all(m->rho, T))
+        return set_err(nullptr, XPBD_E_NONFINITE, "mesh: non-finite value in an input array");
+    for (size_t e = 0; e < T; e++)
+        if (!(m->E[e] > 0) || !(m->nu[e] > -1.0 && m->nu[e] < 0.5) || !(m->rho[e] > 0))
+            return set_err(nullptr, XPBD_E_INVALID_ARG, "mesh: material out of range (E>0, -1<nu<0.5, rho>0)");
+    return XPBD_OK;
+}
+
+// Rest state of one element in FP64: Q = Dm^-1 (adjugate / det), V = |det|/6.
+// include/solver/constraint/ElementConstraint.hpp:36-50
+static bool rest_tet(const double* X0, const double* X1, const double* X2, const double* X3, double* Q,
+                     double* vol)
+{
+    double D[9];
+    for (int k = 0; k < 3; k++) {
+        D[k] = X0[k] - X3[k];
+        D[3 + k] = X1[k] - X3[k];
+        D[6 + k] = X2[k] - X3[k];
+    }
+    auto d = [&](int r, int c) { return D[3 * c + r]; };
+    const double c00 = d(1, 1) * d(2, 2) - d(1, 2) * d(2, 1);
+    const double c01 = d(1, 2) * d(2, 0) - d(1, 0) * d(2, 2);
+    const double c02 = d(1, 0) * d(2, 1) - d(1, 1) * d(2, 0);
+    const double det = d(0, 0) * c00 + d(0, 1) * c01 + d(0, 2) * c02;
+    if (!(std::fabs(det) > 0) || !std::isfinite(det)) return false;
+    const double inv[9] = {
+        c00 / det, c01 / det, c02 / det,                                        // column 0
+        (d(0, 2) * d(2, 1) - d(0, 1) * d(2, 2)) / det,                          // (0,1)
+        (d(0, 0) * d(2, 2) - d(0, 2) * d(2, 0)) / det,                          // (1,1)
+        (d(0, 1) * d(2, 0) - d(0, 0) * d(2, 1)) / det,                          // (2,1)
+        (d(0, 1) * d(1, 2) - d(0, 2) * d(1, 1)) / det,                          // (0,2)
+        (d(0, 2) * d(1, 0) - d(0, 0) * d(1, 2)) / det,                          // (1,2)
+        (d(0, 0) * d(1, 1) - d(0, 1) * d(1, 0)) / det,                          // (2,2)
+    };
+    std::memcpy(Q, inv, sizeof(inv));
+    *vol = std::fabs(det / 6);
+    return true;
+}
+
+static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, xpbd_handle** out)
+{
+    g_create_error.clear();
+    if (!out) return set_err(nullptr, XPBD_E_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (n <= 0 || !meshes || !P) return set_err(nullptr, XPBD_E_INVALID_ARG, "need n > 0 meshes and params");
+    if (!(P->dt > 0) || !std::isfinite(P->dt) || !std::isfinite(P->g) || P->iters < 0 ||
+        (P->solver != XPBD_SOLVER_JACOBI && P->solver != XPBD_SOLVER_GS_COLORED) || P->collision_every < 0)
+        return set_err(nullptr, XPBD_E_INVALID_ARG, "params: need dt > 0, finite g, iters >= 0, known solver");
+    for (int32_t i = 0; i < n; i++) {
+        int rc = validate_desc(&meshes[i]);
+        if (rc) return rc;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return set_err(nullptr, XPBD_E_HIP, "no HIP device");
+
+    xpbd_handle* h = new (std::nothrow) xpbd_handle();
+    if (!h) return set_err(nullptr, XPBD_E_NOMEM, "handle allocation failed");
+    h->device = dev;
+    h->params = *P;
+    h->ninst = n;
+    // ---- union of instances
+    h->v_off.resize(n + 1);
+    h->t_off.resize(n + 1);
+    h->f_off.resize(n + 1);
+    h->v_off[0] = h->t_off[0] = h->f_off[0] = 0;
+    for (int32_t i = 0; i < n; i++) {
+        h->v_off[i + 1] = h->v_off[i] + meshes[i].num_vertices;
+        h->t_off[i + 1] = h->t_off[i] + meshes[i].num_tets;
+        h->f_off[i + 1] = h->f_off[i] + meshes[i].num_faces;
+    }
+    if (h->v_off[n] > INT32_MAX / 4 || h->t_off[n] > INT32_MAX / 12) {
+        delete h;
+        return set_err(nullptr, XPBD_E_INVALID_ARG, "mesh too large for 32-bit indexing");
+    }
+    const int nv = h->nv = (int)h->v_off[n];
+    const int nt = h->nt = (int)h->t_off[n];
+    const int nf = h->nf = (int)h->f_off[n];
+    h->rest.resize(3 * (size_t)nv);
+    std::vector<double> vel(3 * (size_t)nv, 0.0);
+    std::vector<uint8_t> pin(nv, 0);
+    std::vector<int32_t> tets(4 * (size_t)nt), faces(4 * (size_t)(nf > 0 ? nf : 0));
+    std::vector<double> E(nt), nu(nt), rho(nt);
+    for (int32_t i = 0; i < n; i++) {
+        const xpbd_mesh_desc& m = meshes[i];
+        const size_t vo = h->v_off[i], to = h->t_off[i], fo = h->f_off[i];
+        std::memcpy(&h->rest[3 * vo], m.x0, sizeof(double) * 3 * m.num_vertices);
+        if (m.v0) std::memcpy(&vel[3 * vo], m.v0, sizeof(double) * 3 * m.num_vertices);
+        if (m.pinned) std::memcpy(&pin[vo], m.pinned, m.num_vertices);
+        for (size_t k = 0; k < 4 * (size_t)m.num_tets; k++) tets[4 * to + k] = m.tets[k] + (int32_t)vo;
+        std::memcpy(&E[to], m.E, sizeof(double) * m.num_tets);
+        std::memcpy(&nu[to], m.nu, sizeof(double) * m.num_tets);
+        std::memcpy(&rho[to], m.rho, sizeof(double) * m.num_tets);
+        for (int32_t f = 0; f < m.num_faces; f++) {
+            faces[4 * (fo + f) + 0] = m.faces[3 * f + 0] + (int32_t)vo;
+            faces[4 * (fo + f) + 1] = m.faces[3 * f + 1] + (int32_t)vo;
+            faces[4 * (fo + f) + 2] = m.faces[3 * f + 2] + (int32_t)vo;
+            faces[4 * (fo + f) + 3] = i;
+        }
+    }
+    h->pinned = pin;
+    // ---- FP64 rest-state precompute
+    std::vector<double> Q(9 * (size_t)nt), vol(nt);
+    for (int e = 0; e < nt; e++) {
+        const int32_t* t = &tets[4 * (size_t)e];
+        if (!rest_tet(&h->rest[3 * (size_t)t[0]], &h->rest[3 * (size_t)t[1]], &h->rest[3 * (size_t)t[2]],
+                      &h->rest[3 * (size_t)t[3]], &Q[9 * (size_t)e], &vol[e])) {
+            delete h;
+            return set_err(nullptr, XPBD_E_DEGENERATE, "element " + std::to_string(e) + " has zero volume");
+        }
+    }
+    // lumped masses in element order (XPBDMeshObject.cpp:288-313)
+    h->mass.assign(nv, 0.0);
+    for (int e = 0; e < nt; e++) {
+        const double em = vol[e] * rho[e];
+        for (int j = 0; j < 4; j++) h->mass[tets[4 * (size_t)e + j]] += em / 4.0;
+    }
+    // compliances (DeviatoricConstraint.cpp:13, HydrostaticConstraint.cpp:15-16,
+    // ElasticMaterial.hpp:31-32) and alpha tilde = alpha / dt^2
+    const double dt2 = P->dt * P->dt;
+    std::vector<float> atD(nt), atH(nt), gam(nt);
+    for (int e = 0; e < nt; e++) {
+        const double mu = E[e] / (2 * (1 + nu[e]));
+        const double lm = (E[e] * nu[e]) / ((1 + nu[e]) * (1 - 2 * nu[e]));
+        atD[e] = (float)(1 / (mu * vol[e]) / dt2);
+        atH[e] = (float)(1 / (lm * vol[e]) / dt2);
+        gam[e] = (float)(mu / lm);
+    }
+    // ---- solver order of elements
+    h->order.resize(nt);
+    if (P->solver == XPBD_SOLVER_GS_COLORED) {
+        std::vector<int32_t> col;
+        int nc = greedy_color(nv, nt, 4, tets.data(), col);
+        if (nc <= 0) {
+            delete h;
+            return set_err(nullptr, XPBD_E_INVALID_ARG, "tet colouring failed");
+        }
+        std::vector<int32_t> cnt(nc + 1, 0);
+        for (int e = 0; e < nt; e++) cnt[col[e] + 1]++;
+        for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
+        h->color_off.assign(cnt.begin(), cnt.end());
+        std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+        for (int e = 0; e < nt; e++) h->order[pos[col[e]]++] = e;
+        h->ncolors = nc;
+    } else {
+        std::iota(h->order.begin(), h->order.end(), 0);
+        h->color_off = { 0, nt };
+        h->ncolors = 1;
+    }
+    // ---- contact face order: colour-sorted so contacts of one colour share no vertex
+    h->fperm.resize(nf);
+    h->fcol_off = { 0, nf };
+    if (nf > 0) {
+        std::vector<int32_t> f3(3 * (size_t)nf), col;
+        for (int f = 0; f < nf; f++)
+            for (int j = 0; j < 3; j++) f3[3 * (size_t)f + j] = faces[4 * (size_t)f + j];
+        int nc = greedy_color(nv, nf, 3, f3.data(), col);
+        if (nc <= 0) {
+            delete h;
+            return set_err(nullptr, XPBD_E_INVALID_ARG, "face colouring failed");
+        }
+        std::vector<int32_t> cnt(nc + 1, 0);
+        for (int f = 0; f < nf; f++) cnt[col[f] + 1]++;
+        for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
+        h->fcol_off.assign(cnt.begin(), cnt.end());
+        std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+        for (int f = 0; f < nf; f++) h->fperm[pos[col[f]]++] = f;
+    }
+    h->nfcolors = (int)h->fcol_off.size() - 1;
+
+    // ---- device buffers
+    hipError_t he = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        int rc = hip_err(nullptr, he, "hipStreamCreate");
+        delete h;
+        return rc;
+    }
+    int rc = 0;
+    rc |= dalloc(h, &h->d_pos, 2 * (size_t)nv);
+    rc |= dalloc(h, &h->d_lo, nv);
+    rc |= dalloc(h, &h->d_vel, nv);
+    rc |= dalloc(h, &h->d_step_res, 1);
+    rc |= dalloc(h, &h->d_acc, 4 * (size_t)nv);
+    rc |= dalloc(h, &h->d_rest, 3 * (size_t)nv);
+    rc |= dalloc(h, &h->d_tet, nt);
+    rc |= dalloc(h, &h->d_Q, 9 * (size_t)nt);
+    rc |= dalloc(h, &h->d_at, nt);
+    rc |= dalloc(h, &h->d_gamma, nt);
+    rc |= dalloc(h, &h->d_lam, nt);
+    h->red_blocks = std::max(1, std::min(1024, (nt + 255) / 256));
+    rc |= dalloc(h, &h->d_part, h->red_blocks);
+    rc |= dalloc(h, &h->d_scalar, 2);
+    rc |= dalloc(h, &h->d_step, 1);
+    rc |= dalloc(h, &h->d_sdf, n);
+    rc |= dalloc(h, &h->d_faces, nf);
+    rc |= dalloc(h, &h->d_fcol_off, h->fcol_off.size());
+    rc |= dalloc(h, &h->d_flag, nf);
+    rc |= dalloc(h, &h->d_cpos, (size_t)nf + 1);
+    rc |= dalloc(h, &h->d_clist, nf);
+    rc |= dalloc(h, &h->d_cp, 3 * (size_t)nf);
+    rc |= dalloc(h, &h->d_cn, 3 * (size_t)nf);
+    rc |= dalloc(h, &h->d_cb, 3 * (size_t)nf);
+    rc |= dalloc(h, &h->d_clam, nf);
+    if (rc) {
+        std::string msg = h->err;
+        xpbd_destroy(h);
+        return set_err(nullptr, XPBD_E_NOMEM, msg);
+    }
+    // host staging in solver order
+    std::vector<float4> hu(2 * (size_t)nv), hv(nv);
+    for (int i = 0; i < nv; i++) {
+        const double w = h->mass[i] > 0 ? 1.0 / h->mass[i] : 0.0;  // PositionReference.hpp:30-33
+        hu[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, (float)w);
+        hu[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        hv[i] = make_float4((float)vel[3 * (size_t)i], (float)vel[3 * (size_t)i + 1], (float)vel[3 * (size_t)i + 2],
+                            pin[i] ? 1.f : 0.f);
+    }
+    std::vector<int4> ht(nt);
+    std::vector<float> hQ(9 * (size_t)nt);
+    std::vector<float2> hat(nt);
+    std::vector<float> hg(nt);
+    for (int i = 0; i < nt; i++) {
+        const int e = h->order[i];
+        ht[i] = make_int4(tets[4 * (size_t)e], tets[4 * (size_t)e + 1], tets[4 * (size_t)e + 2], tets[4 * (size_t)e + 3]);
+        for (int k = 0; k < 9; k++) hQ[(size_t)k * nt + i] = (float)Q[9 * (size_t)e + k];
+        hat[i] = make_float2(atD[e], atH[e]);
+        hg[i] = gam[e];
+    }
+    std::vector<int4> hf(nf);
+    for (int j = 0; j < nf; j++) {
+        const int f = h->fperm[j];
+        hf[j] = make_int4(faces[4 * (size_t)f], faces[4 * (size_t)f + 1], faces[4 * (size_t)f + 2], faces[4 * (size_t)f + 3]);
+    }
+    h->sdf_host.assign(n, SdfDev{});
+    for (auto& s : h->sdf_host) s.quat[3] = 1.0;
+#define UP(dst, src, bytes)                                                      \
+    do {                                                                         \
+        hipError_t e_ = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);      \
+        if (e_ != hipSuccess) {                                                  \
+            int r_ = hip_err(nullptr, e_, "hipMemcpy");                          \
+            xpbd_destroy(h);                                                     \
+            return r_;                                                           \
+        }                                                                        \
+    } while (0)
+    UP(h->d_pos, hu.data(), sizeof(float4) * 2 * (size_t)nv);
+    UP(h->d_vel, hv.data(), sizeof(float4) * nv);
+    UP(h->d_rest, h->rest.data(), sizeof(double) * 3 * nv);
+    UP(h->d_tet, ht.data(), sizeof(int4) * nt);
+    UP(h->d_Q, hQ.data(), sizeof(float) * 9 * (size_t)nt);
+    UP(h->d_at, hat.data(), sizeof(float2) * nt);
+    UP(h->d_gamma, hg.data(), sizeof(float) * nt);
+    UP(h->d_sdf, h->sdf_host.data(), sizeof(SdfDev) * n);
+    if (nf > 0) UP(h->d_faces, hf.data(), sizeof(int4) * nf);
+    UP(h->d_fcol_off, h->fcol_off.data(), sizeof(int32_t) * h->fcol_off.size());
+#undef UP
+    // zero-initialised device state (fill kernels, then wait)
+    launch_fill_zero(h->stream, h->d_acc, sizeof(float) * 4 * (size_t)nv);
+    launch_fill_zero(h->stream, h->d_lo, sizeof(float4) * (size_t)nv);
+    launch_fill_zero(h->stream, h->d_step_res, sizeof(double));
+    launch_fill_zero(h->stream, h->d_lam, sizeof(float2) * nt);
+    launch_fill_zero(h->stream, h->d_step, sizeof(unsigned long long));
+    launch_fill_zero(h->stream, h->d_cpos, sizeof(int32_t) * ((size_t)nf + 1));
+    launch_fill_zero(h->stream, h->d_clam, sizeof(double) * (nf > 0 ? nf : 1));
+    he = hipStreamSynchronize(h->stream);
+    if (he != hipSuccess) {
+        int r = hip_err(nullptr, he, "initialisation");
+        xpbd_destroy(h);
+        return r;
+    }
+    *out = h;
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_create(const xpbd_mesh_desc* mesh, const xpbd_params* params, xpbd_handle** out)
+{
+    return build(1, mesh, params, out);
+}
+
+extern "C" int xpbd_batch_create(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* params,
+                                 xpbd_batch** out)
+{
+    return build(n, meshes, params, out);
+}
+
+extern "C" int xpbd_destroy(xpbd_handle* h)
+{
+    if (!h) return XPBD_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    release_graphs(h);
+    for (auto& ev : h->events) {
+        hipEventDestroy(ev.first);
+        hipEventDestroy(ev.second);
+    }
+    for (void* p : h->allocs) hipFree(p);
+    destroy_learned(h);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return XPBD_OK;
+}
+
+extern "C" const char* xpbd_last_error(const xpbd_handle* h)
+{
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+// ---------------------------------------------------------------------------
+// time step
+// ---------------------------------------------------------------------------
+namespace xpbd {
+
+// Enqueue the solver iterations of one step (no predict / finish).
+int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
+{
+    hipStream_t s = h->stream;
+    TetArrays ta = tet_arrays(h);
+    const bool con = contacts_enabled(h);
+    hipError_t e = hipSuccess;
+    for (int it = 0; it < iters; it++) {
+        const bool first = it == 0;
+        if (h->params.solver == XPBD_SOLVER_GS_COLORED) {
+            for (int c = 0; c < h->ncolors; c++) {
+                if (timed) timed_begin(h);
+                e = launch_gs_color(s, ta, h->d_pos, h->color_off[c], h->color_off[c + 1], first);
+                if (e != hipSuccess) return hip_err(h, e, "gs_color launch");
+                if (timed) timed_end(h);
+            }
+            if (con) {
+                e = launch_contacts_gs(s, h->d_faces, h->d_fcol_off, h->nfcolors, h->d_cpos, h->d_clist, h->d_rest,
+                                       h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
+                if (e != hipSuccess) return hip_err(h, e, "contacts_gs launch");
+            }
+        } else {
+            if (timed) timed_begin(h);
+            e = launch_jacobi_project(s, ta, h->d_pos, h->d_acc, h->nt, first);
+            if (e != hipSuccess) return hip_err(h, e, "jacobi_project launch");
+            if (timed) timed_end(h);
+            if (con) {
+                e = launch_contacts_jacobi(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
+                                           h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
+                if (e != hipSuccess) return hip_err(h, e, "contacts_jacobi launch");
+            }
+            e = launch_jacobi_apply(s, h->d_pos, h->d_acc, h->nv);
+            if (e != hipSuccess) return hip_err(h, e, "jacobi_apply launch");
+        }
+        if (timed) h->timed_iterations++;
+    }
+    return XPBD_OK;
+}
+
+int enqueue_detect(xpbd_handle* h)
+{
+    hipError_t e = launch_detect(h->stream, h->d_faces, h->nf, h->d_rest, h->d_pos, h->d_lo, h->d_sdf, h->d_step,
+                                 h->params.dt, h->d_flag, h->d_cp, h->d_cn, h->d_cb);
+    if (e != hipSuccess) return hip_err(h, e, "detect launch");
+    e = launch_compact(h->stream, h->d_flag, h->nf, h->d_cpos, h->d_clist);
+    if (e != hipSuccess) return hip_err(h, e, "compact launch");
+    return XPBD_OK;
+}
+
+int enqueue_predict(xpbd_handle* h)
+{
+    const double dt = h->params.dt;
+    hipError_t e = launch_predict(h->stream, h->d_pos, h->d_vel, h->nv, (float)dt, (float)(-h->params.g * dt * dt));
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "predict launch");
+}
+
+int enqueue_finish(xpbd_handle* h)
+{
+    hipError_t e = launch_finish(h->stream, h->d_pos, h->d_lo, h->d_vel, h->nv, (float)h->params.dt, h->d_step);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "finish launch");
+}
+
+// Whole step: [detect] predict iterations finish
+static int enqueue_step(xpbd_handle* h, bool detect, bool timed)
+{
+    int rc;
+    if (detect && contacts_enabled(h) && (rc = enqueue_detect(h))) return rc;
+    if ((rc = enqueue_predict(h))) return rc;
+    if ((rc = enqueue_iterations(h, h->params.iters, timed))) return rc;
+    if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+    return enqueue_finish(h);
+}
+
+void timed_begin(xpbd_handle* h)
+{
+    if (h->ev_used == h->events.size()) {
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        h->events.push_back({ a, b });
+    }
+    hipEventRecord(h->events[h->ev_used].first, h->stream);
+}
+
+void timed_end(xpbd_handle* h)
+{
+    hipEventRecord(h->events[h->ev_used].second, h->stream);
+    h->ev_used++;
+}
+
+int collect_timing(xpbd_handle* h)
+{
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_err(h, e, "step");
+    for (size_t i = 0; i < h->ev_used; i++) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, h->events[i].first, h->events[i].second);
+        h->timed_ms += ms;
+    }
+    h->timed_launches += (int64_t)h->ev_used;
+    h->ev_used = 0;
+    return XPBD_OK;
+}
+
+bool detect_this_step(const xpbd_handle* h)
+{
+    const int k = h->params.collision_every;
+    return contacts_enabled(h) && k > 0 && (h->host_step % k) == 0;
+}
+
+int run_steps(xpbd_handle* h, int32_t nsteps)
+{
+    for (int32_t n = 0; n < nsteps; n++) {
+        const bool det = detect_this_step(h);
+        int rc;
+        if (h->timing) {
+            if ((rc = enqueue_step(h, det, true))) return rc;
+            if ((rc = collect_timing(h))) return rc;
+        } else {
+            const int gi = det ? 1 : 0;
+            if (!h->graph_exec[gi]) {
+                hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+                if (e != hipSuccess) return hip_err(h, e, "hipStreamBeginCapture");
+                rc = enqueue_step(h, det, false);
+                hipGraph_t g = nullptr;
+                e = hipStreamEndCapture(h->stream, &g);
+                if (rc) {
+                    if (g) hipGraphDestroy(g);
+                    return rc;
+                }
+                if (e != hipSuccess) return hip_err(h, e, "hipStreamEndCapture");
+                e = hipGraphInstantiate(&h->graph_exec[gi], g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (e != hipSuccess) return hip_err(h, e, "hipGraphInstantiate");
+            }
+            hipError_t e = hipGraphLaunch(h->graph_exec[gi], h->stream);
+            if (e != hipSuccess) return hip_err(h, e, "hipGraphLaunch");
+        }
+        h->host_step++;
+    }
+    return XPBD_OK;
+}
+
+}  // namespace xpbd
+
+extern "C" int xpbd_step(xpbd_handle* h, int32_t nsteps)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (nsteps < 0) return set_err(h, XPBD_E_INVALID_ARG, "nsteps < 0");
+    return run_steps(h, nsteps);
+}
+
+extern "C" int xpbd_set_iterations(xpbd_handle* h, int32_t iters)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (iters < 0) return set_err(h, XPBD_E_INVALID_ARG, "iters < 0");
+    if (iters != h->params.iters) {
+        h->params.iters = iters;
+        invalidate_graphs(h);
+    }
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_synchronize(xpbd_handle* h)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "hipStreamSynchronize");
+}
+
+// ---------------------------------------------------------------------------
+// state access
+// ---------------------------------------------------------------------------
+namespace xpbd {
+int download_f4(xpbd_handle* h, std::vector<float4>& hu, const float4* src, size_t n)
+{
+    hu.resize(n);
+    hipError_t e = hipMemcpyAsync(hu.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "download");
+}
+}  // namespace xpbd
+
+// position = rest + base + low part + in-step motion (zero between steps)
+static int positions_range(xpbd_handle* h, int64_t v0, int64_t v1, double* out)
+{
+    std::vector<float4> hp, hl;
+    int rc = download_f4(h, hp, h->d_pos, 2 * (size_t)h->nv);
+    if (!rc) rc = download_f4(h, hl, h->d_lo, h->nv);
+    if (rc) return rc;
+    for (int64_t i = v0; i < v1; i++) {
+        const float4 b = hp[2 * i], d = hp[2 * i + 1], l = hl[i];
+        out[3 * (i - v0) + 0] = h->rest[3 * i + 0] + ((double)b.x + (double)l.x + (double)d.x);
+        out[3 * (i - v0) + 1] = h->rest[3 * i + 1] + ((double)b.y + (double)l.y + (double)d.y);
+        out[3 * (i - v0) + 2] = h->rest[3 * i + 2] + ((double)b.z + (double)l.z + (double)d.z);
+    }
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_get_positions(xpbd_handle* h, double* out)
+{
+    if (!h || !out) return h ? set_err(h, XPBD_E_INVALID_ARG, "out is NULL") : XPBD_E_INVALID_ARG;
+    return positions_range(h, 0, h->nv, out);
+}
+
+extern "C" int xpbd_get_positions_f32(xpbd_handle* h, float* out)
+{
+    if (!h || !out) return h ? set_err(h, XPBD_E_INVALID_ARG, "out is NULL") : XPBD_E_INVALID_ARG;
+    std::vector<double> tmp(3 * (size_t)h->nv);
+    int rc = positions_range(h, 0, h->nv, tmp.data());
+    if (rc) return rc;
+    for (size_t i = 0; i < tmp.size(); i++) out[i] = (float)tmp[i];
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_batch_get_positions(xpbd_batch* b, int32_t instance, double* out)
+{
+    if (!b || !out) return b ? set_err(b, XPBD_E_INVALID_ARG, "out is NULL") : XPBD_E_INVALID_ARG;
+    if (instance < 0 || instance >= b->ninst) return set_err(b, XPBD_E_INVALID_ARG, "instance out of range");
+    return positions_range(b, b->v_off[instance], b->v_off[instance + 1], out);
+}
+
+extern "C" int xpbd_batch_num_instances(xpbd_batch* b, int32_t* out)
+{
+    if (!b || !out) return XPBD_E_INVALID_ARG;
+    *out = b->ninst;
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_get_velocities(xpbd_handle* h, double* out)
+{
+    if (!h || !out) return h ? set_err(h, XPBD_E_INVALID_ARG, "out is NULL") : XPBD_E_INVALID_ARG;
+    std::vector<float4> hv;
+    int rc = download_f4(h, hv, h->d_vel, h->nv);
+    if (rc) return rc;
+    for (int i = 0; i < h->nv; i++) {
+        out[3 * (size_t)i] = hv[i].x;
+        out[3 * (size_t)i + 1] = hv[i].y;
+        out[3 * (size_t)i + 2] = hv[i].z;
+    }
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_set_state(xpbd_handle* h, const double* x, const double* v)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    const size_t n3 = 3 * (size_t)h->nv;
+    if ((x && !finite_all(x, n3)) || (v && !finite_all(v, n3)))
+        return set_err(h, XPBD_E_NONFINITE, "set_state: non-finite input");
+    int rc;
+    if (x) {
+        // base = nearest float to the displacement, low part = remainder; no in-step motion
+        std::vector<float4> hp, hl(h->nv);
+        if ((rc = download_f4(h, hp, h->d_pos, 2 * (size_t)h->nv))) return rc;
+        for (int i = 0; i < h->nv; i++) {
+            float hi[3], lw[3];
+            for (int k = 0; k < 3; k++) {
+                const double u = x[3 * (size_t)i + k] - h->rest[3 * (size_t)i + k];
+                hi[k] = (float)u;
+                lw[k] = (float)(u - (double)hi[k]);
+            }
+            hp[2 * (size_t)i] = make_float4(hi[0], hi[1], hi[2], hp[2 * (size_t)i].w);
+            hp[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            hl[i] = make_float4(lw[0], lw[1], lw[2], 0.f);
+        }
+        hipError_t e = hipMemcpy(h->d_pos, hp.data(), sizeof(float4) * hp.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->d_lo, hl.data(), sizeof(float4) * hl.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_err(h, e, "set_state");
+    }
+    if (v) {
+        std::vector<float4> hv;
+        if ((rc = download_f4(h, hv, h->d_vel, h->nv))) return rc;
+        for (int i = 0; i < h->nv; i++) {
+            hv[i].x = (float)v[3 * (size_t)i];
+            hv[i].y = (float)v[3 * (size_t)i + 1];
+            hv[i].z = (float)v[3 * (size_t)i + 2];
+        }
+        hipError_t e = hipMemcpy(h->d_vel, hv.data(), sizeof(float4) * h->nv, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_err(h, e, "set_state");
+    }
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_get_masses(xpbd_handle* h, double* out)
+{
+    if (!h || !out) return XPBD_E_INVALID_ARG;
+    std::memcpy(out, h->mass.data(), sizeof(double) * h->nv);
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_get_lambdas(xpbd_handle* h, double* out)
+{
+    if (!h || !out) return XPBD_E_INVALID_ARG;
+    std::vector<float2> hl(h->nt);
+    hipError_t e = hipMemcpyAsync(hl.data(), h->d_lam, sizeof(float2) * h->nt, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_err(h, e, "get_lambdas");
+    for (int i = 0; i < h->nt; i++) {
+        const int t = h->order[i];
+        out[2 * (size_t)t] = hl[i].x;
+        out[2 * (size_t)t + 1] = hl[i].y;
+    }
+    return XPBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------
+namespace xpbd {
+int device_residual(xpbd_handle* h, double* norm, double* vec_solver_order_dev)
+{
+    hipError_t e = launch_residual(h->stream, tet_arrays(h), h->d_pos, h->d_lo, true, h->nt, h->d_part,
+                                   h->red_blocks, vec_solver_order_dev);
+    if (e == hipSuccess) e = launch_sum_partials(h->stream, h->d_part, h->red_blocks, h->d_scalar);
+    double s = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&s, h->d_scalar, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_err(h, e, "residual");
+    *norm = std::sqrt(s);
+    return XPBD_OK;
+}
+// ||r||^2 at the end of the solve, before the pin reset (where the reference's
+// EVERY_SUBSTEP policy evaluates it, XPBDSolver.hpp:91-96), into d_step_res.
+int enqueue_step_residual(xpbd_handle* h)
+{
+    hipError_t e = launch_residual(h->stream, tet_arrays(h), h->d_pos, h->d_lo, true, h->nt, h->d_part,
+                                   h->red_blocks, nullptr);
+    if (e == hipSuccess) e = launch_sum_partials(h->stream, h->d_part, h->red_blocks, h->d_step_res);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "step residual");
+}
+}  // namespace xpbd
+
+extern "C" int xpbd_set_residual_policy(xpbd_handle* h, int32_t policy)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (policy != 0 && policy != 1) return set_err(h, XPBD_E_INVALID_ARG, "policy must be 0 or 1");
+    if (policy != h->residual_policy) {
+        h->residual_policy = policy;
+        invalidate_graphs(h);
+    }
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_last_step_residual(xpbd_handle* h, double* out)
+{
+    if (!h || !out) return XPBD_E_INVALID_ARG;
+    if (!h->residual_policy) return set_err(h, XPBD_E_STATE, "residual policy is off");
+    double s = 0;
+    hipError_t e = hipMemcpyAsync(&s, h->d_step_res, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_err(h, e, "last_step_residual");
+    *out = std::sqrt(s);
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_constraint_residual(xpbd_handle* h, double* norm_out, double* vec_out)
+{
+    if (!h || !norm_out) return h ? set_err(h, XPBD_E_INVALID_ARG, "norm_out is NULL") : XPBD_E_INVALID_ARG;
+    double* dvec = nullptr;
+    if (vec_out) {
+        if (!h->d_resvec) {
+            if (hipMalloc(reinterpret_cast<void**>(&h->d_resvec), sizeof(double) * 2 * (size_t)h->nt) != hipSuccess)
+                return set_err(h, XPBD_E_NOMEM, "residual vector allocation failed");
+            h->allocs.push_back(h->d_resvec);
+        }
+        dvec = h->d_resvec;
+    }
+    int rc = device_residual(h, norm_out, dvec);
+    if (rc || !vec_out) return rc;
+    std::vector<double> tmp(2 * (size_t)h->nt);
+    hipError_t e = hipMemcpy(tmp.data(), dvec, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_err(h, e, "residual vector");
+    for (int i = 0; i < h->nt; i++) {
+        const int t = h->order[i];
+        vec_out[2 * (size_t)t] = tmp[2 * (size_t)i];
+        vec_out[2 * (size_t)t + 1] = tmp[2 * (size_t)i + 1];
+    }
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_strain_energy(xpbd_handle* h, double* out)
+{
+    if (!h || !out) return XPBD_E_INVALID_ARG;
+    const double dt2 = h->params.dt * h->params.dt;
+    hipError_t e = launch_energy(h->stream, tet_arrays(h), h->d_pos, h->d_lo, h->nt, dt2, h->d_part, h->red_blocks);
+    if (e == hipSuccess) e = launch_sum_partials(h->stream, h->d_part, h->red_blocks, h->d_scalar);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, h->d_scalar, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "energy");
+}
+
+extern "C" int xpbd_num_colors(xpbd_handle* h, int32_t* out)
+{
+    if (!h || !out) return XPBD_E_INVALID_ARG;
+    *out = h->ncolors;
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_get_tet_order(xpbd_handle* h, int32_t* out)
+{
+    if (!h || !out) return XPBD_E_INVALID_ARG;
+    std::memcpy(out, h->order.data(), sizeof(int32_t) * h->nt);
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_get_face_rank(xpbd_handle* h, int32_t* out)
+{
+    if (!h || !out) return XPBD_E_INVALID_ARG;
+    for (int j = 0; j < h->nf; j++) out[h->fperm[j]] = j;
+    return XPBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// contacts
+// ---------------------------------------------------------------------------
+static int put_sdf(xpbd_handle* h, int32_t inst, int32_t kind, const double* pose7, const double* params3)
+{
+    if (kind != XPBD_SDF_NONE && kind != XPBD_SDF_SPHERE && kind != XPBD_SDF_BOX)
+        return set_err(h, XPBD_E_INVALID_ARG, "unknown SDF kind");
+    if (kind != XPBD_SDF_NONE && (!pose7 || !params3)) return set_err(h, XPBD_E_INVALID_ARG, "pose/params NULL");
+    if (kind != XPBD_SDF_NONE && (!finite_all(pose7, 7) || !finite_all(params3, 3)))
+        return set_err(h, XPBD_E_NONFINITE, "SDF pose/params not finite");
+    SdfDev& s = h->sdf_host[inst];
+    const double vel[3] = { s.vel[0], s.vel[1], s.vel[2] };
+    s = SdfDev{};
+    s.kind = kind;
+    for (int k = 0; k < 3; k++) s.vel[k] = vel[k];
+    if (kind != XPBD_SDF_NONE) {
+        for (int k = 0; k < 3; k++) s.pos[k] = pose7[k];
+        for (int k = 0; k < 4; k++) s.quat[k] = pose7[3 + k];
+        for (int k = 0; k < 3; k++) s.prm[k] = params3[k];
+        if (kind == XPBD_SDF_SPHERE && !(params3[0] > 0)) return set_err(h, XPBD_E_INVALID_ARG, "radius <= 0");
+        const double qn = s.quat[0] * s.quat[0] + s.quat[1] * s.quat[1] + s.quat[2] * s.quat[2] + s.quat[3] * s.quat[3];
+        if (!(qn > 0)) return set_err(h, XPBD_E_INVALID_ARG, "zero quaternion");
+    } else {
+        s.quat[3] = 1.0;
+    }
+    hipError_t e = hipMemcpyAsync(h->d_sdf + inst, &s, sizeof(SdfDev), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_fill_zero(h->stream, h->d_step, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_err(h, e, "set_sdf");
+    bool any = false;
+    for (auto& q : h->sdf_host) any = any || q.kind != XPBD_SDF_NONE;
+    if (any != h->sdf_active) {
+        h->sdf_active = any;
+        invalidate_graphs(h);
+    }
+    h->host_step = 0;
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_set_sdf(xpbd_handle* h, int32_t kind, const double* pose7, const double* params3)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    for (int i = 0; i < h->ninst; i++) {
+        int rc = put_sdf(h, i, kind, pose7, params3);
+        if (rc) return rc;
+    }
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_batch_set_sdf(xpbd_batch* b, int32_t instance, int32_t kind, const double* pose7,
+                                  const double* params3)
+{
+    if (!b) return XPBD_E_INVALID_ARG;
+    if (instance < 0 || instance >= b->ninst) return set_err(b, XPBD_E_INVALID_ARG, "instance out of range");
+    return put_sdf(b, instance, kind, pose7, params3);
+}
+
+extern "C" int xpbd_set_sdf_velocity(xpbd_handle* h, const double* vel3)
+{
+    if (!h || !vel3) return XPBD_E_INVALID_ARG;
+    if (!finite_all(vel3, 3)) return set_err(h, XPBD_E_NONFINITE, "velocity not finite");
+    for (int i = 0; i < h->ninst; i++) {
+        for (int k = 0; k < 3; k++) h->sdf_host[i].vel[k] = vel3[k];
+    }
+    hipError_t e = hipMemcpyAsync(h->d_sdf, h->sdf_host.data(), sizeof(SdfDev) * h->ninst, hipMemcpyHostToDevice,
+                                  h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "set_sdf_velocity");
+}
+
+extern "C" int xpbd_detect_contacts(xpbd_handle* h)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (!contacts_enabled(h)) return XPBD_OK;
+    int rc = enqueue_detect(h);
+    if (rc) return rc;
+    return xpbd_synchronize(h);
+}
+
+extern "C" int xpbd_get_contacts(xpbd_handle* h, int32_t cap, int32_t* count, int32_t* face, double* p, double* n,
+                                 double* bary)
+{
+    if (!h || !count) return XPBD_E_INVALID_ARG;
+    *count = 0;
+    if (h->nf == 0) return XPBD_OK;
+    std::vector<int32_t> cpos(h->nf + 1), clist(h->nf);
+    std::vector<double> cp(3 * (size_t)h->nf), cn(3 * (size_t)h->nf), cb(3 * (size_t)h->nf);
+    hipError_t e = hipMemcpyAsync(cpos.data(), h->d_cpos, sizeof(int32_t) * (h->nf + 1), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(clist.data(), h->d_clist, sizeof(int32_t) * h->nf, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(cp.data(), h->d_cp, sizeof(double) * cp.size(), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(cn.data(), h->d_cn, sizeof(double) * cn.size(), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(cb.data(), h->d_cb, sizeof(double) * cb.size(), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_err(h, e, "get_contacts");
+    const int total = cpos[h->nf];
+    *count = total;
+    for (int k = 0; k < total && k < cap; k++) {
+        const int j = clist[k];
+        if (face) face[k] = h->fperm[j];
+        for (int c = 0; c < 3; c++) {
+            if (p) p[3 * k + c] = cp[3 * (size_t)j + c];
+            if (n) n[3 * k + c] = cn[3 * (size_t)j + c];
+            if (bary) bary[3 * k + c] = cb[3 * (size_t)j + c];
+        }
+    }
+    return XPBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// timing
+// ---------------------------------------------------------------------------
+extern "C" int xpbd_set_kernel_timing(xpbd_handle* h, int32_t on)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    h->timing = on != 0;
+    h->timed_ms = 0;
+    h->timed_iterations = 0;
+    h->timed_launches = 0;
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_kernel_stats(xpbd_handle* h, double* proj_ms, int64_t* iterations, int64_t* launches)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (proj_ms) *proj_ms = h->timed_ms;
+    if (iterations) *iterations = h->timed_iterations;
+    if (launches) *launches = h->timed_launches;
+    return XPBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-side mesh utilities
+// ---------------------------------------------------------------------------
+extern "C" int xpbd_mesh_grid(int32_t nx, int32_t ny, int32_t nz, double hx, double hy, double hz,
+                              const double* origin3, double* x, int32_t* tets)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !(hx > 0) || !(hy > 0) || !(hz > 0) || !x || !tets)
+        return XPBD_E_INVALID_ARG;
+    if ((int64_t)nx * ny * nz * 24 > INT32_MAX) return XPBD_E_INVALID_ARG;
+    const double o[3] = { origin3 ? origin3[0] : 0.0, origin3 ? origin3[1] : 0.0, origin3 ? origin3[2] : 0.0 };
+    std::vector<double> vx;
+    std::vector<int32_t> vt;
+    grid_mesh(nx, ny, nz, hx, hy, hz, o, vx, vt);
+    std::memcpy(x, vx.data(), sizeof(double) * vx.size());
+    std::memcpy(tets, vt.data(), sizeof(int32_t) * vt.size());
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_mesh_surface(int32_t num_vertices, int32_t num_tets, const int32_t* tets, int32_t* num_faces,
+                                 int32_t* faces)
+{
+    if (num_vertices <= 0 || num_tets <= 0 || !tets || !num_faces || !faces) return XPBD_E_INVALID_ARG;
+    std::vector<int32_t> f;
+    if (surface_faces(num_vertices, num_tets, tets, f) < 0) return XPBD_E_INVALID_ARG;
+    *num_faces = (int32_t)(f.size() / 3);
+    std::memcpy(faces, f.data(), sizeof(int32_t) * f.size());
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_color_items(int32_t num_vertices, int32_t num_items, int32_t verts_per_item,
+                                const int32_t* items, int32_t* colors, int32_t* num_colors)
+{
+    if (num_vertices <= 0 || num_items < 0 || verts_per_item <= 0 || !items || !colors || !num_colors)
+        return XPBD_E_INVALID_ARG;
+    std::vector<int32_t> c;
+    const int nc = greedy_color(num_vertices, num_items, verts_per_item, items, c);
+    if (nc < 0) return XPBD_E_INVALID_ARG;
+    std::memcpy(colors, c.data(), sizeof(int32_t) * num_items);
+    *num_colors = nc;
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_read_msh(const char* path, int32_t* num_vertices, int32_t* num_tets, int32_t* num_tris,
+                             double* x, int32_t* tets, int32_t* tris)
+{
+    if (!path || !num_vertices || !num_tets || !num_tris) return XPBD_E_INVALID_ARG;
+    std::vector<double> vx;
+    std::vector<int32_t> vt, vf;
+    std::string err;
+    if (read_msh41(path, vx, vt, vf, err)) return set_err(nullptr, XPBD_E_IO, err);
+    *num_vertices = (int32_t)(vx.size() / 3);
+    *num_tets = (int32_t)(vt.size() / 4);
+    *num_tris = (int32_t)(vf.size() / 3);
+    if (x) std::memcpy(x, vx.data(), sizeof(double) * vx.size());
+    if (tets) std::memcpy(tets, vt.data(), sizeof(int32_t) * vt.size());
+    if (tris) std::memcpy(tris, vf.data(), sizeof(int32_t) * vf.size());
+    return XPBD_OK;
+}

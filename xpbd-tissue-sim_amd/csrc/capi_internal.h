@@ -1,0 +1,103 @@
+// Internal state behind the C-ABI handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "kernels.h"
+#include "xpbd_hip.h"
+
+namespace xpbd {
+struct Learned;
+}
+
+struct xpbd_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    xpbd_params params{};
+    int ninst = 1;
+    int nv = 0, nt = 0, nf = 0;
+    std::vector<int64_t> v_off, t_off, f_off;  // per-instance offsets in the concatenated arrays
+    std::vector<double> rest;                  // 3*nv rest positions (FP64, host copy)
+    std::vector<double> mass;                  // nv lumped masses
+    std::vector<uint8_t> pinned;
+    std::vector<int32_t> order;                // solver position -> caller's tet index
+    std::vector<int32_t> color_off;            // colour c occupies [color_off[c], color_off[c+1])
+    int ncolors = 1;
+    std::vector<int32_t> fperm;                // solver face position -> caller's face index
+    std::vector<int32_t> fcol_off;
+    int nfcolors = 1;
+    std::vector<xpbd::SdfDev> sdf_host;
+    bool sdf_active = false;
+
+    // device state
+    float4* d_pos = nullptr;     // 2 per vertex: (base displacement, inverse mass), (in-step motion, 0)
+    float4* d_lo = nullptr;      // low-order part of the base displacement
+    float4* d_vel = nullptr;     // velocity xyz, pinned flag w
+    float* d_acc = nullptr;      // Jacobi accumulator (4 per vertex)
+    double* d_rest = nullptr;
+    int4* d_tet = nullptr;
+    float* d_Q = nullptr;
+    float2* d_at = nullptr;
+    float* d_gamma = nullptr;
+    float2* d_lam = nullptr;
+    double* d_part = nullptr;
+    int red_blocks = 1;
+    double* d_scalar = nullptr;
+    double* d_resvec = nullptr;
+    unsigned long long* d_step = nullptr;
+    xpbd::SdfDev* d_sdf = nullptr;
+    int4* d_faces = nullptr;
+    int32_t* d_fcol_off = nullptr;
+    int32_t* d_flag = nullptr;
+    int32_t* d_cpos = nullptr;
+    int32_t* d_clist = nullptr;
+    double* d_cp = nullptr;
+    double* d_cn = nullptr;
+    double* d_cb = nullptr;
+    double* d_clam = nullptr;
+    std::vector<void*> allocs;
+
+    // stepping
+    int64_t host_step = 0;
+    int residual_policy = 0;     // 1: record ||r|| at the end of every solve (reference EVERY_SUBSTEP)
+    double* d_step_res = nullptr;
+    hipGraphExec_t graph_exec[2] = { nullptr, nullptr };  // [0] plain step, [1] step with detection
+
+    // kernel timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    size_t ev_used = 0;
+    double timed_ms = 0;
+    int64_t timed_iterations = 0;
+    int64_t timed_launches = 0;
+
+    // learned warm start (owned by learned.hip)
+    xpbd::Learned* learned = nullptr;
+
+    std::string err;
+};
+
+namespace xpbd {
+int set_err(xpbd_handle* h, int code, const std::string& msg);
+int hip_err(xpbd_handle* h, hipError_t e, const char* what);
+void invalidate_graphs(xpbd_handle* h);
+TetArrays tet_arrays(xpbd_handle* h);
+bool contacts_enabled(const xpbd_handle* h);
+int enqueue_iterations(xpbd_handle* h, int iters, bool timed);
+int enqueue_detect(xpbd_handle* h);
+int enqueue_predict(xpbd_handle* h);
+int enqueue_finish(xpbd_handle* h);
+bool detect_this_step(const xpbd_handle* h);
+int run_steps(xpbd_handle* h, int32_t nsteps);
+void timed_begin(xpbd_handle* h);
+void timed_end(xpbd_handle* h);
+int collect_timing(xpbd_handle* h);
+int download_f4(xpbd_handle* h, std::vector<float4>& hu, const float4* src, size_t n);
+int device_residual(xpbd_handle* h, double* norm, double* vec_solver_order_dev);
+int enqueue_step_residual(xpbd_handle* h);
+void destroy_learned(xpbd_handle* h);
+}  // namespace xpbd

@@ -1,0 +1,64 @@
+// Launch wrappers for the solver kernels (implemented in solver_kernels.hip).
+// Every wrapper enqueues on the given stream and returns the launch status.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace xpbd {
+
+// Kinematic signed-distance field of one instance.  Pose at step n is
+// pos + vel * (n * dt), orientation quat (x, y, z, w).
+struct SdfDev {
+    int32_t kind;  // 0 none, 1 sphere, 2 box
+    int32_t pad;
+    double pos[3];
+    double vel[3];
+    double quat[4];
+    double prm[3];  // sphere: radius in prm[0]; box: edge lengths
+};
+
+// Structure-of-arrays element data in solver order.
+struct TetArrays {
+    const int4* tet;      // vertex ids
+    const float* Q;       // 9 planes, Q[k * qstride + e]
+    int qstride;
+    const float2* at;     // (alpha_D, alpha_H) / dt^2
+    const float* gamma;   // mu / lambda
+    float2* lam;          // (lambda_D, lambda_H)
+};
+
+// Vertex records: pos[2i] = (base displacement xyz, inverse mass),
+// pos[2i+1] = (in-step motion xyz, 0); lo[i] = low-order part of the base;
+// vel[i] = (velocity xyz, pinned flag).
+hipError_t launch_predict(hipStream_t s, float4* pos, const float4* vel, int nv, float dt, float dz);
+hipError_t launch_finish(hipStream_t s, float4* pos, float4* lo, float4* vel, int nv, float dt,
+                         unsigned long long* step_counter);
+hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int begin, int end, bool first);
+hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float4* pos, float* acc, int nt,
+                                 bool first);
+hipError_t launch_jacobi_apply(hipStream_t s, float4* pos, float* acc, int nv);
+hipError_t launch_fill_zero(hipStream_t s, void* p, size_t bytes);
+
+// FP64 residual r = C + alpha_tilde * lambda per element, block partial sums of
+// r^2 into part[nblocks]; vec (2*nt, optional) receives r in solver order.
+// with_delta: evaluate at base + in-step motion (end of a solve) instead of base.
+hipError_t launch_residual(hipStream_t s, const TetArrays& ta, const float4* pos, const float4* lo, bool with_delta,
+                           int nt, double* part, int nblocks, double* vec);
+// Strain energy sum C^2/alpha (alpha = alpha_tilde * dt^2), block partials.
+hipError_t launch_energy(hipStream_t s, const TetArrays& ta, const float4* pos, const float4* lo, int nt, double dt2,
+                         double* part, int nblocks);
+hipError_t launch_sum_partials(hipStream_t s, const double* part, int n, double* out);
+
+// Contacts.  faces: (a, b, c, instance) in colour-sorted order.
+hipError_t launch_detect(hipStream_t s, const int4* faces, int nf, const double* rest, const float4* pos,
+                         const float4* lo, const SdfDev* sdf, const unsigned long long* step, double dt, int* flag,
+                         double* cp, double* cn, double* cb);
+hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int* clist);
+hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fcol_off, int ncol, const int* cpos,
+                              const int* clist, const double* rest, float4* pos, const float4* lo, const double* cp,
+                              const double* cn, const double* cb, double* clam, bool first);
+hipError_t launch_contacts_jacobi(hipStream_t s, const int4* faces, int nf, const int* cpos, const int* clist,
+                                  const double* rest, const float4* pos, const float4* lo, float* acc,
+                                  const double* cp, const double* cn, const double* cb, double* clam, bool first);
+
+}  // namespace xpbd

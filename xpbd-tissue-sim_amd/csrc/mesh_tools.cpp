@@ -1,0 +1,194 @@
+#include "mesh_tools.h"
+
+#include <algorithm>
+#include <array>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <unordered_map>
+
+namespace xpbd {
+
+void grid_mesh(int nx, int ny, int nz, double hx, double hy, double hz,
+               const double origin[3], std::vector<double>& x, std::vector<int32_t>& tets)
+{
+    const int vx = nx + 1, vy = ny + 1, vz = nz + 1;
+    x.resize(3ull * vx * vy * vz);
+    for (int k = 0; k < vz; k++)
+        for (int j = 0; j < vy; j++)
+            for (int i = 0; i < vx; i++) {
+                const size_t id = i + (size_t)vx * (j + (size_t)vy * k);
+                x[3 * id + 0] = origin[0] + i * hx;
+                x[3 * id + 1] = origin[1] + j * hy;
+                x[3 * id + 2] = origin[2] + k * hz;
+            }
+    static const int perm[6][3] = { {0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0} };
+    tets.resize(24ull * nx * ny * nz);
+    size_t t = 0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++)
+                for (int p = 0; p < 6; p++) {
+                    int c[3] = { i, j, k };
+                    auto vid = [&](const int* cc) {
+                        return (int32_t)(cc[0] + vx * (cc[1] + vy * cc[2]));
+                    };
+                    tets[t++] = vid(c);
+                    c[perm[p][0]]++;
+                    tets[t++] = vid(c);
+                    c[perm[p][1]]++;
+                    tets[t++] = vid(c);
+                    c[perm[p][2]]++;
+                    tets[t++] = vid(c);
+                }
+}
+
+namespace {
+struct TriKey {
+    int32_t a, b, c;
+    bool operator==(const TriKey& o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct TriHash {
+    size_t operator()(const TriKey& k) const
+    {
+        uint64_t h = (uint64_t)(uint32_t)k.a * 0x9E3779B97F4A7C15ull;
+        h ^= (uint64_t)(uint32_t)k.b + 0x7F4A7C159E3779B9ull + (h << 6) + (h >> 2);
+        h ^= (uint64_t)(uint32_t)k.c * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+        return (size_t)h;
+    }
+};
+}  // namespace
+
+int surface_faces(int nv, int nt, const int32_t* tets, std::vector<int32_t>& faces)
+{
+    static const int lf[4][3] = { {1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1} };
+    std::unordered_map<TriKey, std::pair<int64_t, int>, TriHash> count;
+    count.reserve((size_t)nt * 3);
+    for (int e = 0; e < nt; e++)
+        for (int f = 0; f < 4; f++) {
+            std::array<int32_t, 3> v = { tets[4 * e + lf[f][0]], tets[4 * e + lf[f][1]], tets[4 * e + lf[f][2]] };
+            for (int32_t q : v)
+                if (q < 0 || q >= nv) return -1;
+            std::sort(v.begin(), v.end());
+            auto& slot = count[TriKey{ v[0], v[1], v[2] }];
+            if (slot.second == 0) slot.first = 4ll * e + f;
+            slot.second++;
+        }
+    std::vector<int64_t> owners;
+    for (auto& kv : count) {
+        if (kv.second.second > 2) return -2;
+        if (kv.second.second == 1) owners.push_back(kv.second.first);
+    }
+    std::sort(owners.begin(), owners.end());
+    faces.resize(3 * owners.size());
+    for (size_t i = 0; i < owners.size(); i++) {
+        const int64_t e = owners[i] / 4;
+        const int f = (int)(owners[i] % 4);
+        for (int j = 0; j < 3; j++) faces[3 * i + j] = tets[4 * e + lf[f][j]];
+    }
+    return 0;
+}
+
+int greedy_color(int nv, int n_items, int verts_per_item, const int32_t* items,
+                 std::vector<int32_t>& color)
+{
+    // per-vertex bit set of colours already used by an incident item
+    constexpr int WORDS = 4;  // up to 256 colours
+    std::vector<std::array<uint64_t, WORDS>> used((size_t)nv);
+    for (auto& u : used) u.fill(0);
+    color.assign(n_items, -1);
+    std::vector<int64_t> count;
+    int ncol = 0;
+    for (int e = 0; e < n_items; e++) {
+        std::array<uint64_t, WORDS> m{};
+        for (int j = 0; j < verts_per_item; j++) {
+            const int32_t v = items[(size_t)verts_per_item * e + j];
+            if (v < 0 || v >= nv) return -1;
+            for (int w = 0; w < WORDS; w++) m[w] |= used[v][w];
+        }
+        // balanced choice: the least-populated admissible colour; a new colour
+        // only when every existing one is blocked
+        int c = -1;
+        for (int k = 0; k < ncol; k++)
+            if (!((m[k / 64] >> (k % 64)) & 1ull) && (c < 0 || count[k] < count[c])) c = k;
+        if (c < 0) {
+            if (ncol == 64 * WORDS) return -2;
+            c = ncol++;
+            count.push_back(0);
+        }
+        count[c]++;
+        color[e] = c;
+        for (int j = 0; j < verts_per_item; j++) {
+            const int32_t v = items[(size_t)verts_per_item * e + j];
+            used[v][c / 64] |= 1ull << (c % 64);
+        }
+    }
+    return ncol;
+}
+
+int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int32_t>& tets,
+               std::vector<int32_t>& tris, std::string& err)
+{
+    std::ifstream in(path);
+    if (!in) { err = "cannot open " + path; return -1; }
+    std::string line;
+    std::unordered_map<long long, int32_t> tag2id;
+    x.clear(); tets.clear(); tris.clear();
+    bool have_format = false;
+    while (std::getline(in, line)) {
+        if (line.rfind("$MeshFormat", 0) == 0) {
+            std::getline(in, line);
+            std::istringstream ss(line);
+            double ver = 0; int ftype = 1, dsize = 0;
+            ss >> ver >> ftype >> dsize;
+            if (ver < 4.0 || ver >= 5.0 || ftype != 0) { err = "only GMSH 4.x ASCII is supported"; return -2; }
+            have_format = true;
+        } else if (line.rfind("$Nodes", 0) == 0) {
+            long long nblocks, nnodes, mintag, maxtag;
+            in >> nblocks >> nnodes >> mintag >> maxtag;
+            x.reserve(3 * (size_t)nnodes);
+            for (long long b = 0; b < nblocks; b++) {
+                int edim, etag, param; long long n;
+                in >> edim >> etag >> param >> n;
+                std::vector<long long> tags(n);
+                for (long long i = 0; i < n; i++) in >> tags[i];
+                for (long long i = 0; i < n; i++) {
+                    double p[3];
+                    in >> p[0] >> p[1] >> p[2];
+                    if (param) {  // skip parametric coordinates
+                        for (int d = 0; d < edim; d++) { double u; in >> u; }
+                    }
+                    tag2id[tags[i]] = (int32_t)(x.size() / 3);
+                    x.insert(x.end(), p, p + 3);
+                }
+            }
+        } else if (line.rfind("$Elements", 0) == 0) {
+            long long nblocks, nel, mintag, maxtag;
+            in >> nblocks >> nel >> mintag >> maxtag;
+            for (long long b = 0; b < nblocks; b++) {
+                int edim, etag, etype; long long n;
+                in >> edim >> etag >> etype >> n;
+                const int nn = etype == 4 ? 4 : etype == 2 ? 3 : etype == 1 ? 2 : etype == 15 ? 1 :
+                               etype == 11 ? 10 : etype == 9 ? 6 : etype == 5 ? 8 : -1;
+                if (nn < 0) { err = "unsupported element type " + std::to_string(etype); return -3; }
+                for (long long i = 0; i < n; i++) {
+                    long long etagv; in >> etagv;
+                    long long nt[10];
+                    for (int j = 0; j < nn; j++) in >> nt[j];
+                    if (etype == 4 || etype == 2) {
+                        auto& dst = etype == 4 ? tets : tris;
+                        for (int j = 0; j < nn; j++) {
+                            auto it = tag2id.find(nt[j]);
+                            if (it == tag2id.end()) { err = "element references unknown node"; return -4; }
+                            dst.push_back(it->second);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (!have_format) { err = "missing $MeshFormat"; return -5; }
+    return 0;
+}
+
+}  // namespace xpbd

@@ -1,0 +1,712 @@
+// XPBD solver kernels for gfx950 (MI355X).
+//
+// Vertex state is one 32-byte record per vertex: float4 base (displacement at
+// the start of the step, inverse mass in w) followed by float4 delta (motion in
+// the current step), so a tet's four random-access gathers touch four aligned
+// 32-byte blocks.  A separate float4 per vertex keeps the low-order part of the
+// base (compensated once per step) and is read only outside the solve.
+// Element data is structure-of-arrays in solver order so consecutive lanes
+// read consecutive words.  All kernels use 64-lane wavefronts and
+// 256-thread workgroups.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "xpbd_math.h"
+
+namespace xpbd {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Gather {
+    float b[12];  // base displacement of the 4 vertices
+    float d[12];  // in-step motion
+    float w[4];   // inverse masses
+};
+
+__device__ __forceinline__ void gather4(const float4* __restrict__ pos, const int4 t, Gather& g)
+{
+    const int vid[4] = { t.x, t.y, t.z, t.w };
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float4 b = pos[2 * (size_t)vid[i]];
+        const float4 d = pos[2 * (size_t)vid[i] + 1];
+        g.b[3 * i] = b.x;
+        g.b[3 * i + 1] = b.y;
+        g.b[3 * i + 2] = b.z;
+        g.w[i] = b.w;
+        g.d[3 * i] = d.x;
+        g.d[3 * i + 1] = d.y;
+        g.d[3 * i + 2] = d.z;
+    }
+}
+
+__device__ __forceinline__ void load_tet(const TetArrays& ta, int e, float* q, float2& at, float& g)
+{
+#pragma unroll
+    for (int k = 0; k < 9; k++) q[k] = ta.Q[(size_t)k * ta.qstride + e];
+    at = ta.at[e];
+    g = ta.gamma[e];
+}
+
+// ---------------------------------------------------------------------------
+// predict: x_prev = x ; x += dt v ; z += -g dt^2   (XPBDMeshObject.cpp:426-470)
+// The step's motion starts as delta = dt v + (0, 0, -g dt^2); the base is the
+// position at the start of the step, so x_prev needs no copy.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_predict(float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                   int nv, float dt, float dz)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nv) return;
+    const float4 v = vel[i];
+    pos[2 * (size_t)i + 1] = make_float4(dt * v.x, dt * v.y, dt * v.z + dz, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// finish: pinned vertices back to x_prev (XPBDMeshObject.cpp:497-503), then
+// v = (x - x_prev) / dt (XPBDMeshObject.cpp:507-535) = delta / dt, and the
+// base absorbs delta with a compensated (two-sum) update.  vel.w != 0 marks pins.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
+{
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_finish(float4* __restrict__ pos, float4* __restrict__ lo,
+                                                  float4* __restrict__ vel, int nv, float dt,
+                                                  unsigned long long* __restrict__ step)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) step[0] = step[0] + 1ull;
+    if (i >= nv) return;
+    float4 b = pos[2 * (size_t)i];
+    float4 d = pos[2 * (size_t)i + 1];
+    float4 l = lo[i];
+    float4 v = vel[i];
+    if (v.w != 0.0f) d = make_float4(0.f, 0.f, 0.f, 0.f);
+    v.x = d.x / dt;
+    v.y = d.y / dt;
+    v.z = d.z / dt;
+    vel[i] = v;
+    two_sum(b.x, d.x + l.x, b.x, l.x);
+    two_sum(b.y, d.y + l.y, b.y, l.y);
+    two_sum(b.z, d.z + l.z, b.z, l.z);
+    pos[2 * (size_t)i] = b;
+    pos[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    lo[i] = l;
+}
+
+// ---------------------------------------------------------------------------
+// Colour-ordered Gauss-Seidel: one launch per colour; tets of one colour share
+// no vertex, so each thread reads and writes its four vertices directly.
+// ---------------------------------------------------------------------------
+template <bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_gs_color(TetArrays ta, float4* __restrict__ pos, int begin, int end)
+{
+    const int e = begin + blockIdx.x * kBlock + threadIdx.x;
+    if (e >= end) return;
+    const int4 t = ta.tet[e];
+    Gather g;
+    gather4(pos, t, g);
+    float q[9];
+    float2 at;
+    float gm;
+    load_tet(ta, e, q, at, gm);
+    float l[2] = { 0.0f, 0.0f };
+    if (!FIRST) {
+        const float2 lv = ta.lam[e];
+        l[0] = lv.x;
+        l[1] = lv.y;
+    }
+    float dx[12];
+    project_tet_split<float>(g.b, g.d, g.w, q, at.x, at.y, gm, l, dx);
+    ta.lam[e] = make_float2(l[0], l[1]);
+    const int vid[4] = { t.x, t.y, t.z, t.w };
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        pos[2 * (size_t)vid[i] + 1] =
+            make_float4(g.d[3 * i] + dx[3 * i], g.d[3 * i + 1] + dx[3 * i + 1], g.d[3 * i + 2] + dx[3 * i + 2], 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi: every projection reads the same positions; corrections are summed
+// into acc (4 floats per vertex) with hardware float atomics, then applied.
+// ---------------------------------------------------------------------------
+template <bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_jacobi_project(TetArrays ta, const float4* __restrict__ pos,
+                                                          float* __restrict__ acc, int nt)
+{
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nt) return;
+    const int4 t = ta.tet[e];
+    Gather g;
+    gather4(pos, t, g);
+    float q[9];
+    float2 at;
+    float gm;
+    load_tet(ta, e, q, at, gm);
+    float l[2] = { 0.0f, 0.0f };
+    if (!FIRST) {
+        const float2 lv = ta.lam[e];
+        l[0] = lv.x;
+        l[1] = lv.y;
+    }
+    float dx[12];
+    project_tet_split<float>(g.b, g.d, g.w, q, at.x, at.y, gm, l, dx);
+    ta.lam[e] = make_float2(l[0], l[1]);
+    const int vid[4] = { t.x, t.y, t.z, t.w };
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) unsafeAtomicAdd(acc + 4 * (size_t)vid[i] + k, dx[3 * i + k]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_jacobi_apply(float4* __restrict__ pos, float4* __restrict__ acc, int nv)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nv) return;
+    float4 d = pos[2 * (size_t)i + 1];
+    const float4 a = acc[i];
+    d.x += a.x;
+    d.y += a.y;
+    d.z += a.z;
+    pos[2 * (size_t)i + 1] = d;
+    acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero(uint32_t* __restrict__ p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residual / energy (XPBDSolver.hpp:327-391, XPBDMeshObject.cpp:537-549).
+// The FP32 state is promoted and the constraint maths evaluated in double.
+// Partial sums use a fixed tree, so results are reproducible for a fixed grid.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void disp_f64(const float4* __restrict__ pos, const float4* __restrict__ lo, int v,
+                                         bool with_delta, double* u)
+{
+    const float4 b = pos[2 * (size_t)v];
+    const float4 l = lo[v];
+    u[0] = (double)b.x + (double)l.x;
+    u[1] = (double)b.y + (double)l.y;
+    u[2] = (double)b.z + (double)l.z;
+    if (with_delta) {
+        const float4 d = pos[2 * (size_t)v + 1];
+        u[0] += (double)d.x;
+        u[1] += (double)d.y;
+        u[2] += (double)d.z;
+    }
+}
+
+__device__ __forceinline__ void eval_tet_f64(const TetArrays& ta, const float4* pos, const float4* lo, bool with_delta,
+                                             int e, double* CD, double* CH)
+{
+    const int4 t = ta.tet[e];
+    double u0[3], u1[3], u2[3], u3[3];
+    disp_f64(pos, lo, t.x, with_delta, u0);
+    disp_f64(pos, lo, t.y, with_delta, u1);
+    disp_f64(pos, lo, t.z, with_delta, u2);
+    disp_f64(pos, lo, t.w, with_delta, u3);
+    double q[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) q[k] = ta.Q[(size_t)k * ta.qstride + e];
+    double F[9];
+    deformation_gradient<double>(u0, u1, u2, u3, q, F);
+    double J;
+    element_constraints<double>(F, (double)ta.gamma[e], CD, CH, &J);
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_residual(TetArrays ta, const float4* __restrict__ pos,
+                                                    const float4* __restrict__ lo, int with_delta, int nt,
+                                                    double* __restrict__ part, double* __restrict__ vec)
+{
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < nt; e += gridDim.x * kBlock) {
+        double CD, CH;
+        eval_tet_f64(ta, pos, lo, with_delta != 0, e, &CD, &CH);
+        const float2 at = ta.at[e];
+        const float2 l = ta.lam[e];
+        const double r1 = CD + (double)at.x * (double)l.x;
+        const double r2 = CH + (double)at.y * (double)l.y;
+        if (vec) {
+            vec[2 * (size_t)e] = r1;
+            vec[2 * (size_t)e + 1] = r2;
+        }
+        acc += r1 * r1 + r2 * r2;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy(TetArrays ta, const float4* __restrict__ pos,
+                                                  const float4* __restrict__ lo, int nt, double dt2,
+                                                  double* __restrict__ part)
+{
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < nt; e += gridDim.x * kBlock) {
+        double CD, CH;
+        eval_tet_f64(ta, pos, lo, false, e, &CD, &CH);
+        const float2 at = ta.at[e];
+        acc += CD * CD / ((double)at.x * dt2) + CH * CH / ((double)at.y * dt2);
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ part, int n,
+                                                        double* __restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) acc += part[i];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Signed distance fields and contact detection in FP64:
+// SphereSDF.cpp:16-26, BoxSDF.cpp:19-84, GeometryUtils.cpp:6-24,92-102,
+// CollisionScene.cpp:171-214 (face loop) and 253-283 (Frank-Wolfe).
+// ---------------------------------------------------------------------------
+struct Pose {
+    int kind;
+    double p[3], q[4], prm[3];
+};
+
+__device__ __forceinline__ void qrot(const double* v, const double* q, double* o)
+{
+    const double uv = q[0] * v[0] + q[1] * v[1] + q[2] * v[2];
+    const double cx = q[1] * v[2] - q[2] * v[1];
+    const double cy = q[2] * v[0] - q[0] * v[2];
+    const double cz = q[0] * v[1] - q[1] * v[0];
+    const double s = 2 * q[3] * q[3] - 1;
+    o[0] = 2 * uv * q[0] + s * v[0] + 2 * q[3] * cx;
+    o[1] = 2 * uv * q[1] + s * v[1] + 2 * q[3] * cy;
+    o[2] = 2 * uv * q[2] + s * v[2] + 2 * q[3] * cz;
+}
+
+__device__ __forceinline__ void to_body(const Pose& P, const double* x, double* xb)
+{
+    const double n2 = P.q[0] * P.q[0] + P.q[1] * P.q[1] + P.q[2] * P.q[2] + P.q[3] * P.q[3];
+    const double qi[4] = { -P.q[0] / n2, -P.q[1] / n2, -P.q[2] / n2, P.q[3] / n2 };
+    const double rel[3] = { x[0] - P.p[0], x[1] - P.p[1], x[2] - P.p[2] };
+    qrot(rel, qi, xb);
+}
+
+__device__ double sdf_eval(const Pose& P, const double* x)
+{
+    if (P.kind == 1) {
+        const double d0 = x[0] - P.p[0], d1 = x[1] - P.p[1], d2 = x[2] - P.p[2];
+        return sqrt(d0 * d0 + d1 * d1 + d2 * d2) - P.prm[0];
+    }
+    double xb[3];
+    to_body(P, x, xb);
+    double q[3], qc[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        q[k] = fabs(xb[k]) - 0.5 * P.prm[k];
+        qc[k] = q[k] > 0.0 ? q[k] : 0.0;
+    }
+    const double mq = fmax(q[0], fmax(q[1], q[2]));
+    return sqrt(qc[0] * qc[0] + qc[1] * qc[1] + qc[2] * qc[2]) + (mq < 0.0 ? mq : 0.0);
+}
+
+__device__ void sdf_grad(const Pose& P, const double* x, double* g)
+{
+    if (P.kind == 1) {
+        const double d[3] = { x[0] - P.p[0], x[1] - P.p[1], x[2] - P.p[2] };
+        const double n = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) g[k] = n > 0 ? d[k] / n : d[k];
+        return;
+    }
+    double xb[3];
+    to_body(P, x, xb);
+    double q[3], qc[3], sg[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        q[k] = fabs(xb[k]) - P.prm[k] / 2.0;
+        qc[k] = q[k] > 0.0 ? q[k] : 0.0;
+        sg[k] = xb[k] > 0 ? 1.0 : -1.0;
+    }
+    const double mq = fmax(q[0], fmax(q[1], q[2]));
+    const double md = mq < 0.0 ? mq : 0.0;
+    double go[3] = { sg[0] * qc[0], sg[1] * qc[1], sg[2] * qc[2] };
+    const double n = sqrt(go[0] * go[0] + go[1] * go[1] + go[2] * go[2]);
+    if (n > 0) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) go[k] /= n;
+    }
+    double gb[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) gb[k] = go[k] + sg[k] * (q[k] == md ? 1.0 : 0.0);
+    qrot(gb, P.q, g);
+}
+
+// world position rest + base + low part (+ in-step motion during a solve)
+__device__ __forceinline__ void world_pos(const double* rest, const float4* pos, const float4* lo, int v,
+                                          bool with_delta, double* x)
+{
+    double u[3];
+    disp_f64(pos, lo, v, with_delta, u);
+    x[0] = rest[3 * (size_t)v] + u[0];
+    x[1] = rest[3 * (size_t)v + 1] + u[1];
+    x[2] = rest[3 * (size_t)v + 2] + u[2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_detect(const int4* __restrict__ faces, int nf,
+                                                  const double* __restrict__ rest,
+                                                  const float4* __restrict__ pos, const float4* __restrict__ lo,
+                                                  const SdfDev* __restrict__ sdf,
+                                                  const unsigned long long* __restrict__ step, double dt,
+                                                  int* __restrict__ flag, double* __restrict__ cp,
+                                                  double* __restrict__ cn, double* __restrict__ cb)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= nf) return;
+    const int4 f = faces[j];
+    const SdfDev S = sdf[f.w];
+    int hit = 0;
+    if (S.kind != 0) {
+        Pose P;
+        P.kind = S.kind;
+        const double t = (double)step[0] * dt;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            P.p[k] = S.pos[k] + S.vel[k] * t;
+            P.prm[k] = S.prm[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) P.q[k] = S.quat[k];
+        double p1[3], p2[3], p3[3];
+        world_pos(rest, pos, lo, f.x, false, p1);
+        world_pos(rest, pos, lo, f.y, false, p2);
+        world_pos(rest, pos, lo, f.z, false, p3);
+        double e12 = 0, e13 = 0, e23 = 0, cen[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            e12 += (p2[k] - p1[k]) * (p2[k] - p1[k]);
+            e13 += (p3[k] - p1[k]) * (p3[k] - p1[k]);
+            e23 += (p3[k] - p2[k]) * (p3[k] - p2[k]);
+            cen[k] = (p1[k] + p2[k] + p3[k]) / 3;
+        }
+        const double me = fmax(e12, fmax(e13, e23));
+        const double cd = sdf_eval(P, cen);
+        if (!(cd * cd > me)) {
+            // Frank-Wolfe, 32 steps of size 2/(i+3), started at the closest vertex
+            const double d1 = sdf_eval(P, p1), d2 = sdf_eval(P, p2), d3 = sdf_eval(P, p3);
+            double x[3];
+            const double* st = (d1 <= d2 && d1 <= d3) ? p1 : ((d2 <= d1 && d2 <= d3) ? p2 : p3);
+#pragma unroll
+            for (int k = 0; k < 3; k++) x[k] = st[k];
+            for (int it = 0; it < 32; it++) {
+                const double al = 2.0 / (it + 3);
+                double g[3];
+                sdf_grad(P, x, g);
+                const double s1 = p1[0] * g[0] + p1[1] * g[1] + p1[2] * g[2];
+                const double s2 = p2[0] * g[0] + p2[1] * g[1] + p2[2] * g[2];
+                const double s3 = p3[0] * g[0] + p3[1] * g[1] + p3[2] * g[2];
+                const double* sv = (s1 < s2 && s1 < s3) ? p1 : ((s2 < s1 && s2 < s3) ? p2 : p3);
+#pragma unroll
+                for (int k = 0; k < 3; k++) x[k] = x[k] + al * (sv[k] - x[k]);
+            }
+            const double dist = sdf_eval(P, x);
+            if (dist <= 1e-4) {
+                double v0[3], v1[3], v2[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    v0[k] = p2[k] - p1[k];
+                    v1[k] = p3[k] - p1[k];
+                    v2[k] = x[k] - p1[k];
+                }
+                const double d00 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2];
+                const double d01 = v0[0] * v1[0] + v0[1] * v1[1] + v0[2] * v1[2];
+                const double d11 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+                const double d20 = v2[0] * v0[0] + v2[1] * v0[1] + v2[2] * v0[2];
+                const double d21 = v2[0] * v1[0] + v2[1] * v1[1] + v2[2] * v1[2];
+                const double den = d00 * d11 - d01 * d01;
+                const double bv = (d11 * d20 - d01 * d21) / den;
+                const double bw = (d00 * d21 - d01 * d20) / den;
+                double g[3];
+                sdf_grad(P, x, g);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    cp[3 * (size_t)j + k] = x[k] - g[k] * dist;
+                    cn[3 * (size_t)j + k] = g[k];
+                }
+                cb[3 * (size_t)j] = 1 - bv - bw;
+                cb[3 * (size_t)j + 1] = bv;
+                cb[3 * (size_t)j + 2] = bw;
+                hit = 1;
+            }
+        }
+    }
+    flag[j] = hit;
+}
+
+// Single-workgroup ordered compaction: cpos[j] = number of hits before j,
+// cpos[nf] = total, clist[cpos[j]] = j for every hit.
+constexpr int kScan = 1024;
+__global__ __launch_bounds__(kScan) void k_compact(const int* __restrict__ flag, int nf,
+                                                  int* __restrict__ cpos, int* __restrict__ clist)
+{
+    __shared__ int sh[kScan];
+    const int chunk = (nf + kScan - 1) / kScan;
+    const int b = threadIdx.x * chunk;
+    const int e = min(nf, b + chunk);
+    int cnt = 0;
+    for (int j = b; j < e; j++) cnt += flag[j];
+    sh[threadIdx.x] = cnt;
+    __syncthreads();
+    // Hillis-Steele inclusive scan
+    for (int off = 1; off < kScan; off <<= 1) {
+        const int v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = sh[threadIdx.x] - cnt;
+    for (int j = b; j < e; j++) {
+        cpos[j] = run;
+        if (flag[j]) clist[run++] = j;
+    }
+    if (threadIdx.x == kScan - 1) cpos[nf] = sh[kScan - 1];
+}
+
+// Static contact projection in FP64 (alpha = 0, inequality):
+// ConstraintProjector.hpp:111-172, StaticDeformableCollisionConstraint.cpp:20-55
+__device__ __forceinline__ bool contact_dx(const double* const x[3], const float* w, const double* cp,
+                                           const double* cn, const double* cb, double* lam, double* dx)
+{
+    double a[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) a[k] = cb[0] * x[0][k] + cb[1] * x[1][k] + cb[2] * x[2][k];
+    const double C = cn[0] * (a[0] - cp[0]) + cn[1] * (a[1] - cp[1]) + cn[2] * (a[2] - cp[2]);
+    if (C > 0) return false;
+    double L = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double gx = cb[i] * cn[0], gy = cb[i] * cn[1], gz = cb[i] * cn[2];
+        L += (double)w[i] * (gx * gx + gy * gy + gz * gz);
+    }
+    const double dl = -C / L;
+    *lam += dl;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) dx[3 * i + k] = (double)w[i] * (cb[i] * cn[k]) * dl;
+    return true;
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__ faces,
+                                                       const int* __restrict__ fcol_off, int ncol,
+                                                       const int* __restrict__ cpos, const int* __restrict__ clist,
+                                                       const double* __restrict__ rest, float4* __restrict__ pos,
+                                                       const float4* __restrict__ lo,
+                                                       const double* __restrict__ cp, const double* __restrict__ cn,
+                                                       const double* __restrict__ cb, double* __restrict__ clam)
+{
+    for (int c = 0; c < ncol; c++) {
+        const int s = cpos[fcol_off[c]], e = cpos[fcol_off[c + 1]];
+        for (int k = s + (int)threadIdx.x; k < e; k += kBlock) {
+            const int j = clist[k];
+            const int4 f = faces[j];
+            double x0[3], x1[3], x2[3];
+            world_pos(rest, pos, lo, f.x, true, x0);
+            world_pos(rest, pos, lo, f.y, true, x1);
+            world_pos(rest, pos, lo, f.z, true, x2);
+            const double* xs[3] = { x0, x1, x2 };
+            const float w[3] = { pos[2 * (size_t)f.x].w, pos[2 * (size_t)f.y].w, pos[2 * (size_t)f.z].w };
+            double lam = FIRST ? 0.0 : clam[j];
+            double dx[9];
+            if (contact_dx(xs, w, cp + 3 * (size_t)j, cn + 3 * (size_t)j, cb + 3 * (size_t)j, &lam, dx)) {
+                const int vid[3] = { f.x, f.y, f.z };
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    float4 d = pos[2 * (size_t)vid[i] + 1];
+                    d.x = (float)((double)d.x + dx[3 * i]);
+                    d.y = (float)((double)d.y + dx[3 * i + 1]);
+                    d.z = (float)((double)d.z + dx[3 * i + 2]);
+                    pos[2 * (size_t)vid[i] + 1] = d;
+                }
+            }
+            clam[j] = lam;
+        }
+        __syncthreads();
+    }
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_contacts_jacobi(const int4* __restrict__ faces, int nf,
+                                                           const int* __restrict__ cpos, const int* __restrict__ clist,
+                                                           const double* __restrict__ rest, const float4* __restrict__ pos,
+                                                           const float4* __restrict__ lo,
+                                                           float* __restrict__ acc, const double* __restrict__ cp,
+                                                           const double* __restrict__ cn, const double* __restrict__ cb,
+                                                           double* __restrict__ clam)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= cpos[nf]) return;
+    const int j = clist[k];
+    const int4 f = faces[j];
+    double x0[3], x1[3], x2[3];
+    world_pos(rest, pos, lo, f.x, true, x0);
+    world_pos(rest, pos, lo, f.y, true, x1);
+    world_pos(rest, pos, lo, f.z, true, x2);
+    const double* xs[3] = { x0, x1, x2 };
+    const float w[3] = { pos[2 * (size_t)f.x].w, pos[2 * (size_t)f.y].w, pos[2 * (size_t)f.z].w };
+    double lam = FIRST ? 0.0 : clam[j];
+    double dx[9];
+    if (contact_dx(xs, w, cp + 3 * (size_t)j, cn + 3 * (size_t)j, cb + 3 * (size_t)j, &lam, dx)) {
+        const int vid[3] = { f.x, f.y, f.z };
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) unsafeAtomicAdd(acc + 4 * (size_t)vid[i] + kk, (float)dx[3 * i + kk]);
+    }
+    clam[j] = lam;
+}
+
+inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_predict(hipStream_t s, float4* pos, const float4* vel, int nv, float dt, float dz)
+{
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_predict, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos, vel, nv, dt, dz);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(hipStream_t s, float4* pos, float4* lo, float4* vel, int nv, float dt,
+                         unsigned long long* step_counter)
+{
+    hipLaunchKernelGGL(k_finish, dim3(nblk(nv > 0 ? nv : 1, kBlock)), dim3(kBlock), 0, s, pos, lo, vel, nv, dt,
+                       step_counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int begin, int end, bool first)
+{
+    if (end <= begin) return hipSuccess;
+    const dim3 g(nblk(end - begin, kBlock));
+    if (first) hipLaunchKernelGGL(k_gs_color<true>, g, dim3(kBlock), 0, s, ta, pos, begin, end);
+    else hipLaunchKernelGGL(k_gs_color<false>, g, dim3(kBlock), 0, s, ta, pos, begin, end);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float4* pos, float* acc, int nt,
+                                 bool first)
+{
+    if (nt <= 0) return hipSuccess;
+    const dim3 g(nblk(nt, kBlock));
+    if (first) hipLaunchKernelGGL(k_jacobi_project<true>, g, dim3(kBlock), 0, s, ta, pos, acc, nt);
+    else hipLaunchKernelGGL(k_jacobi_project<false>, g, dim3(kBlock), 0, s, ta, pos, acc, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_apply(hipStream_t s, float4* pos, float* acc, int nv)
+{
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jacobi_apply, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos,
+                       reinterpret_cast<float4*>(acc), nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(hipStream_t s, void* p, size_t bytes)
+{
+    const size_t n = bytes / 4;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_zero, dim3(nblk((long long)n, kBlock)), dim3(kBlock), 0, s,
+                       static_cast<uint32_t*>(p), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(hipStream_t s, const TetArrays& ta, const float4* pos, const float4* lo, bool with_delta,
+                           int nt, double* part, int nblocks, double* vec)
+{
+    hipLaunchKernelGGL(k_residual, dim3(nblocks), dim3(kBlock), 0, s, ta, pos, lo, with_delta ? 1 : 0, nt, part,
+                       vec);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(hipStream_t s, const TetArrays& ta, const float4* pos, const float4* lo, int nt, double dt2,
+                         double* part, int nblocks)
+{
+    hipLaunchKernelGGL(k_energy, dim3(nblocks), dim3(kBlock), 0, s, ta, pos, lo, nt, dt2, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(hipStream_t s, const double* part, int n, double* out)
+{
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, part, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_detect(hipStream_t s, const int4* faces, int nf, const double* rest, const float4* pos,
+                         const float4* lo, const SdfDev* sdf, const unsigned long long* step, double dt, int* flag,
+                         double* cp, double* cn, double* cb)
+{
+    if (nf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_detect, dim3(nblk(nf, kBlock)), dim3(kBlock), 0, s, faces, nf, rest, pos, lo, sdf, step, dt,
+                       flag, cp, cn, cb);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int* clist)
+{
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(kScan), 0, s, flag, nf, cpos, clist);
+    return hipGetLastError();
+}
+
+hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fcol_off, int ncol, const int* cpos,
+                              const int* clist, const double* rest, float4* pos, const float4* lo, const double* cp,
+                              const double* cn, const double* cb, double* clam, bool first)
+{
+    if (first)
+        hipLaunchKernelGGL(k_contacts_gs<true>, dim3(1), dim3(kBlock), 0, s, faces, fcol_off, ncol, cpos, clist,
+                           rest, pos, lo, cp, cn, cb, clam);
+    else
+        hipLaunchKernelGGL(k_contacts_gs<false>, dim3(1), dim3(kBlock), 0, s, faces, fcol_off, ncol, cpos, clist,
+                           rest, pos, lo, cp, cn, cb, clam);
+    return hipGetLastError();
+}
+
+hipError_t launch_contacts_jacobi(hipStream_t s, const int4* faces, int nf, const int* cpos, const int* clist,
+                                  const double* rest, const float4* pos, const float4* lo, float* acc,
+                                  const double* cp, const double* cn, const double* cb, double* clam, bool first)
+{
+    if (nf <= 0) return hipSuccess;
+    const dim3 g(nblk(nf, kBlock));
+    if (first)
+        hipLaunchKernelGGL(k_contacts_jacobi<true>, g, dim3(kBlock), 0, s, faces, nf, cpos, clist, rest, pos, lo,
+                           acc, cp, cn, cb, clam);
+    else
+        hipLaunchKernelGGL(k_contacts_jacobi<false>, g, dim3(kBlock), 0, s, faces, nf, cpos, clist, rest, pos, lo,
+                           acc, cp, cn, cb, clam);
+    return hipGetLastError();
+}
+
+}  // namespace xpbd
