@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Benchmark: simulated steps/s and tet projections/s of the XPBD solver on one MI355X.
+
+Default workload: the heterogeneous tissue block (56^3 cells x 6 tets =
+1,053,696 tets, 185,193 vertices, base pinned), colour-ordered Gauss-Seidel
+with 20 iterations, dt = 1e-3, a sphere (r = 0.05 m) pressed down at 0.1 m/s
+with contacts re-detected every step.  Inputs are synthetic (seeded) and
+resident in HBM before timing; a step is one full time step (detect, predict,
+20 iterations, pin reset, velocity update) replayed from a HIP graph.
+
+Prints one JSON line.  `roofline` covers the element-projection kernel
+(algorithmic bytes per solver iteration, 80 B/tet + 28 B/vertex, divided by
+the summed projection-kernel time per iteration, HIP events on the solver's
+stream); `cpu_baseline` times the FP64 oracle (single thread, same loop order
+as the reference CPU solver) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "xpbd-tissue-sim_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="block", choices=["block", "beam", "cube", "batch"])
+    ap.add_argument("--solver", default=None, choices=["gs", "jacobi"])
+    ap.add_argument("--iters", type=int, default=None)
+    ap.add_argument("--timing-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def build_workload(name, solver, iters):
+    import workloads as W
+    import xpbd_amd as X
+    if name == "block":
+        mesh = W.block(56)
+        s = X.Solver(mesh, dt=1e-3, iters=iters or 20, solver=solver or "gs", collision_every=1)
+        kind, pos, quat, prm, vel = W.block_sphere(56)
+        s.set_sdf(kind, pos, quat, prm)
+        s.set_sdf_velocity(vel)
+        cfg = {"workload": "tissue block 56^3 cells x6 tets, heterogeneous E/nu, sphere contact",
+               "tets": mesh.nt, "vertices": mesh.nv, "solver": solver or "gs", "iterations": iters or 20,
+               "dt": 1e-3}
+        return s, [mesh], cfg, (kind, pos, quat, prm, vel)
+    if name == "beam":
+        mesh = W.beam(200, 20, 20)
+        s = X.Solver(mesh, dt=1e-3, iters=iters or 10, solver=solver or "gs", collision_every=0)
+        cfg = {"workload": "beam 200x20x20 cells x6 tets, E=1e4, nu=0.49", "tets": mesh.nt, "vertices": mesh.nv,
+               "solver": solver or "gs", "iterations": iters or 10, "dt": 1e-3}
+        return s, [mesh], cfg, None
+    if name == "cube":
+        mesh = W.cube(10)
+        s = X.Solver(mesh, dt=1e-3, iters=iters or 5, solver=solver or "gs", collision_every=0)
+        cfg = {"workload": "cube 10^3 cells x6 tets, E=3e3, nu=0.45", "tets": mesh.nt, "vertices": mesh.nv,
+               "solver": solver or "gs", "iterations": iters or 5, "dt": 1e-3}
+        return s, [mesh], cfg, None
+    meshes, sdfs = W.batch(256)
+    s = X.Solver(meshes, dt=1e-3, iters=iters or 10, solver=solver or "jacobi", collision_every=1)
+    for i, (kind, pos, quat, prm) in enumerate(sdfs):
+        s.set_sdf(kind, pos, quat, prm, instance=i)
+    cfg = {"workload": "256 cubes of 4..24 cells per axis x6 tets, box contacts", "instances": 256,
+           "tets": s.nt, "vertices": s.nv, "solver": solver or "jacobi", "iterations": iters or 10, "dt": 1e-3}
+    return s, meshes, cfg, None
+
+
+def cpu_baseline(meshes, cfg, sdf, nsteps):
+    """FP64 oracle, one thread, on the first mesh of the workload (bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    mesh = meshes[0]
+    o = orc.Sim(mesh, dt=cfg["dt"], iters=cfg["iterations"], mode=cfg["solver"])
+    if sdf is not None:
+        kind, pos, quat, prm, vel = sdf
+        o.set_sdf(kind, pos, quat, prm)
+    t0 = time.perf_counter()
+    for n in range(nsteps):
+        if sdf is not None:
+            p = [pos[k] + vel[k] * n * cfg["dt"] for k in range(3)]
+            o.set_sdf(kind, p, quat, prm)
+        o.step(1, detect=sdf is not None)
+    el = time.perf_counter() - t0
+    return {"value": nsteps / el, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"{nsteps} steps of the same workload ({mesh.nt} tets, {cfg['iterations']} "
+                      f"{cfg['solver']} iterations) on one host core, FP64 oracle",
+            "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el,
+            "host": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this solver runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md but not built)"}))
+        sys.exit(2)
+    import xpbd_amd as X  # noqa: F401  (fails loudly if the HIP library is missing)
+    s, meshes, cfg, sdf = build_workload(a.workload, a.solver, a.iters)
+    T, V = s.nt, s.nv
+    iters = cfg["iterations"]
+
+    s.step(a.warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(a.steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    steps_per_s = a.steps / el
+    tet_proj_per_s = steps_per_s * iters * T
+
+    # projection-kernel timing with HIP events on the solver stream (no graph)
+    s.set_kernel_timing(True)
+    s.step(a.timing_steps)
+    ms, nit, nlaunch = s.kernel_stats()
+    s.set_kernel_timing(False)
+    bytes_iter = 80.0 * T + 28.0 * V
+    us_per_iter = 1e3 * ms / max(nit, 1)
+    achieved = bytes_iter / (us_per_iter * 1e-6) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{a.workload}_{cfg['solver']}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_iteration")
+
+    res = s.residual()
+    out = {
+        "metric": "sim steps/sec & tet projections/sec at equal residual, learned vs plain, 1 GPU",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 / steps_per_s,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": cfg,
+        "tet_projections_per_s": tet_proj_per_s,
+        "residual_after_run": res,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "element projection (all colour launches of one iteration)",
+                     "us_per_iteration": us_per_iter, "algorithmic_bytes_per_iteration": bytes_iter,
+                     "launches_timed": nlaunch},
+    }
+    if not a.no_cpu:
+        n = a.cpu_steps if a.cpu_steps is not None else (5 if a.workload == "block" else 20)
+        out["cpu_baseline"] = cpu_baseline(meshes, cfg, sdf, n)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
