@@ -63,6 +63,15 @@ _sig("orc_residual", [C.c_void_p, _dp], C.c_double)
 _sig("orc_energy", [C.c_void_p], C.c_double)
 _sig("orc_set_contact_order", [C.c_void_p, _ip])
 _sig("orc_last_step_residual", [C.c_void_p], C.c_double)
+_sig("orc_project_contact", [_dp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int)
+
+
+def project_contact(x9, w3, cp, cn, bary, lam=0.0):
+    lam_c = C.c_double(lam)
+    dx = np.zeros(9)
+    active = lib.orc_project_contact(p(_d(x9).reshape(9)), p(_d(w3)), p(_d(cp)), p(_d(cn)), p(_d(bary)),
+                                     C.byref(lam_c), p(dx))
+    return bool(active), dx, lam_c.value
 
 
 def p(a, ct=C.c_double):

@@ -251,6 +251,15 @@ static int project_contact(const double* const p[3], const double* w,
     return 1;
 }
 
+/* One contact projection on three given positions (x9), for unit checks.
+ * Returns 1 when the constraint was active and dx9 / *lam were updated. */
+ORC_API int orc_project_contact(const double* x9, const double* w3, const double* cp,
+                                const double* cn, const double* bary, double* lam, double* dx9)
+{
+    const double* p[3] = { x9, x9 + 3, x9 + 6 };
+    return project_contact(p, w3, cp, cn, bary, lam, dx9);
+}
+
 /* ------------------------------------------------------------------------ */
 /* a19 SDFs: src/geometry/SphereSDF.cpp:16-26, src/geometry/BoxSDF.cpp:19-84 */
 /*     Quaternions are (x, y, z, w) as in src/utils/GeometryUtils.cpp:92-102  */

@@ -1,0 +1,143 @@
+"""Host-side logic and the C-ABI boundary (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import xpbd_amd as X
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "xpbd_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(xpbd_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) > 40
+    lib = C.CDLL(X.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+# ---------------------------------------------------------------- mesh tools
+@pytest.mark.parametrize("n", [(1, 1, 1), (3, 2, 4), (10, 10, 10)])
+def test_grid_counts_volume_and_conformity(n):
+    nx, ny, nz = n
+    x, t = X.grid_mesh(nx, ny, nz, 0.01)
+    assert len(x) == (nx + 1) * (ny + 1) * (nz + 1) and len(t) == 6 * nx * ny * nz
+    D = np.stack([x[t[:, 0]] - x[t[:, 3]], x[t[:, 1]] - x[t[:, 3]], x[t[:, 2]] - x[t[:, 3]]], axis=2)
+    vol = np.abs(np.linalg.det(D)) / 6
+    np.testing.assert_allclose(vol, 1e-6 / 6, rtol=1e-9)             # 6 equal tets per cell
+    assert vol.sum() == pytest.approx(nx * ny * nz * 1e-6, rel=1e-12)
+    f = X.surface_faces(len(x), t)
+    # conforming split: every boundary quad is exactly two triangles, all on the box surface
+    assert len(f) == 4 * (nx * ny + ny * nz + nx * nz)
+    assert np.all(_on_box_surface(x, f))
+
+
+def _on_box_surface(x, f):
+    lo, hi = x.min(0), x.max(0)
+    P = x[f]                                                  # (F, 3 vertices, 3 coords)
+    return np.any([(np.isclose(P[:, :, a], lo[a]).all(1) | np.isclose(P[:, :, a], hi[a]).all(1))
+                   for a in range(3)], axis=0)
+
+
+def test_surface_rejects_non_manifold():
+    t = np.array([[0, 1, 2, 3], [0, 1, 2, 4], [0, 1, 2, 5]], np.int32)
+    with pytest.raises(X.XpbdError):
+        X.surface_faces(6, t)
+
+
+@pytest.mark.parametrize("n", [4, 9])
+def test_coloring_is_valid_and_balanced(n):
+    x, t = X.grid_mesh(n, n, n, 0.01)
+    col, nc = X.color_items(len(x), t)
+    assert nc >= 24                          # interior vertices touch 24 tets
+    for c in range(nc):
+        sel = t[col == c].ravel()
+        assert len(np.unique(sel)) == len(sel), f"colour {c} shares a vertex"
+    counts = np.bincount(col)
+    assert counts.min() > 0.5 * counts.mean()
+
+
+def test_face_coloring_is_valid():
+    x, t = X.grid_mesh(5, 4, 3, 0.01)
+    f = X.surface_faces(len(x), t)
+    col, nc = X.color_items(len(x), f)
+    for c in range(nc):
+        sel = f[col == c].ravel()
+        assert len(np.unique(sel)) == len(sel)
+
+
+@pytest.mark.parametrize("name,nv,nt,nf", [("single.msh", 4, 1, 4), ("cube2.msh", 27, 40, 48),
+                                           ("cube4.msh", 125, 320, 192)])
+def test_msh_reader_fixture_counts(name, nv, nt, nf):
+    x, t, f = X.read_msh(os.path.join(GOLDEN, "msh", name))
+    assert (len(x), len(t), len(f)) == (nv, nt, nf)
+    assert t.min() >= 0 and t.max() < nv
+    # the shipped surface triangles are the unshared faces on the box surface; these
+    # 5-tets-per-hex meshes also leave interior faces unshared (the diagonals of
+    # neighbouring hexes do not match), which face hashing reports too
+    fs = X.surface_faces(len(x), t)
+    outer = fs[_on_box_surface(x, fs)]
+    assert {tuple(sorted(r)) for r in outer} == {tuple(sorted(r)) for r in f}
+
+
+def test_msh_reader_missing_file():
+    with pytest.raises(X.XpbdError) as e:
+        X.read_msh("/nonexistent/file.msh")
+    assert e.value.code == -8
+
+
+# ---------------------------------------------------------------- argument validation
+def _desc(mesh):
+    return mesh.desc()
+
+
+def _create(mesh, **kw):
+    p = X.make_params(**kw)
+    h = C.c_void_p()
+    rc = X.lib.xpbd_create(C.byref(_desc(mesh)), C.byref(p), C.byref(h))
+    return rc, h
+
+
+def _small():
+    import workloads as W
+    return W.grid_object(2, 2, 2, 0.01, noise=0.0)
+
+
+def test_create_rejects_bad_arguments_before_touching_the_device():
+    m = _small()
+    assert _create(m, dt=0.0)[0] == -1
+    assert _create(m, solver=7)[0] == -1
+    assert _create(m, iters=-1)[0] == -1
+    bad = _small()
+    bad.tets = bad.tets.copy()
+    bad.tets[0, 0] = 10_000
+    assert _create(bad)[0] == -1
+    assert b"out of range" in X.lib.xpbd_last_error(None)
+    nan = _small()
+    nan.x0 = nan.x0.copy()
+    nan.x0[3, 1] = np.nan
+    assert _create(nan)[0] == -3
+    mat = _small()
+    mat.nu = np.full(mat.nt, 0.5)
+    assert _create(mat)[0] == -1
+    flat = _small()
+    flat.x0 = flat.x0.copy()
+    flat.x0[:, 2] = 0.0
+    rc, _ = _create(flat)
+    assert rc in (-4, -2)       # zero volume (or no device here)
+
+
+def test_null_handles_are_rejected():
+    assert X.lib.xpbd_step(None, 1) == -1
+    assert X.lib.xpbd_destroy(None) == 0
+    assert X.lib.xpbd_last_error(None) is not None

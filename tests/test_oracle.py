@@ -1,0 +1,390 @@
+"""Pinning the FP64 oracle.
+
+The reference has no tests, golden vectors or published numbers for this path
+and cannot be built here, so each oracle function is pinned by something that
+does not restate its own formula: closed forms, invariants, finite
+differences, an independent autograd restatement (torch_ref.py), brute force
+on tiny inputs, or the mesh fixtures shipped with the reference
+(tests/golden/msh/, copied data files).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from helpers import rel_err
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+RNG = np.random.default_rng(1234)
+
+
+def random_tet(scale=0.01, rng=RNG):
+    while True:
+        X = rng.normal(size=(4, 3)) * scale
+        D = np.stack([X[0] - X[3], X[1] - X[3], X[2] - X[3]], 1)
+        if abs(np.linalg.det(D)) > 0.05 * scale ** 3:
+            return X
+
+
+# ---------------------------------------------------------------- a1 material
+def test_lame_known_answers():
+    mu, lam = orc.lame(3e3, 0.45)
+    assert mu == pytest.approx(1034.4827586206898, rel=1e-14)
+    assert lam == pytest.approx(9310.344827586207, rel=1e-14)
+    mu, lam = orc.lame(1e4, 0.25)          # nu = 1/4 gives lambda = mu
+    assert lam == pytest.approx(mu, rel=1e-14) and mu == pytest.approx(4000.0, rel=1e-14)
+
+
+# ---------------------------------------------------------------- a2 rest state
+def test_rest_inverse_and_volume():
+    for _ in range(50):
+        X = random_tet()
+        Q, V, rc = orc.rest_tet(X)
+        D = np.stack([X[0] - X[3], X[1] - X[3], X[2] - X[3]], 1)
+        assert rc == 0
+        np.testing.assert_allclose(Q @ D, np.eye(3), atol=1e-12)
+        assert V == pytest.approx(abs(np.linalg.det(D)) / 6, rel=1e-12)
+
+
+def test_rest_unit_and_fixture_volumes():
+    _, V, _ = orc.rest_tet([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]])
+    assert V == pytest.approx(1 / 6, rel=1e-15)
+    import xpbd_amd as X
+    x, t, _ = X.read_msh(os.path.join(GOLDEN, "msh", "single.msh"))
+    _, V, _ = orc.rest_tet(x[t[0]])
+    # base triangle area sqrt(3), apex height 1.73205080757
+    assert V == pytest.approx(math.sqrt(3) * 1.73205080757 / 3, rel=1e-10)
+
+
+def test_rest_translation_invariant():
+    X = random_tet()
+    Q1, V1, _ = orc.rest_tet(X)
+    Q2, V2, _ = orc.rest_tet(X + np.array([3.0, -2.0, 7.0]))
+    np.testing.assert_allclose(Q1, Q2, rtol=1e-9)
+    assert V1 == pytest.approx(V2, rel=1e-9)
+
+
+def test_rest_degenerate_flagged():
+    _, _, rc = orc.rest_tet([[0, 0, 0], [1, 0, 0], [2, 0, 0], [3, 0, 0]])
+    assert rc != 0
+
+
+# ---------------------------------------------------------------- a3/a4 constraints
+def test_constraints_at_rest():
+    X = random_tet()
+    Q, _, _ = orc.rest_tet(X)
+    gamma = 0.0347
+    C, g = orc.eval_tet(X, Q, gamma)
+    assert C[0] == pytest.approx(math.sqrt(3), rel=1e-13)
+    assert C[1] == pytest.approx(-gamma, abs=1e-13)
+    # at F = I: grad C_D at vertex i = (row i of Q) / sqrt(3); grad C_H = row i of Q
+    for i in range(3):
+        np.testing.assert_allclose(g[3 * i:3 * i + 3], Q[i] / math.sqrt(3), rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(g[12 + 3 * i:12 + 3 * i + 3], Q[i], rtol=1e-12, atol=1e-9)
+
+
+def test_gradients_sum_to_zero_and_match_autograd():
+    import torch_ref
+    for trial in range(20):
+        X = random_tet()
+        Q, _, _ = orc.rest_tet(X)
+        x = X + RNG.normal(size=X.shape) * (0.002 if trial % 2 else 0.0005)
+        C, g = orc.eval_tet(x, Q, 0.02)
+        for k in range(3):
+            assert abs(g[k] + g[3 + k] + g[6 + k] + g[9 + k]) < 1e-9 * np.abs(g[:12]).max()
+            assert abs(g[12 + k] + g[15 + k] + g[18 + k] + g[21 + k]) < 1e-9 * np.abs(g[12:]).max()
+        Qt, _ = torch_ref.rest(X)
+        CD, CH, gD, gH = torch_ref.grads(x, Qt.numpy(), 0.02)
+        assert C[0] == pytest.approx(CD, rel=1e-12)
+        assert C[1] == pytest.approx(CH, rel=1e-10, abs=1e-12)
+        np.testing.assert_allclose(g[:12], gD, rtol=1e-9, atol=1e-9 * np.abs(gD).max())
+        np.testing.assert_allclose(g[12:], gH, rtol=1e-9, atol=1e-9 * np.abs(gH).max())
+
+
+@pytest.mark.parametrize("squash", [0.9, 1.1])   # J < 1 and J > 1 branches
+def test_gradients_central_differences(squash):
+    X = random_tet()
+    Q, _, _ = orc.rest_tet(X)
+    x = (X - X.mean(0)) * squash + RNG.normal(size=X.shape) * 1e-4
+    C0, g = orc.eval_tet(x, Q, 0.03)
+    h = 1e-7
+    fd = np.zeros(24)
+    for j in range(12):
+        e = np.zeros(12)
+        e[j] = h
+        Cp, _ = orc.eval_tet(x.reshape(12) + e, Q, 0.03)
+        Cm, _ = orc.eval_tet(x.reshape(12) - e, Q, 0.03)
+        fd[j], fd[12 + j] = (Cp - Cm) / (2 * h)
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-6 * np.abs(g).max())
+
+
+def test_hydrostatic_branch_continuity_and_taylor_value():
+    Q = np.eye(3)
+    gamma = 0.05
+    X = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]], float)
+
+    def CH(J):
+        s = J ** (1 / 3)
+        return orc.eval_tet(X * s, Q, gamma)[0][1]
+
+    eps = 1e-7
+    assert CH(1 - eps) == pytest.approx(CH(1 + eps), abs=3e-7)        # C0 at J = 1
+    slope_lo = (CH(1 - eps) - CH(1 - 3 * eps)) / (2 * eps)
+    slope_hi = (CH(1 + 3 * eps) - CH(1 + eps)) / (2 * eps)
+    assert slope_lo == pytest.approx(1.0, abs=1e-5) and slope_hi == pytest.approx(1.0, abs=1e-5)  # C1
+    # J = 0.5: (J-1) - (J-1)^2/2 + (J-1)^3/3 = -0.5 - 0.125 - 0.041666...
+    assert CH(0.5) == pytest.approx(-gamma - 0.5 - 0.125 - 1 / 24, rel=1e-12)
+    assert CH(2.0) == pytest.approx(math.log(2.0) - gamma, rel=1e-12)
+
+
+def test_rotation_invariance():
+    X = random_tet()
+    Q, _, _ = orc.rest_tet(X)
+    x = X + RNG.normal(size=X.shape) * 1e-3
+    a, b, c = 0.3, -1.1, 0.7
+    Rz = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+    Ry = np.array([[math.cos(b), 0, math.sin(b)], [0, 1, 0], [-math.sin(b), 0, math.cos(b)]])
+    Rx = np.array([[1, 0, 0], [0, math.cos(c), -math.sin(c)], [0, math.sin(c), math.cos(c)]])
+    R = Rz @ Ry @ Rx
+    C1, _ = orc.eval_tet(x, Q, 0.02)
+    C2, _ = orc.eval_tet(x @ R.T + 5.0, Q, 0.02)
+    np.testing.assert_allclose(C1, C2, rtol=1e-10)
+
+
+# ---------------------------------------------------------------- a7 combined projection
+def test_projection_matches_autograd_restatement():
+    import torch_ref
+    for trial in range(20):
+        X = random_tet()
+        Q, V, _ = orc.rest_tet(X)
+        mu, lam = orc.lame(1e4, 0.49)
+        aD, aH, gamma = 1 / (mu * V), 1 / (lam * V), mu / lam
+        w = 1.0 / (RNG.uniform(0.5, 2.0, 4) * 1e-3)
+        x = X + RNG.normal(size=X.shape) * 1e-3
+        lam0 = RNG.normal(size=2) * 1e-4 if trial % 2 else np.zeros(2)
+        dx, l1 = orc.project_tet(x, w, Q, aD, aH, gamma, 1e-3, lam0)
+        Qt, _ = torch_ref.rest(X)
+        dx_t, l1_t = torch_ref.project(x, w, Qt.numpy(), aD, aH, gamma, 1e-3, lam0)
+        np.testing.assert_allclose(dx, dx_t, rtol=1e-7, atol=1e-9 * np.abs(dx_t).max())
+        np.testing.assert_allclose(l1, l1_t, rtol=1e-7)
+
+
+def test_projection_rest_stationary():
+    """At F = I the gradients are parallel (grad C_D = grad C_H / sqrt 3) and the
+    2x2 solve gives dlam_D / sqrt 3 + dlam_H = (gamma alpha_D - alpha_H)/det = 0."""
+    X = random_tet()
+    Q, V, _ = orc.rest_tet(X)
+    mu, lam = orc.lame(5e4, 0.49)
+    dx, l1 = orc.project_tet(X, np.full(4, 1e3), Q, 1 / (mu * V), 1 / (lam * V), mu / lam, 1e-3, np.zeros(2))
+    assert np.abs(dx).max() < 1e-13 * 0.01
+    assert abs(l1[0]) > 0   # the multipliers do move; the positions do not
+
+
+def test_repeated_projection_converges_quadratically():
+    """Repeating the block projection on one element drives its residual
+    r = C + alpha~ lambda to zero like Newton's method (the local 2x2 system is
+    solved exactly at every linearisation)."""
+    X = random_tet()
+    Q, V, _ = orc.rest_tet(X)
+    mu, lam = orc.lame(1e4, 0.45)
+    aD, aH, gamma, dt = 1 / (mu * V), 1 / (lam * V), mu / lam, 1e-3
+    w = np.full(4, 1e3)
+    x = X + RNG.normal(size=X.shape) * 2e-3
+    l = np.zeros(2)
+    res = []
+    for _ in range(6):
+        dx, l = orc.project_tet(x, w, Q, aD, aH, gamma, dt, l)
+        x = x + dx.reshape(4, 3)
+        C, _ = orc.eval_tet(x, Q, gamma)
+        res.append(np.hypot(C[0] + aD / dt ** 2 * l[0], C[1] + aH / dt ** 2 * l[1]))
+    assert res[3] < 1e-10 and res[1] < res[0] ** 1.5 * 10
+
+
+# ---------------------------------------------------------------- a8/a9 solvers
+def _mesh(n=3, E=3e3, nu=0.45, noise=0.01, seed=3, pin=True):
+    import workloads as W
+    return W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2 if pin else None, noise=noise, seed=seed)
+
+
+def test_rest_mesh_stays_at_rest():
+    m = _mesh(4, noise=0.0)
+    for mode in ("gs", "jacobi"):
+        s = orc.Sim(m, g=0.0, iters=5, mode=mode)
+        s.step(100)
+        assert np.abs(s.x() - m.x0).max() < 1e-15 and np.abs(s.v()).max() < 1e-12
+
+
+def test_free_fall_closed_form():
+    m = _mesh(2, noise=0.0, pin=False)
+    s = orc.Sim(m, g=9.81, iters=3, mode="gs")
+    n, dt = 37, 1e-3
+    s.step(n)
+    dz = s.x()[:, 2] - m.x0[:, 2]
+    np.testing.assert_allclose(dz, -9.81 * dt * dt * n * (n + 1) / 2, rtol=1e-12)
+    np.testing.assert_allclose(s.v()[:, 2], -9.81 * dt * n, rtol=1e-9)
+
+
+def test_pinned_vertices_fixed_and_velocity_definition():
+    m = _mesh(3)
+    s = orc.Sim(m, iters=4, mode="gs")
+    for _ in range(5):
+        xp = s.x()
+        s.step(1)
+        x = s.x()
+        pin = m.pinned.astype(bool)
+        assert np.array_equal(x[pin], m.x0[pin])
+        np.testing.assert_allclose(s.v(), (x - xp) / 1e-3, rtol=1e-12, atol=1e-12)
+
+
+def test_jacobi_independent_of_element_order():
+    m = _mesh(3)
+    a = orc.Sim(m, iters=3, mode="jacobi")
+    perm = RNG.permutation(m.nt)
+    m2 = _mesh(3)
+    m2.tets = m.tets[perm]
+    m2.E, m2.nu, m2.rho = m.E[perm], m.nu[perm], m.rho[perm]
+    b = orc.Sim(m2, iters=3, mode="jacobi")
+    a.step(10)
+    b.step(10)
+    assert rel_err(b.x(), a.x()) < 1e-13
+
+
+def test_gs_order_equals_permuted_mesh():
+    m = _mesh(3)
+    perm = RNG.permutation(m.nt).astype(np.int32)
+    a = orc.Sim(m, iters=3, mode="gs")
+    a.set_order(perm)
+    m2 = _mesh(3)
+    m2.tets = m.tets[perm]
+    m2.E, m2.nu, m2.rho = m.E[perm], m.nu[perm], m.rho[perm]
+    b = orc.Sim(m2, iters=3, mode="gs")
+    a.step(10)
+    b.step(10)
+    # equal up to the rounding of the lumped masses, which are summed in element order
+    assert rel_err(b.x(), a.x()) < 1e-12
+    np.testing.assert_allclose(a.lambdas()[perm], b.lambdas(), rtol=1e-9, atol=1e-15)
+
+
+def test_gs_within_one_colour_equals_jacobi():
+    """Tets of one colour share no vertex, so one GS sweep over a colour class
+    is one Jacobi sweep over the same class."""
+    import xpbd_amd as X
+    m = _mesh(3)
+    col, nc = X.color_items(m.nv, m.tets)
+    sel = np.where(col == 0)[0]
+    m1 = _mesh(3)
+    m1.tets, m1.E, m1.nu, m1.rho = m.tets[sel], m.E[sel], m.nu[sel], m.rho[sel]
+    gs = orc.Sim(m1, iters=1, mode="gs")
+    ja = orc.Sim(m1, iters=1, mode="jacobi")
+    gs.step(1)
+    ja.step(1)
+    assert rel_err(gs.x(), ja.x()) < 1e-15
+
+
+# ---------------------------------------------------------------- a11/a12 residual, energy
+def test_energy_at_rest_closed_form():
+    m = _mesh(3, noise=0.0)
+    s = orc.Sim(m, g=0.0, iters=1, mode="gs")
+    _, vol = s.rest()
+    mu, lam = orc.lame(3e3, 0.45)
+    expected = np.sum(vol * (3 * mu + lam * (mu / lam) ** 2))
+    assert s.energy() == pytest.approx(expected, rel=1e-12)
+
+
+def test_residual_definition_against_autograd_evaluation():
+    import torch_ref
+    m = _mesh(3)
+    s = orc.Sim(m, iters=2, mode="gs")
+    s.step(3)
+    norm, vec = s.residual(vec=True)
+    x, lam = s.x(), s.lambdas()
+    aD, aH, gam = s.compliance()
+    for e in range(0, m.nt, 7):
+        t = m.tets[e]
+        Qt, _ = torch_ref.rest(m.x0[t])
+        CD, CH, _, _ = torch_ref.grads(x[t], Qt.numpy(), gam[e])
+        assert vec[e, 0] == pytest.approx(CD + aD[e] / 1e-6 * lam[e, 0], rel=1e-9, abs=1e-12)
+        assert vec[e, 1] == pytest.approx(CH + aH[e] / 1e-6 * lam[e, 1], rel=1e-9, abs=1e-12)
+    assert norm == pytest.approx(np.linalg.norm(vec), rel=1e-12)
+
+
+# ---------------------------------------------------------------- a18/a19 contacts
+def test_contact_projection_solves_linear_constraint_exactly():
+    x = np.array([[0, 0, -0.001], [0.01, 0, -0.001], [0, 0.01, -0.001]], float)
+    n = np.array([0, 0, 1.0])
+    bary = np.array([0.2, 0.3, 0.5])
+    w = np.array([1e3, 2e3, 5e2])
+    active, dx, lam = orc.project_contact(x, w, np.zeros(3), n, bary)
+    assert active
+    xn = x + dx.reshape(3, 3)
+    assert n @ (bary @ xn) == pytest.approx(0.0, abs=1e-18)
+    active, dx, _ = orc.project_contact(x + [0, 0, 0.002], w, np.zeros(3), n, bary)
+    assert not active and not dx.any()        # separated: inequality inactive
+
+
+def test_sdf_known_values():
+    d, g = orc.sdf_eval(orc.SDF_SPHERE, [1, 2, 3], [0, 0, 0, 1], [0.5, 0, 0], [1, 2, 4])
+    assert d == pytest.approx(0.5) and np.allclose(g, [0, 0, 1])
+    box = dict(pos=[0, 0, 0], quat=[0, 0, 0, 1], prm=[2, 4, 6])
+    d, g = orc.sdf_eval(orc.SDF_BOX, box["pos"], box["quat"], box["prm"], [3, 0, 0])
+    assert d == pytest.approx(2.0) and np.allclose(g, [1, 0, 0])
+    d, _ = orc.sdf_eval(orc.SDF_BOX, box["pos"], box["quat"], box["prm"], [2, 3, 0])   # edge region
+    assert d == pytest.approx(math.hypot(1, 1))
+    d, g = orc.sdf_eval(orc.SDF_BOX, box["pos"], box["quat"], box["prm"], [0.2, 0, 2.5])  # inside, nearest +z
+    assert d == pytest.approx(-0.5) and np.allclose(g, [0, 0, 1])
+    # rotated box: 90 degrees about z swaps the x and y extents
+    q = [0, 0, math.sin(math.pi / 4), math.cos(math.pi / 4)]
+    d, g = orc.sdf_eval(orc.SDF_BOX, [0, 0, 0], q, [2, 4, 6], [0, 3, 0])
+    assert d == pytest.approx(2.0) and np.allclose(g, [0, 1, 0], atol=1e-12)
+
+
+def closest_point_on_triangle(p, a, b, c):
+    """Ericson, Real-Time Collision Detection, 5.1.5 (brute force over regions)."""
+    cands = []
+    n = np.cross(b - a, c - a)
+    q = p - (p - a) @ n / (n @ n) * n
+    bc = orc.barycentric(q, a, b, c)
+    if np.all(bc >= 0):
+        cands.append(q)
+    for u, v in ((a, b), (b, c), (c, a)):
+        t = np.clip((p - u) @ (v - u) / ((v - u) @ (v - u)), 0, 1)
+        cands.append(u + t * (v - u))
+    return min(cands, key=lambda z: np.linalg.norm(z - p))
+
+
+def test_frank_wolfe_vs_closed_form_closest_point():
+    for _ in range(20):
+        a, b, c = RNG.normal(size=(3, 3)) * 0.01
+        ctr = RNG.normal(size=3) * 0.02
+        x = orc.frank_wolfe(orc.SDF_SPHERE, ctr, [0, 0, 0, 1], [0.001, 0, 0], a, b, c)
+        cp = closest_point_on_triangle(ctr, a, b, c)
+        size = max(np.linalg.norm(b - a), np.linalg.norm(c - a), np.linalg.norm(c - b))
+        # FW with step 2/(k+3): the distance gap decays like O(1/k)
+        assert np.linalg.norm(x - ctr) - np.linalg.norm(cp - ctr) <= 4 * size / 32
+
+
+def test_barycentric_reconstructs_point():
+    a, b, c = RNG.normal(size=(3, 3))
+    uvw = RNG.dirichlet(np.ones(3))
+    p = uvw[0] * a + uvw[1] * b + uvw[2] * c
+    np.testing.assert_allclose(orc.barycentric(p, a, b, c), uvw, atol=1e-12)
+
+
+def test_detection_sphere_on_block():
+    import workloads as W
+    m = W.grid_object(6, 6, 6, 0.01, pin_axis=2, noise=0.0, faces=True)
+    s = orc.Sim(m, iters=2, mode="gs")
+    r = 0.02
+    s.set_sdf(orc.SDF_SPHERE, [0.03, 0.03, 0.06 + r - 0.001], [0, 0, 0, 1], [r, 0, 0])
+    n = s.detect()
+    face, p, nrm, bary = s.contacts()
+    assert n > 0 and len(face) == n
+    ctr = np.array([0.03, 0.03, 0.06 + r - 0.001])
+    np.testing.assert_allclose(np.linalg.norm(p - ctr, axis=1), r, rtol=1e-9)     # p on the sphere
+    np.testing.assert_allclose(bary.sum(1), 1.0, atol=1e-12)
+    top = m.x0[m.faces[face]][:, :, 2]
+    assert np.all(top > 0.0599)                                                   # only top faces
+    s.set_sdf(orc.SDF_SPHERE, [0.03, 0.03, 0.2], [0, 0, 0, 1], [r, 0, 0])
+    assert s.detect() == 0

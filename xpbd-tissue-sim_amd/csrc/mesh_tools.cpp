@@ -89,40 +89,119 @@ int surface_faces(int nv, int nt, const int32_t* tets, std::vector<int32_t>& fac
     return 0;
 }
 
+namespace {
+constexpr int kWords = 4;  // up to 256 colours
+#ifndef XPBD_COLOR_PASSES
+#define XPBD_COLOR_PASSES 24
+#endif
+constexpr int kPasses = XPBD_COLOR_PASSES;
+using Mask = std::array<uint64_t, kWords>;
+
+inline bool has(const Mask& m, int c) { return (m[c / 64] >> (c % 64)) & 1ull; }
+
+// First-fit greedy in the given visiting order.  Returns the colour count.
+int first_fit(int nv, int n, int k, const int32_t* items, const std::vector<int32_t>& order,
+              std::vector<int32_t>& color)
+{
+    std::vector<Mask> used((size_t)nv);
+    for (auto& u : used) u.fill(0);
+    color.assign(n, -1);
+    int ncol = 0;
+    for (int e : order) {
+        Mask m{};
+        for (int j = 0; j < k; j++) {
+            const int32_t v = items[(size_t)k * e + j];
+            for (int w = 0; w < kWords; w++) m[w] |= used[v][w];
+        }
+        int c = -1;
+        for (int w = 0; w < kWords && c < 0; w++)
+            if (~m[w]) c = 64 * w + __builtin_ctzll(~m[w]);
+        if (c < 0) return -2;
+        color[e] = c;
+        for (int j = 0; j < k; j++) {
+            const int32_t v = items[(size_t)k * e + j];
+            used[v][c / 64] |= 1ull << (c % 64);
+        }
+        ncol = std::max(ncol, c + 1);
+    }
+    return ncol;
+}
+}  // namespace
+
 int greedy_color(int nv, int n_items, int verts_per_item, const int32_t* items,
                  std::vector<int32_t>& color)
 {
-    // per-vertex bit set of colours already used by an incident item
-    constexpr int WORDS = 4;  // up to 256 colours
-    std::vector<std::array<uint64_t, WORDS>> used((size_t)nv);
-    for (auto& u : used) u.fill(0);
-    color.assign(n_items, -1);
-    std::vector<int64_t> count;
-    int ncol = 0;
-    for (int e = 0; e < n_items; e++) {
-        std::array<uint64_t, WORDS> m{};
-        for (int j = 0; j < verts_per_item; j++) {
-            const int32_t v = items[(size_t)verts_per_item * e + j];
-            if (v < 0 || v >= nv) return -1;
-            for (int w = 0; w < WORDS; w++) m[w] |= used[v][w];
-        }
-        // balanced choice: the least-populated admissible colour; a new colour
-        // only when every existing one is blocked
-        int c = -1;
-        for (int k = 0; k < ncol; k++)
-            if (!((m[k / 64] >> (k % 64)) & 1ull) && (c < 0 || count[k] < count[c])) c = k;
-        if (c < 0) {
-            if (ncol == 64 * WORDS) return -2;
-            c = ncol++;
-            count.push_back(0);
-        }
-        count[c]++;
-        color[e] = c;
-        for (int j = 0; j < verts_per_item; j++) {
-            const int32_t v = items[(size_t)verts_per_item * e + j];
-            used[v][c / 64] |= 1ull << (c % 64);
+    const int k = verts_per_item;
+    for (size_t i = 0; i < (size_t)k * n_items; i++)
+        if (items[i] < 0 || items[i] >= nv) return -1;
+    std::vector<int32_t> order(n_items);
+    for (int e = 0; e < n_items; e++) order[e] = e;
+    int ncol = first_fit(nv, n_items, k, items, order, color);
+    if (ncol < 0 || n_items == 0) return ncol;
+
+    // Iterated greedy: re-colour class by class.  Visiting every class
+    // contiguously can never need more colours than before, and alternating
+    // the class order (reverse, largest first, fixed shuffle) removes colours.
+    uint64_t rs = 0x9E3779B97F4A7C15ull;
+    std::vector<int32_t> trial;
+    for (int pass = 0; pass < kPasses; pass++) {
+        std::vector<int64_t> cnt(ncol, 0);
+        for (int e = 0; e < n_items; e++) cnt[color[e]]++;
+        std::vector<int32_t> cls(ncol);
+        for (int c = 0; c < ncol; c++) cls[c] = c;
+        if (pass % 3 == 0) std::reverse(cls.begin(), cls.end());
+        else if (pass % 3 == 1) std::stable_sort(cls.begin(), cls.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+        else
+            for (int c = ncol - 1; c > 0; c--) {
+                rs ^= rs << 13; rs ^= rs >> 7; rs ^= rs << 17;
+                std::swap(cls[c], cls[rs % (uint64_t)(c + 1)]);
+            }
+        std::vector<int32_t> rank(ncol);
+        for (int c = 0; c < ncol; c++) rank[cls[c]] = c;
+        std::vector<int64_t> off(ncol + 1, 0);
+        for (int e = 0; e < n_items; e++) off[rank[color[e]] + 1]++;
+        for (int c = 0; c < ncol; c++) off[c + 1] += off[c];
+        for (int e = 0; e < n_items; e++) order[off[rank[color[e]]]++] = e;
+        const int nc = first_fit(nv, n_items, k, items, order, trial);
+        if (nc > 0 && nc <= ncol) {
+            ncol = nc;
+            color.swap(trial);
         }
     }
+
+    // Balance: move items from crowded classes to the least-populated class
+    // their vertices allow (never adds a colour).
+    std::vector<Mask> used((size_t)nv);
+    for (auto& u : used) u.fill(0);
+    std::vector<int64_t> cnt(ncol, 0);
+    for (int e = 0; e < n_items; e++) {
+        cnt[color[e]]++;
+        for (int j = 0; j < k; j++) {
+            const int32_t v = items[(size_t)k * e + j];
+            used[v][color[e] / 64] |= 1ull << (color[e] % 64);
+        }
+    }
+    for (int sweep = 0; sweep < 4; sweep++)
+        for (int e = 0; e < n_items; e++) {
+            const int c0 = color[e];
+            Mask m{};
+            for (int j = 0; j < k; j++) {
+                const int32_t v = items[(size_t)k * e + j];
+                for (int w = 0; w < kWords; w++) m[w] |= used[v][w];
+            }
+            int best = c0;
+            for (int c = 0; c < ncol; c++)
+                if (c != c0 && !has(m, c) && cnt[c] < cnt[best] - 1) best = c;
+            if (best == c0) continue;
+            for (int j = 0; j < k; j++) {
+                const int32_t v = items[(size_t)k * e + j];
+                used[v][c0 / 64] &= ~(1ull << (c0 % 64));
+                used[v][best / 64] |= 1ull << (best % 64);
+            }
+            cnt[c0]--;
+            cnt[best]++;
+            color[e] = best;
+        }
     return ncol;
 }
 

@@ -23,11 +23,12 @@ void grid_mesh(int nx, int ny, int nz, double hx, double hy, double hz,
 // Returns a negative value if some face is shared by more than two tets.
 int surface_faces(int nv, int nt, const int32_t* tets, std::vector<int32_t>& faces);
 
-// Greedy colouring of items (tets: 4 vertices, faces: 3 vertices) so that no
-// two items of one colour share a vertex.  Items are visited in index order and
-// take the least-populated colour not used at any of their vertices (a new
-// colour only when all are blocked), which keeps colour classes balanced.
-// Returns the number of colours, or a negative value on bad input.
+// Conflict colouring of items (tets: 4 vertices, faces: 3 vertices) so that no
+// two items of one colour share a vertex: first-fit greedy in index order,
+// then iterated greedy (re-colouring class by class in changing class orders,
+// which never adds a colour), then a balancing pass that moves items to the
+// least-populated admissible class.  Deterministic.  Returns the number of
+// colours, or a negative value on bad input.
 int greedy_color(int nv, int n_items, int verts_per_item, const int32_t* items,
                  std::vector<int32_t>& color);
 
