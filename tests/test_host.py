@@ -82,12 +82,20 @@ def test_msh_reader_fixture_counts(name, nv, nt, nf):
     x, t, f = X.read_msh(os.path.join(GOLDEN, "msh", name))
     assert (len(x), len(t), len(f)) == (nv, nt, nf)
     assert t.min() >= 0 and t.max() < nv
-    # the shipped surface triangles are the unshared faces on the box surface; these
-    # 5-tets-per-hex meshes also leave interior faces unshared (the diagonals of
-    # neighbouring hexes do not match), which face hashing reports too
-    fs = X.surface_faces(len(x), t)
-    outer = fs[_on_box_surface(x, fs)]
-    assert {tuple(sorted(r)) for r in outer} == {tuple(sorted(r)) for r in f}
+    D = np.stack([x[t[:, 0]] - x[t[:, 3]], x[t[:, 1]] - x[t[:, 3]], x[t[:, 2]] - x[t[:, 3]]], axis=2)
+    vol = np.abs(np.linalg.det(D)).sum() / 6
+    P = x[f]
+    area = 0.5 * np.linalg.norm(np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), axis=1).sum()
+    if name == "single.msh":
+        # one tet, its four faces listed as the surface
+        assert vol == pytest.approx(np.sqrt(3) * 1.73205080757 / 3, rel=1e-10)
+        assert {tuple(sorted(r)) for r in f} == {tuple(sorted(r)) for r in X.surface_faces(4, t)}
+    else:
+        # unit cube: tets fill it, triangles cover its surface
+        ext = x.max(0) - x.min(0)
+        assert vol == pytest.approx(np.prod(ext), rel=1e-12)
+        assert area == pytest.approx(2 * (ext[0] * ext[1] + ext[1] * ext[2] + ext[0] * ext[2]), rel=1e-12)
+        assert np.all(_on_box_surface(x, f))
 
 
 def test_msh_reader_missing_file():
