@@ -35,18 +35,21 @@ def parse():
     ap.add_argument("--workload", default="block", choices=["block", "beam", "cube", "batch"])
     ap.add_argument("--solver", default=None, choices=["gs", "jacobi"])
     ap.add_argument("--iters", type=int, default=None)
+    ap.add_argument("--variant", default="auto", choices=["auto", "simple", "cluster", "patch"])
+    ap.add_argument("--nondeterministic", action="store_true", help="Jacobi: LDS float atomics")
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
 
-def build_workload(name, solver, iters):
+def build_workload(name, solver, iters, variant="auto", det=1):
     import workloads as W
     import xpbd_amd as X
+    kw = dict(variant=variant, deterministic=det)
     if name == "block":
         mesh = W.block(56)
-        s = X.Solver(mesh, dt=1e-3, iters=iters or 20, solver=solver or "gs", collision_every=1)
+        s = X.Solver(mesh, dt=1e-3, iters=iters or 20, solver=solver or "gs", collision_every=1, **kw)
         kind, pos, quat, prm, vel = W.block_sphere(56)
         s.set_sdf(kind, pos, quat, prm)
         s.set_sdf_velocity(vel)
@@ -56,18 +59,18 @@ def build_workload(name, solver, iters):
         return s, [mesh], cfg, (kind, pos, quat, prm, vel)
     if name == "beam":
         mesh = W.beam(200, 20, 20)
-        s = X.Solver(mesh, dt=1e-3, iters=iters or 10, solver=solver or "gs", collision_every=0)
+        s = X.Solver(mesh, dt=1e-3, iters=iters or 10, solver=solver or "gs", collision_every=0, **kw)
         cfg = {"workload": "beam 200x20x20 cells x6 tets, E=1e4, nu=0.49", "tets": mesh.nt, "vertices": mesh.nv,
                "solver": solver or "gs", "iterations": iters or 10, "dt": 1e-3}
         return s, [mesh], cfg, None
     if name == "cube":
         mesh = W.cube(10)
-        s = X.Solver(mesh, dt=1e-3, iters=iters or 5, solver=solver or "gs", collision_every=0)
+        s = X.Solver(mesh, dt=1e-3, iters=iters or 5, solver=solver or "gs", collision_every=0, **kw)
         cfg = {"workload": "cube 10^3 cells x6 tets, E=3e3, nu=0.45", "tets": mesh.nt, "vertices": mesh.nv,
                "solver": solver or "gs", "iterations": iters or 5, "dt": 1e-3}
         return s, [mesh], cfg, None
     meshes, sdfs = W.batch(256)
-    s = X.Solver(meshes, dt=1e-3, iters=iters or 10, solver=solver or "jacobi", collision_every=1)
+    s = X.Solver(meshes, dt=1e-3, iters=iters or 10, solver=solver or "jacobi", collision_every=1, **kw)
     for i, (kind, pos, quat, prm) in enumerate(sdfs):
         s.set_sdf(kind, pos, quat, prm, instance=i)
     cfg = {"workload": "256 cubes of 4..24 cells per axis x6 tets, box contacts", "instances": 256,
@@ -105,7 +108,10 @@ def main():
                                    "described in DESIGN.md but not built)"}))
         sys.exit(2)
     import xpbd_amd as X  # noqa: F401  (fails loudly if the HIP library is missing)
-    s, meshes, cfg, sdf = build_workload(a.workload, a.solver, a.iters)
+    s, meshes, cfg, sdf = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
+    cfg["variant"] = a.variant
+    cfg["deterministic"] = not a.nondeterministic
+    cfg["colors"] = s.num_colors()
     T, V = s.nt, s.nv
     iters = cfg["iterations"]
 

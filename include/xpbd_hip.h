@@ -40,6 +40,17 @@ extern "C" {
 #define XPBD_SOLVER_JACOBI 0
 #define XPBD_SOLVER_GS_COLORED 1
 
+/* Kernel variants (xpbd_params.variant).  AUTO picks CLUSTER for Gauss-Seidel
+ * and PATCH for Jacobi.  SIMPLE: one thread per tet (Gauss-Seidel: one launch
+ * per tet colour; Jacobi: float atomics).  CLUSTER (Gauss-Seidel): runs of
+ * consecutive tets sharing at most 8 vertices form clusters, clusters are
+ * coloured, one thread projects its cluster's tets in order.  PATCH (Jacobi):
+ * workgroups stage a patch of tets and their vertices in LDS. */
+#define XPBD_VARIANT_AUTO 0
+#define XPBD_VARIANT_SIMPLE 1
+#define XPBD_VARIANT_CLUSTER 2
+#define XPBD_VARIANT_PATCH 3
+
 /* Signed-distance field kinds (src/geometry/SphereSDF.cpp, BoxSDF.cpp). */
 #define XPBD_SDF_NONE 0
 #define XPBD_SDF_SPHERE 1
@@ -78,6 +89,7 @@ typedef struct xpbd_params {
     int32_t solver;           /* XPBD_SOLVER_* */
     int32_t deterministic;    /* Jacobi only: 1 = fixed-order accumulation */
     int32_t collision_every;  /* re-detect contacts every k steps; 0 = never */
+    int32_t variant;          /* XPBD_VARIANT_* */
 } xpbd_params;
 
 /* ---- object life cycle ------------------------------------------------- */

@@ -16,46 +16,55 @@ TOL = 1e-5
 pytestmark = pytest.mark.gpu
 
 
-def _pair(mesh, solver, iters, dt=1e-3, g=9.81):
-    gpu = X.Solver(mesh, dt=dt, g=g, iters=iters, solver=solver, collision_every=1)
+def _pair(mesh, solver, iters, dt=1e-3, g=9.81, variant="auto", det=1):
+    gpu = X.Solver(mesh, dt=dt, g=g, iters=iters, solver=solver, collision_every=1, variant=variant,
+                   deterministic=det)
     o = orc.Sim(mesh, dt=dt, g=g, iters=iters, mode="gs" if solver == "gs" else "jacobi")
     if solver == "gs":
         o.set_order(gpu.tet_order())
     return gpu, o
 
 
-@pytest.mark.parametrize("solver,iters", [("jacobi", 10), ("gs", 5)])
-def test_cube_100_steps(solver, iters):
+# every kernel variant: GS one-thread-per-tet colours, GS clusters; Jacobi
+# atomics, Jacobi LDS patches (float-atomic and fixed-order accumulation)
+VARIANTS = [("gs", "simple", 1), ("gs", "cluster", 1), ("jacobi", "simple", 0), ("jacobi", "patch", 0),
+            ("jacobi", "patch", 1)]
+
+
+@pytest.mark.parametrize("solver,variant,det", VARIANTS)
+def test_cube_100_steps(solver, variant, det):
     mesh = W.cube(10, seed=11)
-    gpu, o = _pair(mesh, solver, iters)
+    gpu, o = _pair(mesh, solver, 10 if solver == "jacobi" else 5, variant=variant, det=det)
     errs = run_pair(gpu, o, 100)
     print(f"{solver}: max rel err {errs.max():.3e} final {errs[-1]:.3e}")
     assert np.all(np.isfinite(errs))
     assert errs.max() <= TOL
 
 
-@pytest.mark.parametrize("solver,E", [("gs", 1e4), ("jacobi", 1e3)])
-def test_stiff_beam_100_steps(solver, E):
+@pytest.mark.parametrize("solver,E,variant", [("gs", 1e4, "cluster"), ("gs", 1e4, "simple"),
+                                               ("jacobi", 1e3, "patch"), ("jacobi", 1e3, "simple")])
+def test_stiff_beam_100_steps(solver, E, variant):
     """nu = 0.49 beam.  Gauss-Seidel at the beam configuration's E = 1e4; the
     reference's plain-sum Jacobi diverges at that stiffness in FP64 (see
     test_reference_jacobi_diverges_on_stiff_beam), so Jacobi is compared at
     E = 1e3 where the reference algorithm is stable."""
     mesh = W.beam(24, 4, 4, seed=12, E=E, nu=0.49)
-    gpu, o = _pair(mesh, solver, 10)
+    gpu, o = _pair(mesh, solver, 10, variant=variant)
     errs = run_pair(gpu, o, 100)
     print(f"{solver} beam: max rel err {errs.max():.3e}")
     assert errs.max() <= TOL
 
 
-@pytest.mark.parametrize("solver,E_range", [("gs", (1e3, 5e4)), ("jacobi", (3e2, 1e3))])
-def test_heterogeneous_block_with_sphere(solver, E_range):
+@pytest.mark.parametrize("solver,E_range,variant", [("gs", (1e3, 5e4), "cluster"), ("gs", (1e3, 5e4), "simple"),
+                                                     ("jacobi", (3e2, 1e3), "patch"), ("jacobi", (3e2, 1e3), "simple")])
+def test_heterogeneous_block_with_sphere(solver, E_range, variant):
     """Per-tet E (log-uniform) and nu, base pinned, sphere pressed down at 0.1 m/s,
     contacts re-detected every step.  Jacobi again in its stable stiffness range."""
     n = 8
     rng = np.random.default_rng(5)
     E, nu = W.heterogeneous(6 * n ** 3, rng, E_range=E_range)
     mesh = W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.0, faces=True, rng=rng)
-    gpu, o = _pair(mesh, solver, 10)
+    gpu, o = _pair(mesh, solver, 10, variant=variant)
     r = 0.02
     L = 0.01 * n
     pos0, quat, prm, vel = (L / 2, L / 2, L + r - 0.002), (0, 0, 0, 1), (r, 0, 0), (0, 0, -0.1)
@@ -116,3 +125,23 @@ def test_end_of_solve_residual_matches_oracle():
         o.step(1)
         rg, ro = gpu.last_step_residual(), o.last_step_residual()
         assert abs(rg - ro) <= 1e-3 * ro + 1e-6
+
+
+def test_deterministic_patch_jacobi_is_bitwise_reproducible():
+    mesh = W.cube(12, seed=21, E=1e3, nu=0.45)
+    runs = []
+    for _ in range(2):
+        gpu = X.Solver(mesh, iters=10, solver="jacobi", variant="patch", deterministic=1)
+        gpu.step(30)
+        runs.append(gpu.positions())
+    np.testing.assert_array_equal(runs[0], runs[1])
+
+
+def test_cluster_order_is_a_valid_gauss_seidel_order():
+    """Clusters of one colour share no vertex: every tet appears once in the
+    order, and the oracle replaying that order matches (checked above)."""
+    mesh = W.cube(10, seed=11)
+    gpu = X.Solver(mesh, iters=1, solver="gs", variant="cluster")
+    order = gpu.tet_order()
+    assert sorted(order) == list(range(mesh.nt))
+    assert gpu.num_colors() <= 12          # cells of a grid need 8 colours

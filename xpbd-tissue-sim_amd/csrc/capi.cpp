@@ -72,6 +72,25 @@ TetArrays tet_arrays(xpbd_handle* h)
 
 bool contacts_enabled(const xpbd_handle* h) { return h->nf > 0 && h->sdf_active; }
 
+PatchArrays patch_arrays(xpbd_handle* h)
+{
+    PatchArrays pa;
+    pa.tloc = reinterpret_cast<const uint2*>(h->d_ptloc);
+    pa.pvoff = h->d_pvoff;
+    pa.pverts = h->d_pverts;
+    pa.pslot = h->d_pslot;
+    pa.inc_off = h->d_inc_off;
+    pa.inc = h->d_inc;
+    pa.partial = h->d_partial;
+    pa.shared = h->d_shared;
+    pa.soff = h->d_soff;
+    pa.sslot = h->d_sslot;
+    pa.npatch = h->patches.npatch;
+    pa.maxl = h->patches.maxl;
+    pa.nshared = (int)h->patches.shared.size();
+    return pa;
+}
+
 }  // namespace xpbd
 
 // ---------------------------------------------------------------------------
@@ -227,7 +246,49 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     }
     // ---- solver order of elements
     h->order.resize(nt);
-    if (P->solver == XPBD_SOLVER_GS_COLORED) {
+    int variant = P->variant;
+    if (variant == XPBD_VARIANT_AUTO)
+        variant = P->solver == XPBD_SOLVER_GS_COLORED ? XPBD_VARIANT_CLUSTER : XPBD_VARIANT_PATCH;
+    if ((P->solver == XPBD_SOLVER_GS_COLORED && variant == XPBD_VARIANT_PATCH) ||
+        (P->solver == XPBD_SOLVER_JACOBI && variant == XPBD_VARIANT_CLUSTER) || variant < 0 || variant > 3) {
+        delete h;
+        return set_err(nullptr, XPBD_E_INVALID_ARG, "variant does not apply to this solver");
+    }
+    h->variant = variant;
+    std::vector<uint32_t> tloc_solver;
+    std::vector<int32_t> cverts_solver;
+    if (P->solver == XPBD_SOLVER_GS_COLORED && variant == XPBD_VARIANT_CLUSTER) {
+        // clusters of consecutive tets, coloured; solver order = colour, cluster, tet
+        std::vector<int32_t> cst, cv, ccol;
+        std::vector<uint32_t> tl;
+        const int ncl = build_clusters(nt, tets.data(), cst, cv, tl);
+        const int nc = greedy_color(nv, ncl, kClusterVerts, cv.data(), ccol);
+        if (nc <= 0) {
+            delete h;
+            return set_err(nullptr, XPBD_E_INVALID_ARG, "cluster colouring failed");
+        }
+        std::vector<int32_t> cnt(nc + 1, 0);
+        for (int c = 0; c < ncl; c++) cnt[ccol[c] + 1]++;
+        for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
+        h->color_off.assign(cnt.begin(), cnt.end());  // offsets over clusters
+        std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1), corder(ncl);
+        for (int c = 0; c < ncl; c++) corder[pos[ccol[c]]++] = c;
+        h->cl_start.assign(1, 0);
+        cverts_solver.reserve((size_t)kClusterVerts * ncl);
+        tloc_solver.reserve(nt);
+        int at_pos = 0;
+        for (int i = 0; i < ncl; i++) {
+            const int c = corder[i];
+            for (int e = cst[c]; e < cst[c + 1]; e++) {
+                h->order[at_pos++] = e;
+                tloc_solver.push_back(tl[e]);
+            }
+            h->cl_start.push_back(at_pos);
+            for (int j = 0; j < kClusterVerts; j++) cverts_solver.push_back(cv[(size_t)kClusterVerts * c + j]);
+        }
+        h->ncolors = nc;
+        h->nclusters = ncl;
+    } else if (P->solver == XPBD_SOLVER_GS_COLORED) {
         std::vector<int32_t> col;
         int nc = greedy_color(nv, nt, 4, tets.data(), col);
         if (nc <= 0) {
@@ -241,6 +302,21 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
         std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
         for (int e = 0; e < nt; e++) h->order[pos[col[e]]++] = e;
         h->ncolors = nc;
+    } else if (variant == XPBD_VARIANT_PATCH) {
+        std::vector<int32_t> tinst(nt);
+        for (int32_t i = 0; i < n; i++)
+            for (int64_t e = h->t_off[i]; e < h->t_off[i + 1]; e++) tinst[e] = i;
+        const int maxl = build_patches(nv, nt, tets.data(), h->rest.data(), tinst.data(), jacobi_patch_size(),
+                                       h->patches);
+        if (maxl < 0 || jacobi_patch_smem(maxl, true) > 64 * 1024) {
+            // pathological vertex spread: fall back to one thread per tet
+            h->variant = XPBD_VARIANT_SIMPLE;
+            std::iota(h->order.begin(), h->order.end(), 0);
+        } else {
+            h->order = h->patches.order;
+        }
+        h->color_off = { 0, nt };
+        h->ncolors = 1;
     } else {
         std::iota(h->order.begin(), h->order.end(), 0);
         h->color_off = { 0, nt };
@@ -286,6 +362,24 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     rc |= dalloc(h, &h->d_at, nt);
     rc |= dalloc(h, &h->d_gamma, nt);
     rc |= dalloc(h, &h->d_lam, nt);
+    if (h->variant == XPBD_VARIANT_PATCH) {
+        const Patches& Pt = h->patches;
+        rc |= dalloc(h, &h->d_ptloc, Pt.tloc.size());  // 2 words (4 x 16-bit slots) per tet
+        rc |= dalloc(h, &h->d_pvoff, Pt.pvoff.size());
+        rc |= dalloc(h, &h->d_pverts, Pt.pverts.size());
+        rc |= dalloc(h, &h->d_pslot, Pt.pslot.size());
+        rc |= dalloc(h, &h->d_inc_off, Pt.inc_off.size());
+        rc |= dalloc(h, &h->d_inc, Pt.inc.size());
+        rc |= dalloc(h, &h->d_partial, (size_t)Pt.nslots);
+        rc |= dalloc(h, &h->d_shared, Pt.shared.size());
+        rc |= dalloc(h, &h->d_soff, Pt.soff.size());
+        rc |= dalloc(h, &h->d_sslot, Pt.sslot.size());
+    }
+    if (h->nclusters > 0) {
+        rc |= dalloc(h, &h->d_cverts, 2 * (size_t)h->nclusters);
+        rc |= dalloc(h, &h->d_cstart, (size_t)h->nclusters + 1);
+        rc |= dalloc(h, &h->d_tloc, nt);
+    }
     h->red_blocks = std::max(1, std::min(1024, (nt + 255) / 256));
     rc |= dalloc(h, &h->d_part, h->red_blocks);
     rc |= dalloc(h, &h->d_scalar, 2);
@@ -349,6 +443,25 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     UP(h->d_at, hat.data(), sizeof(float2) * nt);
     UP(h->d_gamma, hg.data(), sizeof(float) * nt);
     UP(h->d_sdf, h->sdf_host.data(), sizeof(SdfDev) * n);
+    if (h->variant == XPBD_VARIANT_PATCH) {
+        const Patches& Pt = h->patches;
+        UP(h->d_ptloc, Pt.tloc.data(), sizeof(uint32_t) * Pt.tloc.size());
+        UP(h->d_pvoff, Pt.pvoff.data(), sizeof(int32_t) * Pt.pvoff.size());
+        UP(h->d_pverts, Pt.pverts.data(), sizeof(int32_t) * Pt.pverts.size());
+        UP(h->d_pslot, Pt.pslot.data(), sizeof(int32_t) * Pt.pslot.size());
+        UP(h->d_inc_off, Pt.inc_off.data(), sizeof(int32_t) * Pt.inc_off.size());
+        UP(h->d_inc, Pt.inc.data(), sizeof(uint16_t) * Pt.inc.size());
+        if (!Pt.shared.empty()) {
+            UP(h->d_shared, Pt.shared.data(), sizeof(int32_t) * Pt.shared.size());
+            UP(h->d_sslot, Pt.sslot.data(), sizeof(int32_t) * Pt.sslot.size());
+        }
+        UP(h->d_soff, Pt.soff.data(), sizeof(int32_t) * Pt.soff.size());
+    }
+    if (h->nclusters > 0) {
+        UP(h->d_cverts, cverts_solver.data(), sizeof(int32_t) * cverts_solver.size());
+        UP(h->d_cstart, h->cl_start.data(), sizeof(int32_t) * h->cl_start.size());
+        UP(h->d_tloc, tloc_solver.data(), sizeof(uint32_t) * tloc_solver.size());
+    }
     if (nf > 0) UP(h->d_faces, hf.data(), sizeof(int4) * nf);
     UP(h->d_fcol_off, h->fcol_off.data(), sizeof(int32_t) * h->fcol_off.size());
 #undef UP
@@ -419,7 +532,11 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
         if (h->params.solver == XPBD_SOLVER_GS_COLORED) {
             for (int c = 0; c < h->ncolors; c++) {
                 if (timed) timed_begin(h);
-                e = launch_gs_color(s, ta, h->d_pos, h->color_off[c], h->color_off[c + 1], first);
+                if (h->variant == XPBD_VARIANT_CLUSTER)
+                    e = launch_gs_cluster(s, ta, h->d_cverts, h->d_cstart, h->d_tloc, h->d_pos, h->color_off[c],
+                                          h->color_off[c + 1], first);
+                else
+                    e = launch_gs_color(s, ta, h->d_pos, h->color_off[c], h->color_off[c + 1], first);
                 if (e != hipSuccess) return hip_err(h, e, "gs_color launch");
                 if (timed) timed_end(h);
             }
@@ -428,6 +545,18 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
                                        h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
                 if (e != hipSuccess) return hip_err(h, e, "contacts_gs launch");
             }
+        } else if (h->variant == XPBD_VARIANT_PATCH) {
+            if (con) {
+                // contact corrections read the pre-iteration positions, so they run first
+                e = launch_contacts_jacobi(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
+                                           h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
+                if (e != hipSuccess) return hip_err(h, e, "contacts_jacobi launch");
+            }
+            if (timed) timed_begin(h);
+            e = launch_jacobi_patch(s, ta, patch_arrays(h), h->d_pos, reinterpret_cast<float4*>(h->d_acc), h->nt,
+                                    first, h->params.deterministic != 0, con);
+            if (e != hipSuccess) return hip_err(h, e, "jacobi_patch launch");
+            if (timed) timed_end(h);
         } else {
             if (timed) timed_begin(h);
             e = launch_jacobi_project(s, ta, h->d_pos, h->d_acc, h->nt, first);

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "mesh_tools.h"
 #include "xpbd_hip.h"
 
 namespace xpbd {
@@ -26,7 +27,12 @@ struct xpbd_handle {
     std::vector<uint8_t> pinned;
     std::vector<int32_t> order;                // solver position -> caller's tet index
     std::vector<int32_t> color_off;            // colour c occupies [color_off[c], color_off[c+1])
+                                               // (tets, or clusters for the cluster variant)
     int ncolors = 1;
+    int variant = XPBD_VARIANT_SIMPLE;         // resolved kernel variant
+    std::vector<int32_t> cl_start;             // cluster variant: tet range of each cluster (solver order)
+    int nclusters = 0;
+    xpbd::Patches patches;                     // patch variant (host copy)
     std::vector<int32_t> fperm;                // solver face position -> caller's face index
     std::vector<int32_t> fcol_off;
     int nfcolors = 1;
@@ -44,6 +50,19 @@ struct xpbd_handle {
     float2* d_at = nullptr;
     float* d_gamma = nullptr;
     float2* d_lam = nullptr;
+    int4* d_cverts = nullptr;    // cluster variant: 8 vertex ids per cluster (2 x int4)
+    int32_t* d_cstart = nullptr;
+    uint32_t* d_tloc = nullptr;
+    uint32_t* d_ptloc = nullptr; // patch variant (see PatchArrays)
+    int32_t* d_pvoff = nullptr;
+    int32_t* d_pverts = nullptr;
+    int32_t* d_pslot = nullptr;
+    int32_t* d_inc_off = nullptr;
+    uint16_t* d_inc = nullptr;
+    float4* d_partial = nullptr;
+    int32_t* d_shared = nullptr;
+    int32_t* d_soff = nullptr;
+    int32_t* d_sslot = nullptr;
     double* d_part = nullptr;
     int red_blocks = 1;
     double* d_scalar = nullptr;
@@ -86,6 +105,7 @@ int set_err(xpbd_handle* h, int code, const std::string& msg);
 int hip_err(xpbd_handle* h, hipError_t e, const char* what);
 void invalidate_graphs(xpbd_handle* h);
 TetArrays tet_arrays(xpbd_handle* h);
+PatchArrays patch_arrays(xpbd_handle* h);
 bool contacts_enabled(const xpbd_handle* h);
 int enqueue_iterations(xpbd_handle* h, int iters, bool timed);
 int enqueue_detect(xpbd_handle* h);

@@ -34,9 +34,35 @@ hipError_t launch_predict(hipStream_t s, float4* pos, const float4* vel, int nv,
 hipError_t launch_finish(hipStream_t s, float4* pos, float4* lo, float4* vel, int nv, float dt,
                          unsigned long long* step_counter);
 hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int begin, int end, bool first);
+// Cluster Gauss-Seidel over clusters [cbeg, cend) of one colour: cverts holds 8
+// vertex ids per cluster (-1 padded), cstart the tet range of each cluster,
+// tloc the 4-bit local slots of each tet's vertices.
+hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
+                             const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first);
 hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float4* pos, float* acc, int nt,
                                  bool first);
 hipError_t launch_jacobi_apply(hipStream_t s, float4* pos, float* acc, int nv);
+
+// Patch Jacobi (see k_jacobi_patch): device views of the host Patches layout.
+struct PatchArrays {
+    const uint2* tloc;       // 4 x 16-bit local vertex ids per tet (solver order)
+    const int* pvoff;        // patch -> range in pverts
+    const int* pverts;       // distinct vertices of each patch
+    const int* pslot;        // -1 interior, else partial slot
+    const int* inc_off;      // per patch local incidence offsets (deterministic mode)
+    const uint16_t* inc;     // per patch local incidence entries
+    float4* partial;         // partial sums of shared vertices
+    const int* shared;       // shared vertex ids
+    const int* soff;         // shared vertex -> slot range
+    const int* sslot;        // slots
+    int npatch, maxl, nshared;
+};
+int jacobi_patch_size();
+size_t jacobi_patch_smem(int maxl, bool det);
+// One Jacobi iteration: patch kernel + shared-vertex kernel.  cacc holds contact
+// corrections (float4 per vertex) when con is set; it is consumed and zeroed.
+hipError_t launch_jacobi_patch(hipStream_t s, const TetArrays& ta, const PatchArrays& pa, float4* pos, float4* cacc,
+                               int nt, bool first, bool det, bool con);
 hipError_t launch_fill_zero(hipStream_t s, void* p, size_t bytes);
 
 // FP64 residual r = C + alpha_tilde * lambda per element, block partial sums of

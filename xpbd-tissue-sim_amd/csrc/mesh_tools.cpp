@@ -111,6 +111,7 @@ int first_fit(int nv, int n, int k, const int32_t* items, const std::vector<int3
         Mask m{};
         for (int j = 0; j < k; j++) {
             const int32_t v = items[(size_t)k * e + j];
+            if (v < 0) continue;
             for (int w = 0; w < kWords; w++) m[w] |= used[v][w];
         }
         int c = -1;
@@ -120,7 +121,7 @@ int first_fit(int nv, int n, int k, const int32_t* items, const std::vector<int3
         color[e] = c;
         for (int j = 0; j < k; j++) {
             const int32_t v = items[(size_t)k * e + j];
-            used[v][c / 64] |= 1ull << (c % 64);
+            if (v >= 0) used[v][c / 64] |= 1ull << (c % 64);
         }
         ncol = std::max(ncol, c + 1);
     }
@@ -133,7 +134,7 @@ int greedy_color(int nv, int n_items, int verts_per_item, const int32_t* items,
 {
     const int k = verts_per_item;
     for (size_t i = 0; i < (size_t)k * n_items; i++)
-        if (items[i] < 0 || items[i] >= nv) return -1;
+        if (items[i] < -1 || items[i] >= nv) return -1;
     std::vector<int32_t> order(n_items);
     for (int e = 0; e < n_items; e++) order[e] = e;
     int ncol = first_fit(nv, n_items, k, items, order, color);
@@ -178,7 +179,7 @@ int greedy_color(int nv, int n_items, int verts_per_item, const int32_t* items,
         cnt[color[e]]++;
         for (int j = 0; j < k; j++) {
             const int32_t v = items[(size_t)k * e + j];
-            used[v][color[e] / 64] |= 1ull << (color[e] % 64);
+            if (v >= 0) used[v][color[e] / 64] |= 1ull << (color[e] % 64);
         }
     }
     for (int sweep = 0; sweep < 4; sweep++)
@@ -187,7 +188,8 @@ int greedy_color(int nv, int n_items, int verts_per_item, const int32_t* items,
             Mask m{};
             for (int j = 0; j < k; j++) {
                 const int32_t v = items[(size_t)k * e + j];
-                for (int w = 0; w < kWords; w++) m[w] |= used[v][w];
+                if (v >= 0)
+                    for (int w = 0; w < kWords; w++) m[w] |= used[v][w];
             }
             int best = c0;
             for (int c = 0; c < ncol; c++)
@@ -195,6 +197,7 @@ int greedy_color(int nv, int n_items, int verts_per_item, const int32_t* items,
             if (best == c0) continue;
             for (int j = 0; j < k; j++) {
                 const int32_t v = items[(size_t)k * e + j];
+                if (v < 0) continue;
                 used[v][c0 / 64] &= ~(1ull << (c0 % 64));
                 used[v][best / 64] |= 1ull << (best % 64);
             }
@@ -203,6 +206,158 @@ int greedy_color(int nv, int n_items, int verts_per_item, const int32_t* items,
             color[e] = best;
         }
     return ncol;
+}
+
+int build_clusters(int nt, const int32_t* tets, std::vector<int32_t>& start, std::vector<int32_t>& verts,
+                   std::vector<uint32_t>& tloc)
+{
+    start.assign(1, 0);
+    verts.clear();
+    tloc.assign(nt, 0u);
+    int32_t cv[kClusterVerts];
+    int nvc = 0, ntc = 0;
+    auto flush = [&]() {
+        for (int j = 0; j < kClusterVerts; j++) verts.push_back(j < nvc ? cv[j] : -1);
+        start.push_back(start.back() + ntc);
+        nvc = ntc = 0;
+    };
+    for (int e = 0; e < nt; e++) {
+        const int32_t* t = tets + 4 * (size_t)e;
+        // vertices of t not yet in the cluster
+        int add = 0;
+        for (int j = 0; j < 4; j++) {
+            bool found = false;
+            for (int q = 0; q < nvc; q++) found = found || cv[q] == t[j];
+            for (int q = 0; q < j; q++) found = found || t[q] == t[j];
+            add += !found;
+        }
+        if (ntc > 0 && (nvc + add > kClusterVerts || ntc == kClusterTets)) flush();
+        uint32_t loc = 0;
+        for (int j = 0; j < 4; j++) {
+            int q = 0;
+            while (q < nvc && cv[q] != t[j]) q++;
+            if (q == nvc) cv[nvc++] = t[j];
+            loc |= (uint32_t)q << (4 * j);
+        }
+        tloc[e] = loc;
+        ntc++;
+    }
+    if (ntc > 0) flush();
+    return (int)start.size() - 1;
+}
+
+namespace {
+uint64_t spread3(uint64_t v)  // 21 bits -> every third bit
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+}  // namespace
+
+int build_patches(int nv, int nt, const int32_t* tets, const double* x, const int32_t* tet_instance, int patch,
+                  Patches& P)
+{
+    P.patch = patch;
+    // ---- order: (instance, Morton code of the centroid)
+    double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
+    for (int v = 0; v < nv; v++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], x[3 * (size_t)v + k]);
+            hi[k] = std::max(hi[k], x[3 * (size_t)v + k]);
+        }
+    std::vector<std::pair<uint64_t, int32_t>> key(nt);
+    for (int e = 0; e < nt; e++) {
+        uint64_t m = 0;
+        for (int k = 0; k < 3; k++) {
+            double c = 0;
+            for (int j = 0; j < 4; j++) c += x[3 * (size_t)tets[4 * (size_t)e + j] + k];
+            c *= 0.25;
+            const double span = hi[k] - lo[k];
+            const uint64_t q = span > 0 ? (uint64_t)std::min(1048575.0, (c - lo[k]) / span * 1048575.0) : 0;
+            m |= spread3(q) << k;
+        }
+        // with instances: 20 bits of instance id above the top 44 bits of the code
+        const uint64_t inst = tet_instance ? (uint64_t)tet_instance[e] : 0;
+        key[e] = { tet_instance ? ((inst << 44) | (m >> 16)) : m, e };
+    }
+    std::stable_sort(key.begin(), key.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    P.order.resize(nt);
+    for (int i = 0; i < nt; i++) P.order[i] = key[i].second;
+    P.npatch = (nt + patch - 1) / patch;
+    // ---- per-patch local vertices
+    std::vector<int32_t> owner_cnt(nv, 0);  // number of patches touching each vertex
+    P.pvoff.assign(1, 0);
+    P.pverts.clear();
+    P.tloc.assign(2 * (size_t)nt, 0u);
+    P.maxl = 0;
+    std::vector<int32_t> loc;
+    for (int p = 0; p < P.npatch; p++) {
+        const int b = p * patch, e = std::min(nt, b + patch);
+        loc.clear();
+        for (int i = b; i < e; i++)
+            for (int j = 0; j < 4; j++) loc.push_back(tets[4 * (size_t)P.order[i] + j]);
+        std::sort(loc.begin(), loc.end());
+        loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+        if (loc.size() > 65535) return -1;
+        for (int i = b; i < e; i++)
+            for (int j = 0; j < 4; j++) {
+                const int32_t g = tets[4 * (size_t)P.order[i] + j];
+                const uint32_t l = (uint32_t)(std::lower_bound(loc.begin(), loc.end(), g) - loc.begin());
+                P.tloc[2 * (size_t)i + j / 2] |= l << (16 * (j % 2));
+            }
+        for (int32_t g : loc) owner_cnt[g]++;
+        P.pverts.insert(P.pverts.end(), loc.begin(), loc.end());
+        P.pvoff.push_back((int32_t)P.pverts.size());
+        P.maxl = std::max(P.maxl, (int)loc.size());
+    }
+    // ---- slots for shared vertices (touched by more than one patch)
+    P.pslot.assign(P.pverts.size(), -1);
+    P.nslots = 0;
+    std::vector<std::vector<int32_t>> vslots;  // only for shared vertices, indexed via map below
+    std::vector<int32_t> shared_id(nv, -1);
+    P.shared.clear();
+    for (int p = 0; p < P.npatch; p++)
+        for (int j = P.pvoff[p]; j < P.pvoff[p + 1]; j++) {
+            const int32_t g = P.pverts[j];
+            if (owner_cnt[g] < 2) continue;
+            if (shared_id[g] < 0) {
+                shared_id[g] = (int32_t)P.shared.size();
+                P.shared.push_back(g);
+                vslots.emplace_back();
+            }
+            P.pslot[j] = P.nslots;
+            vslots[shared_id[g]].push_back(P.nslots++);
+        }
+    P.soff.assign(1, 0);
+    P.sslot.clear();
+    for (auto& v : vslots) {
+        P.sslot.insert(P.sslot.end(), v.begin(), v.end());
+        P.soff.push_back((int32_t)P.sslot.size());
+    }
+    // ---- per-patch local incidence lists (local vertex -> tet corners)
+    P.inc_off.clear();
+    P.inc.assign((size_t)P.npatch * 4 * patch, 0);
+    for (int p = 0; p < P.npatch; p++) {
+        const int b = p * patch, e = std::min(nt, b + patch);
+        const int nl = P.pvoff[p + 1] - P.pvoff[p];
+        std::vector<int32_t> cnt(nl + 1, 0);
+        for (int i = b; i < e; i++)
+            for (int j = 0; j < 4; j++) cnt[((P.tloc[2 * (size_t)i + j / 2] >> (16 * (j % 2))) & 0xffff) + 1]++;
+        for (int l = 0; l < nl; l++) cnt[l + 1] += cnt[l];
+        P.inc_off.insert(P.inc_off.end(), cnt.begin(), cnt.end());
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (int i = b; i < e; i++)
+            for (int j = 0; j < 4; j++) {
+                const int l = (P.tloc[2 * (size_t)i + j / 2] >> (16 * (j % 2))) & 0xffff;
+                P.inc[(size_t)p * 4 * patch + fill[l]++] = (uint16_t)((i - b) * 4 + j);
+            }
+    }
+    return P.maxl;
 }
 
 int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int32_t>& tets,

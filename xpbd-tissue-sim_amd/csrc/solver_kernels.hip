@@ -133,6 +133,82 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(TetArrays ta, float4* __res
 }
 
 // ---------------------------------------------------------------------------
+// Cluster Gauss-Seidel: one thread per cluster (up to 8 vertices, up to 8
+// tets, consecutive in solver order).  Clusters of one colour share no vertex.
+// The thread stages its cluster's vertices in its own LDS column (slot-major,
+// so lane t always hits bank t), projects the tets in order, and writes the
+// vertices back: sequential Gauss-Seidel in (colour, cluster, tet) order.
+// No barrier is needed because no thread reads another thread's column.
+// ---------------------------------------------------------------------------
+constexpr int kClusterBlock = 64;
+
+template <bool FIRST>
+__global__ __launch_bounds__(kClusterBlock) void k_gs_cluster(TetArrays ta, const int4* __restrict__ cverts,
+                                                             const int* __restrict__ cstart,
+                                                             const uint32_t* __restrict__ tloc,
+                                                             float4* __restrict__ pos, int cbeg, int cend)
+{
+    __shared__ float sb[8][4][kClusterBlock];  // base xyz, inverse mass
+    __shared__ float sd[8][3][kClusterBlock];  // in-step motion
+    const int tid = threadIdx.x;
+    const int c = cbeg + blockIdx.x * kClusterBlock + tid;
+    if (c >= cend) return;
+    const int4 va = cverts[2 * (size_t)c], vb = cverts[2 * (size_t)c + 1];
+    const int vid[8] = { va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w };
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (vid[q] < 0) continue;
+        const float4 b = pos[2 * (size_t)vid[q]];
+        const float4 d = pos[2 * (size_t)vid[q] + 1];
+        sb[q][0][tid] = b.x;
+        sb[q][1][tid] = b.y;
+        sb[q][2][tid] = b.z;
+        sb[q][3][tid] = b.w;
+        sd[q][0][tid] = d.x;
+        sd[q][1][tid] = d.y;
+        sd[q][2][tid] = d.z;
+    }
+    const int t0 = cstart[c], t1 = cstart[c + 1];
+    for (int e = t0; e < t1; e++) {
+        const uint32_t loc = tloc[e];
+        float bb[12], dd[12], w[4];
+        int sl[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            sl[j] = (loc >> (4 * j)) & 15u;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                bb[3 * j + k] = sb[sl[j]][k][tid];
+                dd[3 * j + k] = sd[sl[j]][k][tid];
+            }
+            w[j] = sb[sl[j]][3][tid];
+        }
+        float q[9];
+        float2 at;
+        float gm;
+        load_tet(ta, e, q, at, gm);
+        float l[2] = { 0.0f, 0.0f };
+        if (!FIRST) {
+            const float2 lv = ta.lam[e];
+            l[0] = lv.x;
+            l[1] = lv.y;
+        }
+        float dx[12];
+        project_tet_split<float>(bb, dd, w, q, at.x, at.y, gm, l, dx);
+        ta.lam[e] = make_float2(l[0], l[1]);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) sd[sl[j]][k][tid] = dd[3 * j + k] + dx[3 * j + k];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (vid[q] < 0) continue;
+        pos[2 * (size_t)vid[q] + 1] = make_float4(sd[q][0][tid], sd[q][1][tid], sd[q][2][tid], 0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Jacobi: every projection reads the same positions; corrections are summed
 // into acc (4 floats per vertex) with hardware float atomics, then applied.
 // ---------------------------------------------------------------------------
@@ -176,6 +252,150 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_apply(float4* __restrict__ po
     d.z += a.z;
     pos[2 * (size_t)i + 1] = d;
     acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Patch Jacobi.  A workgroup owns a patch of kPatch consecutive (Morton-sorted)
+// tets: it stages the patch's distinct vertices in LDS, each thread projects
+// one tet reading LDS, the corrections are summed per local vertex in LDS, and
+// then every local vertex is written once: interior vertices (no tet outside
+// the patch touches them, so no other workgroup reads them this iteration)
+// are updated in place, shared vertices leave their partial sum in a fixed
+// slot that k_jacobi_shared adds in fixed slot order.
+//   DET   sum corrections with a fixed-order gather over the patch's local
+//         incidence lists instead of LDS float atomics (bitwise reproducible)
+//   CON   fold in the contact corrections accumulated in cacc
+// ---------------------------------------------------------------------------
+constexpr int kPatch = 256;
+
+template <bool FIRST, bool DET, bool CON>
+__global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uint2* __restrict__ tloc,
+                                                        const int* __restrict__ pvoff, const int* __restrict__ pverts,
+                                                        const int* __restrict__ pslot, const int* __restrict__ inc_off,
+                                                        const uint16_t* __restrict__ inc, float4* __restrict__ pos,
+                                                        float4* __restrict__ partial, float4* __restrict__ cacc,
+                                                        int nt, int maxl)
+{
+    extern __shared__ float4 smem[];
+    float4* lb = smem;                                   // base, inverse mass
+    float* ld = reinterpret_cast<float*>(smem + maxl);   // in-step motion, 3 per local vertex
+    float* red = ld + 3 * maxl;                          // DET: 12 planes of kPatch; else 3 per local vertex
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x;
+    const int v0 = pvoff[p], nl = pvoff[p + 1] - v0;
+    for (int i = tid; i < nl; i += kPatch) {
+        const int g = pverts[v0 + i];
+        lb[i] = pos[2 * (size_t)g];
+        const float4 d = pos[2 * (size_t)g + 1];
+        ld[3 * i] = d.x;
+        ld[3 * i + 1] = d.y;
+        ld[3 * i + 2] = d.z;
+        if (!DET) red[3 * i] = red[3 * i + 1] = red[3 * i + 2] = 0.f;
+    }
+    __syncthreads();
+    const int e = p * kPatch + tid;
+    if (e < nt) {
+        const uint2 lc = tloc[e];
+        const int sl[4] = { (int)(lc.x & 0xffffu), (int)(lc.x >> 16), (int)(lc.y & 0xffffu), (int)(lc.y >> 16) };
+        float bb[12], dd[12], w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float4 b = lb[sl[j]];
+            bb[3 * j] = b.x;
+            bb[3 * j + 1] = b.y;
+            bb[3 * j + 2] = b.z;
+            w[j] = b.w;
+#pragma unroll
+            for (int k = 0; k < 3; k++) dd[3 * j + k] = ld[3 * sl[j] + k];
+        }
+        float q[9];
+        float2 at;
+        float gm;
+        load_tet(ta, e, q, at, gm);
+        float l[2] = { 0.0f, 0.0f };
+        if (!FIRST) {
+            const float2 lv = ta.lam[e];
+            l[0] = lv.x;
+            l[1] = lv.y;
+        }
+        float dx[12];
+        project_tet_split<float>(bb, dd, w, q, at.x, at.y, gm, l, dx);
+        ta.lam[e] = make_float2(l[0], l[1]);
+        if (DET) {
+#pragma unroll
+            for (int k = 0; k < 12; k++) red[k * kPatch + tid] = dx[k];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) atomicAdd(red + 3 * sl[j] + k, dx[3 * j + k]);
+        }
+    }
+    __syncthreads();
+    const int ioff = v0 + p;  // this patch's nl+1 incidence offsets
+    for (int i = tid; i < nl; i += kPatch) {
+        float a0, a1, a2;
+        if (DET) {
+            a0 = a1 = a2 = 0.f;
+            const int s0 = inc_off[ioff + i], s1 = inc_off[ioff + i + 1];
+            const uint16_t* il = inc + (size_t)p * 4 * kPatch;
+            for (int s = s0; s < s1; s++) {
+                const int ent = il[s];
+                const int t = ent >> 2, j = ent & 3;
+                a0 += red[(3 * j) * kPatch + t];
+                a1 += red[(3 * j + 1) * kPatch + t];
+                a2 += red[(3 * j + 2) * kPatch + t];
+            }
+        } else {
+            a0 = red[3 * i];
+            a1 = red[3 * i + 1];
+            a2 = red[3 * i + 2];
+        }
+        const int g = pverts[v0 + i];
+        const int slot = pslot[v0 + i];
+        if (slot < 0) {
+            if (CON) {
+                const float4 c = cacc[g];
+                a0 += c.x;
+                a1 += c.y;
+                a2 += c.z;
+                cacc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            pos[2 * (size_t)g + 1] = make_float4(ld[3 * i] + a0, ld[3 * i + 1] + a1, ld[3 * i + 2] + a2, 0.f);
+        } else {
+            partial[slot] = make_float4(a0, a1, a2, 0.f);
+        }
+    }
+}
+
+template <bool CON>
+__global__ __launch_bounds__(kBlock) void k_jacobi_shared(const int* __restrict__ shared, const int* __restrict__ soff,
+                                                         const int* __restrict__ sslot,
+                                                         const float4* __restrict__ partial,
+                                                         float4* __restrict__ cacc, float4* __restrict__ pos, int ns)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= ns) return;
+    const int g = shared[j];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int k = soff[j]; k < soff[j + 1]; k++) {
+        const float4 v = partial[sslot[k]];
+        a0 += v.x;
+        a1 += v.y;
+        a2 += v.z;
+    }
+    if (CON) {
+        const float4 c = cacc[g];
+        a0 += c.x;
+        a1 += c.y;
+        a2 += c.z;
+        cacc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    float4 d = pos[2 * (size_t)g + 1];
+    d.x += a0;
+    d.y += a1;
+    d.z += a2;
+    pos[2 * (size_t)g + 1] = d;
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_zero(uint32_t* __restrict__ p, size_t n)
@@ -617,6 +837,18 @@ hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int 
     return hipGetLastError();
 }
 
+hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
+                             const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first)
+{
+    if (cend <= cbeg) return hipSuccess;
+    const dim3 g(nblk(cend - cbeg, kClusterBlock));
+    if (first)
+        hipLaunchKernelGGL(k_gs_cluster<true>, g, dim3(kClusterBlock), 0, s, ta, cverts, cstart, tloc, pos, cbeg, cend);
+    else
+        hipLaunchKernelGGL(k_gs_cluster<false>, g, dim3(kClusterBlock), 0, s, ta, cverts, cstart, tloc, pos, cbeg, cend);
+    return hipGetLastError();
+}
+
 hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float4* pos, float* acc, int nt,
                                  bool first)
 {
@@ -624,6 +856,44 @@ hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float
     const dim3 g(nblk(nt, kBlock));
     if (first) hipLaunchKernelGGL(k_jacobi_project<true>, g, dim3(kBlock), 0, s, ta, pos, acc, nt);
     else hipLaunchKernelGGL(k_jacobi_project<false>, g, dim3(kBlock), 0, s, ta, pos, acc, nt);
+    return hipGetLastError();
+}
+
+size_t jacobi_patch_smem(int maxl, bool det)
+{
+    return (size_t)maxl * 16 + (size_t)maxl * 12 + (det ? (size_t)12 * kPatch * 4 : (size_t)maxl * 12);
+}
+
+int jacobi_patch_size() { return kPatch; }
+
+hipError_t launch_jacobi_patch(hipStream_t s, const TetArrays& ta, const PatchArrays& pa, float4* pos, float4* cacc,
+                               int nt, bool first, bool det, bool con)
+{
+    if (pa.npatch <= 0) return hipSuccess;
+    const size_t sm = jacobi_patch_smem(pa.maxl, det);
+#define XPBD_PATCH_LAUNCH(F, D, Cn)                                                                        \
+    hipLaunchKernelGGL((k_jacobi_patch<F, D, Cn>), dim3(pa.npatch), dim3(kPatch), sm, s, ta, pa.tloc, pa.pvoff, \
+                       pa.pverts, pa.pslot, pa.inc_off, pa.inc, pos, pa.partial, cacc, nt, pa.maxl)
+    const int sel = (first ? 4 : 0) | (det ? 2 : 0) | (con ? 1 : 0);
+    switch (sel) {
+    case 0: XPBD_PATCH_LAUNCH(false, false, false); break;
+    case 1: XPBD_PATCH_LAUNCH(false, false, true); break;
+    case 2: XPBD_PATCH_LAUNCH(false, true, false); break;
+    case 3: XPBD_PATCH_LAUNCH(false, true, true); break;
+    case 4: XPBD_PATCH_LAUNCH(true, false, false); break;
+    case 5: XPBD_PATCH_LAUNCH(true, false, true); break;
+    case 6: XPBD_PATCH_LAUNCH(true, true, false); break;
+    default: XPBD_PATCH_LAUNCH(true, true, true); break;
+    }
+#undef XPBD_PATCH_LAUNCH
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || pa.nshared <= 0) return e;
+    if (con)
+        hipLaunchKernelGGL(k_jacobi_shared<true>, dim3(nblk(pa.nshared, kBlock)), dim3(kBlock), 0, s, pa.shared,
+                           pa.soff, pa.sslot, pa.partial, cacc, pos, pa.nshared);
+    else
+        hipLaunchKernelGGL(k_jacobi_shared<false>, dim3(nblk(pa.nshared, kBlock)), dim3(kBlock), 0, s, pa.shared,
+                           pa.soff, pa.sslot, pa.partial, cacc, pos, pa.nshared);
     return hipGetLastError();
 }
 
