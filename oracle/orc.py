@@ -64,6 +64,9 @@ _sig("orc_energy", [C.c_void_p], C.c_double)
 _sig("orc_set_contact_order", [C.c_void_p, _ip])
 _sig("orc_last_step_residual", [C.c_void_p], C.c_double)
 _sig("orc_project_contact", [_dp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int)
+_sig("orc_step_corrected", [C.c_void_p, C.c_int, _dp, C.c_int])
+_sig("orc_step_dual_warm", [C.c_void_p, C.c_int, _dp, C.c_int])
+_sig("orc_step_learned", [C.c_void_p, C.c_int, _dp, C.c_int])
 
 
 def project_contact(x9, w3, cp, cn, bary, lam=0.0):
@@ -199,6 +202,22 @@ class Sim:
 
     def predict(self):
         lib.orc_predict(self.s)
+
+    def step_learned(self, corr, iters, detect=False):
+        """Learned-warm-start step: multipliers and positions started from the
+        predicted correction `corr` (V x 3, or None for zero)."""
+        c = None if corr is None else _d(corr).reshape(-1)
+        lib.orc_step_learned(self.s, 1 if detect else 0, p(c), iters)
+
+    def step_dual_warm(self, lam0, iters, detect=False):
+        """One step whose multipliers start at lam0 (T x 2) with the consistent position start."""
+        l0 = None if lam0 is None else _d(lam0).reshape(-1)
+        lib.orc_step_dual_warm(self.s, 1 if detect else 0, p(l0), iters)
+
+    def step_corrected(self, corr, iters, detect=False):
+        """One step with `corr` (V x 3) added to the predicted positions before `iters` iterations."""
+        c = None if corr is None else _d(corr).reshape(-1)
+        lib.orc_step_corrected(self.s, 1 if detect else 0, p(c), iters)
 
     def _get(self, fn, shape):
         out = np.empty(shape)

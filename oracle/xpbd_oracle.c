@@ -182,6 +182,12 @@ ORC_API void orc_deformation_gradient(const double* x12, const double* Q, double
 /*     by dt^2), gamma, dt and the multipliers lam[2] (updated in place).     */
 /*     dx[12] receives the coordinate updates in vertex-major order.          */
 /* ------------------------------------------------------------------------ */
+/* Relaxation factor on the multiplier step (1 = the reference projector).
+ * Scaling dlambda scales the position update identically, so the momentum
+ * relation between x and lambda is kept. */
+static double g_relax = 1.0;
+ORC_API void orc_set_relaxation(double omega) { g_relax = omega; }
+
 static void project_combined(const double* const p[4], const double* w,
                              const double* Q, double aD, double aH, double gamma,
                              double dt, double* lam, double* dx)
@@ -206,8 +212,8 @@ static void project_combined(const double* const p[4], const double* w,
     const double b0 = -C[0] - at[0] * lam[0];
     const double b1 = -C[1] - at[1] * lam[1];
     const double det = A00 * A11 - A01 * A01;
-    const double d0 = (b0 * A11 - b1 * A01) / det;
-    const double d1 = (b1 * A00 - b0 * A01) / det;
+    const double d0 = g_relax * (b0 * A11 - b1 * A01) / det;
+    const double d1 = g_relax * (b1 * A00 - b0 * A01) / det;
     lam[0] += d0;
     lam[1] += d1;
     for (int i = 0; i < 4; i++)
@@ -734,6 +740,121 @@ ORC_API void orc_step(orc_sim* s, int detect)
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
     for (int it = 0; it < s->iters; it++) iterate(s);
     /* EVERY_SUBSTEP residual is taken here, before the pin reset (XPBDSolver.hpp:91-96) */
+    s->last_res = orc_residual(s, NULL);
+    for (int i = 0; i < nv; i++)
+        if (s->pin[i])
+            for (int k = 0; k < 3; k++) s->x[3 * i + k] = s->xp[3 * i + k];
+    for (int i = 0; i < 3 * nv; i++) s->v[i] = (s->x[i] - s->xp[i]) / dt;
+}
+
+/* Warm-started step: like orc_step, but corr (3*nv, may be NULL) is added to
+ * the predicted positions before the solve.  This is the reference semantics
+ * of the learned step (the network supplies corr); with corr = NULL it is
+ * exactly orc_step. */
+ORC_API void orc_step_corrected(orc_sim* s, int detect, const double* corr, int iters)
+{
+    if (detect) orc_detect(s);
+    const int nv = s->nv;
+    const double dt = s->dt;
+    memcpy(s->xp, s->x, sizeof(double) * 3 * nv);
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
+    const double dz = -s->g * dt * dt;
+    for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
+    if (corr)
+        for (int i = 0; i < 3 * nv; i++) s->x[i] += corr[i];
+    memset(s->lam, 0, sizeof(double) * 2 * (size_t)s->nt);
+    for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
+    for (int it = 0; it < iters; it++) iterate(s);
+    s->last_res = orc_residual(s, NULL);
+    for (int i = 0; i < nv; i++)
+        if (s->pin[i])
+            for (int k = 0; k < 3; k++) s->x[3 * i + k] = s->xp[3 * i + k];
+    for (int i = 0; i < 3 * nv; i++) s->v[i] = (s->x[i] - s->xp[i]) / dt;
+}
+
+/* Dual warm-started step: the multipliers start at lam0 (2*nt) instead of 0,
+ * and the predicted positions are moved to the consistent start
+ *     x0 = x~ + M^-1 sum_e grad C_e(x~)^T lam0_e
+ * so that the momentum equation M (x - x~) = grad C^T lambda, which the XPBD
+ * update preserves, holds at the first iteration.  The fixed point of the
+ * iteration is unchanged; only the starting point moves.  lam0 = NULL gives
+ * exactly orc_step. */
+ORC_API void orc_step_dual_warm(orc_sim* s, int detect, const double* lam0, int iters)
+{
+    if (detect) orc_detect(s);
+    const int nv = s->nv;
+    const double dt = s->dt;
+    memcpy(s->xp, s->x, sizeof(double) * 3 * nv);
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
+    const double dz = -s->g * dt * dt;
+    for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
+    if (lam0) {
+        memcpy(s->lam, lam0, sizeof(double) * 2 * (size_t)s->nt);
+        double* dx = (double*)calloc(3 * (size_t)nv, sizeof(double));
+        for (int e = 0; e < s->nt; e++) {
+            const int* t = s->tets + 4 * e;
+            double x12[12], C[2], g[24];
+            for (int j = 0; j < 4; j++)
+                for (int k = 0; k < 3; k++) x12[3 * j + k] = s->x[3 * t[j] + k];
+            orc_eval_tet(x12, s->Q + 9 * e, s->gam[e], C, g);
+            for (int j = 0; j < 4; j++)
+                for (int k = 0; k < 3; k++)
+                    dx[3 * t[j] + k] += s->w[t[j]] * (g[3 * j + k] * lam0[2 * e] + g[12 + 3 * j + k] * lam0[2 * e + 1]);
+        }
+        for (int i = 0; i < 3 * nv; i++) s->x[i] += dx[i];
+        free(dx);
+    } else {
+        memset(s->lam, 0, sizeof(double) * 2 * (size_t)s->nt);
+    }
+    for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
+    for (int it = 0; it < iters; it++) iterate(s);
+    s->last_res = orc_residual(s, NULL);
+    for (int i = 0; i < nv; i++)
+        if (s->pin[i])
+            for (int k = 0; k < 3; k++) s->x[3 * i + k] = s->xp[3 * i + k];
+    for (int i = 0; i < 3 * nv; i++) s->v[i] = (s->x[i] - s->xp[i]) / dt;
+}
+
+/* Learned-warm-start step (the reference semantics of the device's learned
+ * step).  Given a predicted position correction corr (3*nv; NULL = zero), the
+ * solve starts from the multipliers that satisfy the constraint equations at
+ * the predicted point xl = x~ + corr,
+ *     lam0_e = -C_e(xl) / alpha~_e,
+ * and from the positions that satisfy the momentum equation for them,
+ *     x0 = x~ + M^-1 sum_e grad C_e(xl)^T lam0_e.
+ * If corr is the exact correction this start is the solution of the step; for
+ * any corr the start obeys M (x0 - x~) = grad C^T lam0, which the XPBD updates
+ * preserve, so the iteration converges to the same fixed point as a cold start. */
+ORC_API void orc_step_learned(orc_sim* s, int detect, const double* corr, int iters)
+{
+    if (detect) orc_detect(s);
+    const int nv = s->nv;
+    const double dt = s->dt, dt2 = dt * dt;
+    memcpy(s->xp, s->x, sizeof(double) * 3 * nv);
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
+    const double dz = -s->g * dt * dt;
+    for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
+    double* dx = (double*)calloc(3 * (size_t)nv, sizeof(double));
+    for (int e = 0; e < s->nt; e++) {
+        const int* t = s->tets + 4 * e;
+        double x12[12], C[2], g[24];
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 3; k++)
+                x12[3 * j + k] = s->x[3 * t[j] + k] + (corr ? corr[3 * t[j] + k] : 0.0);
+        orc_eval_tet(x12, s->Q + 9 * e, s->gam[e], C, g);
+        const double l0 = -C[0] / (s->aD[e] / dt2), l1 = -C[1] / (s->aH[e] / dt2);
+        s->lam[2 * e] = l0;
+        s->lam[2 * e + 1] = l1;
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 3; k++) dx[3 * t[j] + k] += s->w[t[j]] * (g[3 * j + k] * l0 + g[12 + 3 * j + k] * l1);
+    }
+    for (int i = 0; i < 3 * nv; i++) s->x[i] += dx[i];
+    free(dx);
+    for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
+    for (int it = 0; it < iters; it++) iterate(s);
     s->last_res = orc_residual(s, NULL);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])

@@ -390,6 +390,7 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     rc |= dalloc(h, &h->d_flag, nf);
     rc |= dalloc(h, &h->d_cpos, (size_t)nf + 1);
     rc |= dalloc(h, &h->d_clist, nf);
+    rc |= dalloc(h, &h->d_bsum, (size_t)compact_blocks(nf));
     rc |= dalloc(h, &h->d_cp, 3 * (size_t)nf);
     rc |= dalloc(h, &h->d_cn, 3 * (size_t)nf);
     rc |= dalloc(h, &h->d_cb, 3 * (size_t)nf);
@@ -580,7 +581,7 @@ int enqueue_detect(xpbd_handle* h)
     hipError_t e = launch_detect(h->stream, h->d_faces, h->nf, h->d_rest, h->d_pos, h->d_lo, h->d_sdf, h->d_step,
                                  h->params.dt, h->d_flag, h->d_cp, h->d_cn, h->d_cb);
     if (e != hipSuccess) return hip_err(h, e, "detect launch");
-    e = launch_compact(h->stream, h->d_flag, h->nf, h->d_cpos, h->d_clist);
+    e = launch_compact(h->stream, h->d_flag, h->nf, h->d_cpos, h->d_clist, h->d_bsum);
     if (e != hipSuccess) return hip_err(h, e, "compact launch");
     return XPBD_OK;
 }

@@ -74,6 +74,7 @@ struct xpbd_handle {
     int32_t* d_flag = nullptr;
     int32_t* d_cpos = nullptr;
     int32_t* d_clist = nullptr;
+    int32_t* d_bsum = nullptr;   // compaction scratch (per-block hit counts)
     double* d_cp = nullptr;
     double* d_cn = nullptr;
     double* d_cb = nullptr;

@@ -79,7 +79,9 @@ hipError_t launch_sum_partials(hipStream_t s, const double* part, int n, double*
 hipError_t launch_detect(hipStream_t s, const int4* faces, int nf, const double* rest, const float4* pos,
                          const float4* lo, const SdfDev* sdf, const unsigned long long* step, double dt, int* flag,
                          double* cp, double* cn, double* cb);
-hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int* clist);
+// Ordered compaction of hit flags; bsum is scratch of compact_blocks(nf) ints.
+int compact_blocks(int nf);
+hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int* clist, int* bsum);
 hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fcol_off, int ncol, const int* cpos,
                               const int* clist, const double* rest, float4* pos, const float4* lo, const double* cp,
                               const double* cn, const double* cb, double* clam, bool first);

@@ -135,12 +135,15 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(TetArrays ta, float4* __res
 // ---------------------------------------------------------------------------
 // Cluster Gauss-Seidel: one thread per cluster (up to 8 vertices, up to 8
 // tets, consecutive in solver order).  Clusters of one colour share no vertex.
-// The thread stages its cluster's vertices in its own LDS column (slot-major,
-// so lane t always hits bank t), projects the tets in order, and writes the
-// vertices back: sequential Gauss-Seidel in (colour, cluster, tet) order.
-// No barrier is needed because no thread reads another thread's column.
+// The workgroup first stages the element data of its clusters (one contiguous
+// tet range) into LDS with coalesced loads; each thread stages its cluster's
+// vertices in its own LDS column (slot-major, so lane t always hits bank t),
+// projects the tets in order, and writes the vertices back: sequential
+// Gauss-Seidel in (colour, cluster, tet) order.  After the one staging
+// barrier no thread reads another thread's vertex column.
 // ---------------------------------------------------------------------------
 constexpr int kClusterBlock = 64;
+constexpr int kClusterStage = kClusterBlock * 8;  // at most 8 tets per cluster
 
 template <bool FIRST>
 __global__ __launch_bounds__(kClusterBlock) void k_gs_cluster(TetArrays ta, const int4* __restrict__ cverts,
@@ -148,58 +151,85 @@ __global__ __launch_bounds__(kClusterBlock) void k_gs_cluster(TetArrays ta, cons
                                                              const uint32_t* __restrict__ tloc,
                                                              float4* __restrict__ pos, int cbeg, int cend)
 {
-    __shared__ float sb[8][4][kClusterBlock];  // base xyz, inverse mass
-    __shared__ float sd[8][3][kClusterBlock];  // in-step motion
+    __shared__ float sb[8][4][kClusterBlock];   // base xyz, inverse mass (thread-private columns)
+    __shared__ float sd[8][3][kClusterBlock];   // in-step motion
+    __shared__ float sq[12][kClusterStage];     // Q (9), alpha tilde (2), gamma of the block's tets
+    __shared__ float sl[2][kClusterStage];      // lambda
+    __shared__ uint32_t stl[kClusterStage];     // local vertex slots
     const int tid = threadIdx.x;
-    const int c = cbeg + blockIdx.x * kClusterBlock + tid;
-    if (c >= cend) return;
-    const int4 va = cverts[2 * (size_t)c], vb = cverts[2 * (size_t)c + 1];
-    const int vid[8] = { va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w };
+    const int cb = cbeg + blockIdx.x * kClusterBlock;
+    const int cl_end = min(cend, cb + kClusterBlock);
+    // the block's clusters are consecutive, so their tets form one contiguous
+    // range: stage its element data with coalesced loads
+    const int tb = cstart[cb];
+    const int n = cstart[cl_end] - tb;
+    for (int i = tid; i < n; i += kClusterBlock) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        if (vid[q] < 0) continue;
-        const float4 b = pos[2 * (size_t)vid[q]];
-        const float4 d = pos[2 * (size_t)vid[q] + 1];
-        sb[q][0][tid] = b.x;
-        sb[q][1][tid] = b.y;
-        sb[q][2][tid] = b.z;
-        sb[q][3][tid] = b.w;
-        sd[q][0][tid] = d.x;
-        sd[q][1][tid] = d.y;
-        sd[q][2][tid] = d.z;
+        for (int k = 0; k < 9; k++) sq[k][i] = ta.Q[(size_t)k * ta.qstride + tb + i];
+        const float2 a = ta.at[tb + i];
+        sq[9][i] = a.x;
+        sq[10][i] = a.y;
+        sq[11][i] = ta.gamma[tb + i];
+        if (!FIRST) {
+            const float2 lv = ta.lam[tb + i];
+            sl[0][i] = lv.x;
+            sl[1][i] = lv.y;
+        }
+        stl[i] = tloc[tb + i];
     }
-    const int t0 = cstart[c], t1 = cstart[c + 1];
+    const int c = cb + tid;
+    const bool active = c < cend;
+    int vid[8];
+    if (active) {
+        const int4 va = cverts[2 * (size_t)c], vb = cverts[2 * (size_t)c + 1];
+        vid[0] = va.x; vid[1] = va.y; vid[2] = va.z; vid[3] = va.w;
+        vid[4] = vb.x; vid[5] = vb.y; vid[6] = vb.z; vid[7] = vb.w;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (vid[q] < 0) continue;
+            const float4 b = pos[2 * (size_t)vid[q]];
+            const float4 d = pos[2 * (size_t)vid[q] + 1];
+            sb[q][0][tid] = b.x;
+            sb[q][1][tid] = b.y;
+            sb[q][2][tid] = b.z;
+            sb[q][3][tid] = b.w;
+            sd[q][0][tid] = d.x;
+            sd[q][1][tid] = d.y;
+            sd[q][2][tid] = d.z;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const int t0 = cstart[c] - tb, t1 = cstart[c + 1] - tb;
     for (int e = t0; e < t1; e++) {
-        const uint32_t loc = tloc[e];
+        const uint32_t loc = stl[e];
         float bb[12], dd[12], w[4];
-        int sl[4];
+        int slot[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            sl[j] = (loc >> (4 * j)) & 15u;
+            slot[j] = (loc >> (4 * j)) & 15u;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                bb[3 * j + k] = sb[sl[j]][k][tid];
-                dd[3 * j + k] = sd[sl[j]][k][tid];
+                bb[3 * j + k] = sb[slot[j]][k][tid];
+                dd[3 * j + k] = sd[slot[j]][k][tid];
             }
-            w[j] = sb[sl[j]][3][tid];
+            w[j] = sb[slot[j]][3][tid];
         }
         float q[9];
-        float2 at;
-        float gm;
-        load_tet(ta, e, q, at, gm);
+#pragma unroll
+        for (int k = 0; k < 9; k++) q[k] = sq[k][e];
         float l[2] = { 0.0f, 0.0f };
         if (!FIRST) {
-            const float2 lv = ta.lam[e];
-            l[0] = lv.x;
-            l[1] = lv.y;
+            l[0] = sl[0][e];
+            l[1] = sl[1][e];
         }
         float dx[12];
-        project_tet_split<float>(bb, dd, w, q, at.x, at.y, gm, l, dx);
-        ta.lam[e] = make_float2(l[0], l[1]);
+        project_tet_split<float>(bb, dd, w, q, sq[9][e], sq[10][e], sq[11][e], l, dx);
+        ta.lam[tb + e] = make_float2(l[0], l[1]);
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) sd[sl[j]][k][tid] = dd[3 * j + k] + dx[3 * j + k];
+            for (int k = 0; k < 3; k++) sd[slot[j]][k][tid] = dd[3 * j + k] + dx[3 * j + k];
     }
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -283,6 +313,23 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
     const int tid = threadIdx.x;
     const int p = blockIdx.x;
     const int v0 = pvoff[p], nl = pvoff[p + 1] - v0;
+    // element data first, so its latency overlaps the vertex staging
+    const int e = p * kPatch + tid;
+    const bool has = e < nt;
+    uint2 lc = make_uint2(0u, 0u);
+    float q[9];
+    float2 at = make_float2(0.f, 0.f);
+    float gm = 0.f;
+    float l[2] = { 0.0f, 0.0f };
+    if (has) {
+        lc = tloc[e];
+        load_tet(ta, e, q, at, gm);
+        if (!FIRST) {
+            const float2 lv = ta.lam[e];
+            l[0] = lv.x;
+            l[1] = lv.y;
+        }
+    }
     for (int i = tid; i < nl; i += kPatch) {
         const int g = pverts[v0 + i];
         lb[i] = pos[2 * (size_t)g];
@@ -293,9 +340,7 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
         if (!DET) red[3 * i] = red[3 * i + 1] = red[3 * i + 2] = 0.f;
     }
     __syncthreads();
-    const int e = p * kPatch + tid;
-    if (e < nt) {
-        const uint2 lc = tloc[e];
+    if (has) {
         const int sl[4] = { (int)(lc.x & 0xffffu), (int)(lc.x >> 16), (int)(lc.y & 0xffffu), (int)(lc.y >> 16) };
         float bb[12], dd[12], w[4];
 #pragma unroll
@@ -307,16 +352,6 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
             w[j] = b.w;
 #pragma unroll
             for (int k = 0; k < 3; k++) dd[3 * j + k] = ld[3 * sl[j] + k];
-        }
-        float q[9];
-        float2 at;
-        float gm;
-        load_tet(ta, e, q, at, gm);
-        float l[2] = { 0.0f, 0.0f };
-        if (!FIRST) {
-            const float2 lv = ta.lam[e];
-            l[0] = lv.x;
-            l[1] = lv.y;
         }
         float dx[12];
         project_tet_split<float>(bb, dd, w, q, at.x, at.y, gm, l, dx);
@@ -685,22 +720,40 @@ __global__ __launch_bounds__(kBlock) void k_detect(const int4* __restrict__ face
     flag[j] = hit;
 }
 
-// Single-workgroup ordered compaction: cpos[j] = number of hits before j,
-// cpos[nf] = total, clist[cpos[j]] = j for every hit.
+// Ordered compaction of the hit flags in three small kernels:
+//   count   per 256-face block, hits counted with wave ballots
+//   scan    one workgroup turns the block counts into block offsets
+//   scatter each hit's position = block offset + hits before it in the block
+// Result: cpos[j] = number of hits before face slot j, cpos[nf] = total,
+// clist[cpos[j]] = j for every hit (face order preserved).
 constexpr int kScan = 1024;
-__global__ __launch_bounds__(kScan) void k_compact(const int* __restrict__ flag, int nf,
-                                                  int* __restrict__ cpos, int* __restrict__ clist)
+constexpr int kWaves = kBlock / 64;
+
+__global__ __launch_bounds__(kBlock) void k_count_hits(const int* __restrict__ flag, int nf, int* __restrict__ bsum)
+{
+    __shared__ int ws[kWaves];
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    const unsigned long long m = __ballot(j < nf && flag[j] != 0);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kWaves; w++) t += ws[w];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kScan) void k_scan_blocks(int* __restrict__ bsum, int nb, int* __restrict__ cpos, int nf)
 {
     __shared__ int sh[kScan];
-    const int chunk = (nf + kScan - 1) / kScan;
+    const int chunk = (nb + kScan - 1) / kScan;
     const int b = threadIdx.x * chunk;
-    const int e = min(nf, b + chunk);
+    const int e = min(nb, b + chunk);
     int cnt = 0;
-    for (int j = b; j < e; j++) cnt += flag[j];
+    for (int j = b; j < e; j++) cnt += bsum[j];
     sh[threadIdx.x] = cnt;
     __syncthreads();
-    // Hillis-Steele inclusive scan
-    for (int off = 1; off < kScan; off <<= 1) {
+    for (int off = 1; off < kScan; off <<= 1) {  // Hillis-Steele inclusive scan
         const int v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
         __syncthreads();
         sh[threadIdx.x] += v;
@@ -708,10 +761,31 @@ __global__ __launch_bounds__(kScan) void k_compact(const int* __restrict__ flag,
     }
     int run = sh[threadIdx.x] - cnt;
     for (int j = b; j < e; j++) {
-        cpos[j] = run;
-        if (flag[j]) clist[run++] = j;
+        const int v = bsum[j];
+        bsum[j] = run;
+        run += v;
     }
     if (threadIdx.x == kScan - 1) cpos[nf] = sh[kScan - 1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter_hits(const int* __restrict__ flag, int nf,
+                                                        const int* __restrict__ boff, int* __restrict__ cpos,
+                                                        int* __restrict__ clist)
+{
+    __shared__ int ws[kWaves];
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    const bool f = j < nf && flag[j] != 0;
+    const unsigned long long m = __ballot(f);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) ws[wv] = __popcll(m);
+    __syncthreads();
+    int base = boff[blockIdx.x];
+    for (int w = 0; w < wv; w++) base += ws[w];
+    const int p = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (j < nf) {
+        cpos[j] = p;
+        if (f) clist[p] = j;
+    }
 }
 
 // Static contact projection in FP64 (alpha = 0, inequality):
@@ -945,9 +1019,14 @@ hipError_t launch_detect(hipStream_t s, const int4* faces, int nf, const double*
     return hipGetLastError();
 }
 
-hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int* clist)
+int compact_blocks(int nf) { return (int)nblk(nf > 0 ? nf : 1, kBlock); }
+
+hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int* clist, int* bsum)
 {
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(kScan), 0, s, flag, nf, cpos, clist);
+    const int nb = compact_blocks(nf);
+    hipLaunchKernelGGL(k_count_hits, dim3(nb), dim3(kBlock), 0, s, flag, nf, bsum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScan), 0, s, bsum, nb, cpos, nf);
+    hipLaunchKernelGGL(k_scatter_hits, dim3(nb), dim3(kBlock), 0, s, flag, nf, bsum, cpos, clist);
     return hipGetLastError();
 }
 

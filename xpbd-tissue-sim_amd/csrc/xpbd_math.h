@@ -31,12 +31,24 @@
 
 namespace xpbd {
 
+// FP32 device code uses the hardware sqrt, reciprocal and log2 (at most 1 ulp
+// from the correctly rounded result) instead of the multi-instruction IEEE
+// sequences; FP64 and host code stay correctly rounded.
 template <typename R> XPBD_HD R xsqrt(R v);
-template <> XPBD_HD float xsqrt<float>(float v) { return sqrtf(v); }
-template <> XPBD_HD double xsqrt<double>(double v) { return sqrt(v); }
+template <typename R> XPBD_HD R xrcp(R v);
 template <typename R> XPBD_HD R xlog(R v);
-template <> XPBD_HD float xlog<float>(float v) { return logf(v); }
+template <> XPBD_HD double xsqrt<double>(double v) { return sqrt(v); }
+template <> XPBD_HD double xrcp<double>(double v) { return 1.0 / v; }
 template <> XPBD_HD double xlog<double>(double v) { return log(v); }
+#if defined(__HIP_DEVICE_COMPILE__)
+template <> XPBD_HD float xsqrt<float>(float v) { return __builtin_amdgcn_sqrtf(v); }
+template <> XPBD_HD float xrcp<float>(float v) { return __builtin_amdgcn_rcpf(v); }
+template <> XPBD_HD float xlog<float>(float v) { return __builtin_amdgcn_logf(v) * 0.693147180559945309f; }
+#else
+template <> XPBD_HD float xsqrt<float>(float v) { return sqrtf(v); }
+template <> XPBD_HD float xrcp<float>(float v) { return 1.0f / v; }
+template <> XPBD_HD float xlog<float>(float v) { return logf(v); }
+#endif
 
 // F = I + DU Q from the four vertex displacements.
 template <typename R>
@@ -78,7 +90,7 @@ XPBD_HD void element_constraints(const R* F, R gamma, R* CD, R* CH, R* J_out)
         *CH = -gamma + xlog<R>(J);
     } else {
         const R a = J - R(1);
-        *CH = -gamma + a - a * a / R(2) + a * a * a / R(3);
+        *CH = -gamma + a - a * a * R(0.5) + a * a * a * (R(1) / R(3));
     }
     if (J_out) *J_out = J;
 }
@@ -114,9 +126,9 @@ XPBD_HD void project_tet_F(const R* F, const R* w, const R* Q, R atD, R atH, R g
 {
     R CD, CH, J;
     element_constraints<R>(F, gamma, &CD, &CH, &J);
-    const R inv_C = R(1) / CD;
+    const R inv_C = xrcp<R>(CD);
     const R a = J - R(1);
-    const R fac = (J >= R(1)) ? R(1) / J : R(1) - a + a * a;
+    const R fac = (J >= R(1)) ? xrcp<R>(J) : R(1) - a + a * a;
 
     // P = F Q^T and Rm = Fcross Q^T ; column i is the (unscaled) gradient at vertex i
     R Fc[9];
@@ -157,9 +169,9 @@ XPBD_HD void project_tet_F(const R* F, const R* w, const R* Q, R atD, R atH, R g
     const R A00 = atD + s00, A11 = atH + s11, A01 = s01;
     const R b0 = -CD - atD * lam[0];
     const R b1 = -CH - atH * lam[1];
-    const R det = A00 * A11 - A01 * A01;
-    const R d0 = (b0 * A11 - b1 * A01) / det;
-    const R d1 = (b1 * A00 - b0 * A01) / det;
+    const R inv_det = xrcp<R>(A00 * A11 - A01 * A01);
+    const R d0 = (b0 * A11 - b1 * A01) * inv_det;
+    const R d1 = (b1 * A00 - b0 * A01) * inv_det;
     lam[0] += d0;
     lam[1] += d1;
 #pragma unroll
