@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--learned-steps", type=int, default=50, help="steps of the learned-vs-plain comparison (0: off)")
+    ap.add_argument("--learned-iters", type=int, default=3)
+    ap.add_argument("--weights", default=os.path.join(ROOT, "xpbd-tissue-sim_amd", "weights", "warmstart.npz"))
     return ap.parse_args()
 
 
@@ -99,6 +102,66 @@ def cpu_baseline(meshes, cfg, sdf, nsteps):
                       f"{cfg['solver']} iterations) on one host core, FP64 oracle",
             "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el,
             "host": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
+
+
+MLP_FLOP_PER_VERTEX = 2 * (24 * 128 + 128 * 128 + 128 * 3)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def learned_vs_plain(a, net_path):
+    """Learned warm start (k iterations + network, fallback at 1.1x the plain
+    residual) against the plain solver on the same workload and steps."""
+    import numpy as np
+    import train_warmstart as TW
+    s_plain, _, cfg, _ = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
+    s_plain.set_residual_policy(True)
+    r_plain, t_plain = [], 0.0
+    for _ in range(a.learned_steps):
+        t0 = time.perf_counter()
+        s_plain.step(1)
+        r_plain.append(s_plain.last_step_residual())
+        t_plain += time.perf_counter() - t0
+    x_plain = s_plain.positions()
+    trained_here = not os.path.exists(net_path)
+    if trained_here:
+        # no saved weights: train a small network on the device now
+        x, y = TW.build_dataset(2, 60)
+        import xpbd_amd as X
+        net = X.Network()
+        net.fit_scaling(x, y)
+        net.train(x, y, batch=8192, epochs=10, lr=2e-3, seed=7)
+    else:
+        net = TW.load(net_path)
+    out = {"iterations": a.learned_iters, "plain_iterations": cfg["iterations"], "fallback_ratio": 1.1,
+           "steps": a.learned_steps, "weights": "trained in this run" if trained_here else os.path.relpath(net_path, ROOT),
+           "plain_steps_per_s_with_residual_readback": a.learned_steps / t_plain}
+    for mode, ref in (("with_fallback", r_plain), ("no_fallback", [0.0] * len(r_plain))):
+        s, _, _, _ = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
+        s.attach(net)
+        s.set_kernel_timing(True)
+        ratios, fb, t = [], 0, 0.0
+        for n in range(a.learned_steps):
+            t0 = time.perf_counter()
+            f, r = s.step_learned(a.learned_iters, ref[n], 1.1)
+            t += time.perf_counter() - t0
+            fb += int(f)
+            ratios.append(r / max(r_plain[n], 1e-300))
+        st = s.learned_stats()
+        x = s.positions()
+        ratios = np.array(ratios)
+        ms_inf = st["mlp_ms"] / max(st["mlp_calls"], 1)
+        out[mode] = {"steps_per_s": a.learned_steps / t, "fallback_rate": fb / a.learned_steps,
+                     "residual_ratio_median": float(np.median(ratios)), "residual_ratio_max": float(ratios.max()),
+                     "rel_pos_err_vs_plain_end": float(np.max(np.abs(x - x_plain)) / np.max(np.abs(x_plain))),
+                     "mlp_ms_per_inference": ms_inf}
+        if mode == "no_fallback":
+            flop = MLP_FLOP_PER_VERTEX * s.nv
+            out["mlp_tflops"] = flop / (ms_inf * 1e-3) / 1e12
+            out["mlp_mfma_util"] = out["mlp_tflops"] / MFMA_BF16_PEAK_TFLOPS
+        s.close()
+    out["learned_over_plain_at_equal_residual"] = (out["with_fallback"]["steps_per_s"] /
+                                                   out["plain_steps_per_s_with_residual_readback"])
+    return out
 
 
 def main():
@@ -161,6 +224,9 @@ def main():
                      "us_per_iteration": us_per_iter, "algorithmic_bytes_per_iteration": bytes_iter,
                      "launches_timed": nlaunch},
     }
+    if a.learned_steps > 0 and a.workload != "batch":
+        s.close()
+        out["learned"] = learned_vs_plain(a, a.weights)
     if not a.no_cpu:
         n = a.cpu_steps if a.cpu_steps is not None else (5 if a.workload == "block" else 20)
         out["cpu_baseline"] = cpu_baseline(meshes, cfg, sdf, n)

@@ -252,6 +252,7 @@ int xpbd_read_msh(const char* path, int32_t* num_vertices, int32_t* num_tets, in
  * 20,099 values): W1[128][24], b1[128], W2[128][128], b2[128], W3[3][128], b3[3]. */
 int xpbd_mlp_create(xpbd_mlp** out);
 int xpbd_mlp_destroy(xpbd_mlp* m);
+const char* xpbd_mlp_last_error(const xpbd_mlp* m);
 int xpbd_mlp_num_params(int32_t* out);
 int xpbd_mlp_set_params(xpbd_mlp* m, const float* params);
 int xpbd_mlp_get_params(xpbd_mlp* m, float* params);
@@ -259,8 +260,10 @@ int xpbd_mlp_get_params(xpbd_mlp* m, float* params);
 int xpbd_mlp_set_scaling(xpbd_mlp* m, const float* mean_in24, const float* std_in24, float scale_out);
 /* Forward on n host rows (n x 24) -> out (n x 3). */
 int xpbd_mlp_forward(xpbd_mlp* m, int32_t n, const float* x, float* out);
-/* Mean-squared-error loss and parameter gradient on n rows (host arrays);
- * grad receives 20,099 floats.  Used by the gradient checks. */
+/* Loss and parameter gradient on n rows (host arrays): the mean over n*3 of
+ * (MLP(x) - y / scale_out)^2, the squared error in units of scale_out; grad
+ * receives 20,099 floats.  Used by the gradient checks; training minimises
+ * the same loss. */
 int xpbd_mlp_loss_grad(xpbd_mlp* m, int32_t n, const float* x, const float* y, double* loss,
                        float* grad);
 /* Adam training on device-resident samples: epochs passes of minibatches. */
@@ -276,12 +279,19 @@ int xpbd_collect_training(xpbd_handle* h, int32_t nsteps, int32_t cap, float* x,
 int xpbd_features(xpbd_handle* h, float* out);
 /* Attach (or detach with NULL) a network to the handle for learned steps. */
 int xpbd_attach_mlp(xpbd_handle* h, xpbd_mlp* m);
-/* One learned step: predict, apply the network's correction, iters solver
- * iterations.  If the residual exceeds fallback_ratio * ref_residual the step
- * is redone from the saved state with the handle's full iteration count;
- * *fell_back reports it and *residual the final ||r||_2. */
+/* One learned step: predict, add the network's correction to the predicted
+ * positions of the free vertices, then iters solver iterations (multipliers
+ * from zero, as in XPBDSolver.hpp:66-97).  If ||r||_2 (r = C + alpha~ lambda at
+ * the end of the solve) exceeds fallback_ratio * ref_residual, the step is
+ * redone from the saved start-of-step state with the handle's full iteration
+ * count, which reproduces the plain step exactly; *fell_back reports it and
+ * *residual the final ||r||_2.  ref_residual <= 0 disables the fallback. */
 int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_residual, double fallback_ratio,
                       int32_t* fell_back, double* residual);
+/* Counters of the learned steps on h: steps taken, fallbacks, and (with kernel
+ * timing on) the summed milliseconds and number of network inferences. */
+int xpbd_learned_stats(xpbd_handle* h, int64_t* steps, int64_t* fallbacks, double* mlp_ms,
+                       int64_t* mlp_calls);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,239 @@
+"""Learned warm start on the device: the MFMA network against the FP64 oracle,
+gradients, training, features, the learned step and its fallback.
+
+Tolerances: network outputs within 2e-3 (relative to the largest output) of
+the oracle with the same bfloat16 operand rounding, 3e-2 of the unrounded FP64
+network; gradients within 1e-2 relative L2 per parameter block of the rounded
+oracle (5e-2 of FP64); features within 1e-5 relative per column group; the
+learned step within the solver's 1e-5 position tolerance of the oracle's step
+from the same corrected start; the fallback bit-identical to a plain step."""
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+import orc
+import workloads as W
+import xpbd_amd as X
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def glorot(seed, bias=0.05):
+    rng = np.random.default_rng(seed)
+    P = {}
+    for name, shp in M.SIZES:
+        if name.startswith("W"):
+            a = np.sqrt(6.0 / (shp[0] + shp[1]))
+            P[name] = rng.uniform(-a, a, shp)
+        else:
+            P[name] = rng.normal(0, bias, shp)
+    return M.pack(P).astype(np.float32)
+
+
+def scaled_inputs(seed, n):
+    rng = np.random.default_rng(seed)
+    mean = rng.normal(size=M.N_IN).astype(np.float32)
+    std = rng.uniform(0.5, 3.0, M.N_IN).astype(np.float32)
+    x = (mean + std * rng.normal(size=(n, M.N_IN))).astype(np.float32)
+    return x, mean, std
+
+
+def rel_max(a, ref):
+    return float(np.max(np.abs(a - ref)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 1000, 4097])
+def test_network_forward_matches_oracle(n):
+    p = glorot(1)
+    x, mean, std = scaled_inputs(2, n)
+    net = X.Network(p)
+    net.set_scaling(mean, std, 2.5e-5)
+    y = net.forward(x).astype(np.float64)
+    ye = M.forward(p, x, mean, std, np.float32(2.5e-5), emulate_bf16=True)
+    y64 = M.forward(p, x, mean, std, np.float32(2.5e-5))
+    print(f"n={n}: vs bf16-rounded oracle {rel_max(y, ye):.2e}, vs FP64 {rel_max(y, y64):.2e}")
+    assert np.all(np.isfinite(y))
+    assert rel_max(y, ye) <= 2e-3
+    assert rel_max(y, y64) <= 3e-2
+
+
+def test_network_params_round_trip():
+    p = glorot(3)
+    net = X.Network(p)
+    assert np.array_equal(net.params(), p)
+    with pytest.raises(X.XpbdError):
+        bad = p.copy()
+        bad[7] = np.nan
+        net.set_params(bad)
+
+
+def _block_rel_l2(g, ref):
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in M.SIZES])
+    out = {}
+    for i, (name, _) in enumerate(M.SIZES):
+        a, b = g[offs[i]:offs[i + 1]], ref[offs[i]:offs[i + 1]]
+        out[name] = (float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)),
+                     float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)))
+    return out
+
+
+@pytest.mark.parametrize("n", [100, 3000])
+def test_network_gradient_matches_oracle(n):
+    """Forward + backward through the MFMA GEMM kernels (split-K for n = 3000)."""
+    p = glorot(4)
+    x, mean, std = scaled_inputs(5, n)
+    t = np.random.default_rng(6).normal(size=(n, 3)).astype(np.float32)
+    net = X.Network(p)
+    net.set_scaling(mean, std, 1.5)
+    loss, g = net.loss_grad(x, t)
+    le, ge = M.loss_grad(p, x, t, mean, std, np.float32(1.5), emulate_bf16=True)
+    l64, g64 = M.loss_grad(p, x, t, mean, std, np.float32(1.5))
+    be, b64 = _block_rel_l2(g.astype(np.float64), ge), _block_rel_l2(g.astype(np.float64), g64)
+    print(f"n={n} loss {loss:.6e} oracle {le:.6e}; per block (rel L2, cos) vs rounded {be}; vs FP64 {b64}")
+    assert abs(loss - le) <= 1e-3 * le
+    for name in be:
+        assert be[name][0] <= 1e-2 and be[name][1] >= 0.9999, (name, be[name])
+        assert b64[name][0] <= 5e-2, (name, b64[name])
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(7)
+    n = 20000
+    x = rng.normal(size=(n, M.N_IN)).astype(np.float32)
+    B = rng.normal(size=(M.N_IN, 3)) / np.sqrt(M.N_IN)
+    t = np.tanh(x @ B).astype(np.float32)
+    net = X.Network()
+    net.fit_scaling(x, t)
+    loss0, _ = net.loss_grad(x, t)
+    loss1 = net.train(x, t, batch=1024, epochs=15, lr=2e-3, seed=1)
+    loss_all, _ = net.loss_grad(x, t)
+    print(f"training loss {loss0:.4e} -> last batch {loss1:.4e}, full set {loss_all:.4e}")
+    assert loss_all < 0.05 * loss0
+
+
+def _mesh_state(seed=7, n=6, steps=4):
+    mesh = W.cube(n, seed=seed)
+    gpu = X.Solver(mesh, dt=1e-3, iters=5, solver="gs")
+    gpu.step(steps)
+    return mesh, gpu
+
+
+def test_features_match_oracle():
+    mesh, gpu = _mesh_state()
+    net = X.Network()
+    gpu.attach(net)
+    f = gpu.features().astype(np.float64)
+    x, v = gpu.positions(), gpu.velocities()
+    fe = M.features(x, v, np.zeros_like(x), mesh.x0, mesh.tets, mesh.pinned, mesh.E, mesh.nu, 1e-3, 9.81)
+    for a, b in [(0, 3), (3, 6), (6, 9), (9, 12), (12, 15), (15, 18), (18, 21), (21, 24)]:
+        err = np.max(np.abs(f[:, a:b] - fe[:, a:b])) / max(np.max(np.abs(fe[:, a:b])), 1e-12)
+        print(f"features {a}-{b - 1}: rel err {err:.2e}")
+        assert err <= 1e-5 or np.max(np.abs(fe[:, a:b])) == 0.0 and np.max(np.abs(f[:, a:b])) == 0.0
+    # the prediction inside features() is undone: stepping continues unchanged
+    ref = X.Solver(mesh, dt=1e-3, iters=5, solver="gs")
+    ref.step(5)
+    gpu.step(1)
+    assert np.array_equal(gpu.positions(), ref.positions())
+
+
+@pytest.mark.parametrize("solver,variant", [("gs", "cluster"), ("jacobi", "patch")])
+def test_learned_step_matches_oracle_from_the_same_corrected_start(solver, variant):
+    mesh = W.cube(6, seed=8, E=3e3 if solver == "gs" else 1e3)
+    iters = 10
+    gpu = X.Solver(mesh, dt=1e-3, iters=iters, solver=solver, variant=variant)
+    o = orc.Sim(mesh, dt=1e-3, iters=iters, mode="gs" if solver == "gs" else "jacobi")
+    if solver == "gs":
+        o.set_order(gpu.tet_order())
+    net = X.Network(glorot(9))
+    net.set_scaling(np.zeros(24), np.full(24, 1e-3), 2e-6)
+    gpu.attach(net)
+    for _ in range(3):
+        gpu.step(1)
+        o.step(1)
+    for k in range(3):
+        corr = net.forward(gpu.features()).astype(np.float64)
+        corr[mesh.pinned.astype(bool)] = 0.0
+        assert 1e-8 < np.max(np.abs(corr)) < 1e-3
+        fb, r = gpu.step_learned(3, ref_residual=0.0)
+        assert not fb
+        o.step_corrected(corr, 3)
+        err = rel_err(gpu.positions(), o.x())
+        print(f"{solver} learned step {k}: rel err {err:.2e}, residual {r:.3e} (oracle {o.last_step_residual():.3e})")
+        assert err <= 1e-5
+        assert abs(r - o.last_step_residual()) <= 1e-3 * o.last_step_residual() + 1e-9
+
+
+@pytest.mark.parametrize("solver,variant", [("gs", "cluster"), ("jacobi", "patch")])
+def test_fallback_reproduces_the_plain_step_exactly(solver, variant):
+    """A corrupted network (huge weights) must trigger the fallback, which redoes
+    the step as the plain solver: positions bit-identical to a plain run."""
+    mesh = W.cube(6, seed=10, E=3e3 if solver == "gs" else 1e3)
+    sph = W.block_sphere(6, r=0.02)
+    mk = lambda: X.Solver(mesh, dt=1e-3, iters=10, solver=solver, variant=variant)  # noqa: E731
+    plain, learned = mk(), mk()
+    for s in (plain, learned):
+        s.set_sdf(sph[0], (0.03, 0.03, 0.06 + 0.02 - 0.0005), sph[2], sph[3])
+        s.set_sdf_velocity(sph[4])
+        s.set_residual_policy(True)
+    p = glorot(11) * 1e3
+    net = X.Network(p)
+    net.set_scaling(np.zeros(24), np.full(24, 1e-6), 1.0)
+    learned.attach(net)
+    for k in range(5):
+        plain.step(1)
+        ref = plain.last_step_residual()
+        fb, r = learned.step_learned(2, ref_residual=ref, ratio=1.1)
+        assert fb, k
+        assert np.array_equal(learned.positions(), plain.positions()), k
+        assert np.array_equal(learned.velocities(), plain.velocities()), k
+        assert r == pytest.approx(ref, rel=1e-12)
+    st = learned.learned_stats()
+    assert st["steps"] == 5 and st["fallbacks"] == 5
+
+
+def test_learned_step_requires_a_network():
+    mesh = W.cube(4, seed=1)
+    gpu = X.Solver(mesh, dt=1e-3, iters=5, solver="gs")
+    with pytest.raises(X.XpbdError):
+        gpu.step_learned(3)
+    net = X.Network()
+    gpu.attach(net)
+    gpu.attach(None)
+    with pytest.raises(X.XpbdError):
+        gpu.step_learned(3)
+
+
+def _trajectory(seed, n=8, steps=40):
+    """A small heterogeneous block pressed by a moving sphere (training family)."""
+    rng = np.random.default_rng(seed)
+    E, nu = W.heterogeneous(6 * n ** 3, rng)
+    mesh = W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.01, faces=True, rng=rng)
+    gpu = X.Solver(mesh, dt=1e-3, iters=20, solver="gs")
+    L = 0.01 * n
+    r = rng.uniform(0.015, 0.03)
+    c = rng.uniform(0.3 * L, 0.7 * L, 2)
+    gpu.set_sdf(X.SDF_SPHERE, (c[0], c[1], L + r - 0.0005), (0, 0, 0, 1), (r, 0, 0))
+    gpu.set_sdf_velocity((0.0, 0.0, -rng.uniform(0.05, 0.2)))
+    net = X.Network()
+    gpu.attach(net)
+    return gpu.collect_training(steps)
+
+
+def test_trained_network_predicts_the_solver_correction():
+    """What the network gains over the method without it: the warm start
+    x~ + c_hat is closer to the plain solver's end-of-step positions than the
+    inertial prediction x~ (c_hat = 0) or the previous step's correction."""
+    xs, ys = zip(*[_trajectory(s) for s in (21, 22, 23, 24)])
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    xe, ye = _trajectory(99)
+    net = X.Network()
+    net.fit_scaling(x, y)
+    net.train(x, y, batch=2048, epochs=40, lr=2e-3, seed=3)
+    pred = net.forward(xe)
+    err_net = np.linalg.norm(pred - ye) / np.linalg.norm(ye)
+    err_prev = np.linalg.norm(xe[:, 15:18] - ye) / np.linalg.norm(ye)
+    print(f"warm-start error / |correction|: network {err_net:.3f}, previous correction {err_prev:.3f}, "
+          f"none 1.000 ({len(x)} training rows, {len(xe)} held-out rows)")
+    assert err_net < 0.5
+    assert err_net < err_prev

@@ -233,6 +233,25 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
         const double em = vol[e] * rho[e];
         for (int j = 0; j < 4; j++) h->mass[tets[4 * (size_t)e + j]] += em / 4.0;
     }
+    // per-vertex material summary (learned warm-start features)
+    h->vmat.assign(2 * (size_t)nv, 0.f);
+    {
+        std::vector<double> sE(nv, 0.0), sN(nv, 0.0);
+        std::vector<int32_t> cnt(nv, 0);
+        for (int e = 0; e < nt; e++)
+            for (int j = 0; j < 4; j++) {
+                const int32_t v = tets[4 * (size_t)e + j];
+                sE[v] += std::log10(E[e]);
+                sN[v] += nu[e];
+                cnt[v]++;
+            }
+        for (int v = 0; v < nv; v++)
+            if (cnt[v]) {
+                h->vmat[2 * (size_t)v] = (float)(sE[v] / cnt[v]);
+                h->vmat[2 * (size_t)v + 1] = (float)(sN[v] / cnt[v]);
+            }
+    }
+    h->tets_host = tets;
     // compliances (DeviatoricConstraint.cpp:13, HydrostaticConstraint.cpp:15-16,
     // ElasticMaterial.hpp:31-32) and alpha tilde = alpha / dt^2
     const double dt2 = P->dt * P->dt;
@@ -817,7 +836,7 @@ extern "C" int xpbd_set_state(xpbd_handle* h, const double* x, const double* v)
         hipError_t e = hipMemcpy(h->d_vel, hv.data(), sizeof(float4) * h->nv, hipMemcpyHostToDevice);
         if (e != hipSuccess) return hip_err(h, e, "set_state");
     }
-    return XPBD_OK;
+    return learned_reset_history(h);
 }
 
 extern "C" int xpbd_get_masses(xpbd_handle* h, double* out)

@@ -25,6 +25,8 @@ struct xpbd_handle {
     std::vector<double> rest;                  // 3*nv rest positions (FP64, host copy)
     std::vector<double> mass;                  // nv lumped masses
     std::vector<uint8_t> pinned;
+    std::vector<int32_t> tets_host;            // 4*nt vertex ids in the caller's element order
+    std::vector<float> vmat;                   // 2*nv: mean log10(E) and mean nu of incident elements
     std::vector<int32_t> order;                // solver position -> caller's tet index
     std::vector<int32_t> color_off;            // colour c occupies [color_off[c], color_off[c+1])
                                                // (tets, or clusters for the cluster variant)
@@ -121,4 +123,6 @@ int download_f4(xpbd_handle* h, std::vector<float4>& hu, const float4* src, size
 int device_residual(xpbd_handle* h, double* norm, double* vec_solver_order_dev);
 int enqueue_step_residual(xpbd_handle* h);
 void destroy_learned(xpbd_handle* h);
+// Clears the previous-step correction the learned warm start reads (state was replaced).
+int learned_reset_history(xpbd_handle* h);
 }  // namespace xpbd

@@ -109,6 +109,8 @@ _sig("xpbd_collect_training", [_vp, C.c_int32, C.c_int32, _fp, _fp, _ip])
 _sig("xpbd_features", [_vp, _fp])
 _sig("xpbd_attach_mlp", [_vp, _vp])
 _sig("xpbd_step_learned", [_vp, C.c_int32, C.c_double, C.c_double, _ip, _dp])
+_sig("xpbd_learned_stats", [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _dp, C.POINTER(C.c_int64)])
+_sig("xpbd_mlp_last_error", [_vp], C.c_char_p)
 
 
 def ptr(a: Optional[np.ndarray], ct):
@@ -383,3 +385,114 @@ class Solver:
 
     def autotune(self, cache_path: Optional[str] = None):
         self._c(lib.xpbd_autotune(self.h, None if cache_path is None else cache_path.encode()))
+
+    # learned warm start
+    def attach(self, net: Optional["Network"]):
+        self._net = net
+        self._c(lib.xpbd_attach_mlp(self.h, None if net is None else net.m))
+
+    def features(self) -> np.ndarray:
+        out = np.empty((self.nv, MLP_IN), np.float32)
+        self._c(lib.xpbd_features(self.h, ptr(out, C.c_float)))
+        return out
+
+    def collect_training(self, nsteps: int, cap: Optional[int] = None):
+        """(features[rows, 24], corrections[rows, 3]) of nsteps plain steps (free vertices)."""
+        nfree = int(sum((m.pinned == 0).sum() if m.pinned is not None else m.nv for m in self.meshes))
+        cap = nsteps * nfree if cap is None else cap
+        x = np.empty((max(cap, 1), MLP_IN), np.float32)
+        y = np.empty((max(cap, 1), 3), np.float32)
+        rows = C.c_int32()
+        self._c(lib.xpbd_collect_training(self.h, nsteps, cap, ptr(x, C.c_float), ptr(y, C.c_float),
+                                          C.byref(rows)))
+        return x[:rows.value], y[:rows.value]
+
+    def step_learned(self, iters: int, ref_residual: float = 0.0, ratio: float = 1.1):
+        """One learned step; returns (fell_back, residual)."""
+        fb, r = C.c_int32(), C.c_double()
+        self._c(lib.xpbd_step_learned(self.h, iters, ref_residual, ratio, C.byref(fb), C.byref(r)))
+        return bool(fb.value), r.value
+
+    def learned_stats(self):
+        st, fb, n = C.c_int64(), C.c_int64(), C.c_int64()
+        ms = C.c_double()
+        self._c(lib.xpbd_learned_stats(self.h, C.byref(st), C.byref(fb), C.byref(ms), C.byref(n)))
+        return {"steps": st.value, "fallbacks": fb.value, "mlp_ms": ms.value, "mlp_calls": n.value}
+
+
+MLP_IN, MLP_HIDDEN, MLP_OUT = 24, 128, 3
+
+
+class Network:
+    """The per-vertex warm-start MLP (24 -> 128 -> 128 -> 3, SiLU) on the device."""
+
+    def __init__(self, params: Optional[np.ndarray] = None):
+        m = _vp()
+        _check(lib.xpbd_mlp_create(C.byref(m)), None)
+        self.m = m
+        if params is not None:
+            self.set_params(params)
+
+    def close(self):
+        if getattr(self, "m", None):
+            lib.xpbd_mlp_destroy(self.m)
+            self.m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        if rc != 0:
+            msg = lib.xpbd_mlp_last_error(self.m)
+            raise XpbdError(rc, msg.decode() if msg else "")
+
+    def params(self) -> np.ndarray:
+        out = np.empty(MLP_PARAMS, np.float32)
+        self._c(lib.xpbd_mlp_get_params(self.m, ptr(out, C.c_float)))
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, np.float32).reshape(MLP_PARAMS)
+        self._c(lib.xpbd_mlp_set_params(self.m, ptr(a, C.c_float)))
+
+    def set_scaling(self, mean, std, scale_out):
+        mu = np.ascontiguousarray(mean, np.float32).reshape(MLP_IN)
+        sd = np.ascontiguousarray(std, np.float32).reshape(MLP_IN)
+        self._c(lib.xpbd_mlp_set_scaling(self.m, ptr(mu, C.c_float), ptr(sd, C.c_float), float(scale_out)))
+
+    def forward(self, x) -> np.ndarray:
+        xa = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_IN)
+        out = np.empty((len(xa), MLP_OUT), np.float32)
+        self._c(lib.xpbd_mlp_forward(self.m, len(xa), ptr(xa, C.c_float), ptr(out, C.c_float)))
+        return out
+
+    def loss_grad(self, x, y):
+        xa = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_IN)
+        ya = np.ascontiguousarray(y, np.float32).reshape(-1, MLP_OUT)
+        loss = C.c_double()
+        g = np.empty(MLP_PARAMS, np.float32)
+        self._c(lib.xpbd_mlp_loss_grad(self.m, len(xa), ptr(xa, C.c_float), ptr(ya, C.c_float), C.byref(loss),
+                                       ptr(g, C.c_float)))
+        return loss.value, g
+
+    def train(self, x, y, batch=4096, epochs=10, lr=1e-3, seed=0):
+        xa = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_IN)
+        ya = np.ascontiguousarray(y, np.float32).reshape(-1, MLP_OUT)
+        loss = C.c_double()
+        self._c(lib.xpbd_mlp_train(self.m, len(xa), ptr(xa, C.c_float), ptr(ya, C.c_float), batch, epochs,
+                                   float(lr), seed, C.byref(loss)))
+        return loss.value
+
+    def fit_scaling(self, x, y):
+        """Standardise inputs by their mean/std and outputs by their RMS (std floor 1e-12)."""
+        x = np.asarray(x, np.float64)
+        mu = x.mean(0)
+        sd = x.std(0)
+        # a (near-)constant input keeps a unit scale instead of amplifying noise
+        sd = np.where(sd > 1e-6 * np.abs(mu) + 1e-12, sd, np.maximum(np.abs(mu), 1.0))
+        sc = float(max(np.sqrt(np.mean(np.asarray(y, np.float64) ** 2)), 1e-12))
+        self.set_scaling(mu, sd, sc)
+        return mu, sd, sc
