@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--learned-steps", type=int, default=50, help="steps of the learned-vs-plain comparison (0: off)")
     ap.add_argument("--learned-iters", type=int, default=3)
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="MiB moved once by a plain device copy (scale for rocprof HBM byte counters)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xpbd-tissue-sim_amd", "weights", "warmstart.npz"))
     return ap.parse_args()
 
@@ -171,6 +173,15 @@ def main():
                                    "described in DESIGN.md but not built)"}))
         sys.exit(2)
     import xpbd_amd as X  # noqa: F401  (fails loudly if the HIP library is missing)
+    if a.copy_calibration > 0:
+        # a copy of known size, so that FETCH_SIZE / WRITE_SIZE per dispatch can be scaled
+        import torch
+        n = int(a.copy_calibration * (1 << 20)) // 4
+        src = torch.ones(n, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": 4 * n}), file=sys.stderr)
     s, meshes, cfg, sdf = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
     cfg["variant"] = a.variant
     cfg["deterministic"] = not a.nondeterministic

@@ -1,0 +1,120 @@
+// Command-line driver of the host layer: runs a simulation config on the GPU
+// as fast as possible and writes the per-step log.
+//
+//   xpbd_sim CONFIG.yaml [--beam] [--steps N] [--dump-positions FILE]
+//            [--dump-order FILE] [--dump-face-rank FILE] [--print-config]
+//
+// --beam            BeamSimulation set-up (fix the minimum-Y side, log tip
+//                   deflection and constraint residual)
+// --steps N         stop after N steps (default: end-time / time-step)
+// --dump-positions  final vertex positions of object 0, raw float64 (V x 3)
+// --dump-order      the device's element order of object 0, raw int32 (T)
+// --dump-face-rank  the device's contact visiting rank of object 0's faces, raw int32
+// --print-config    parse the config, print a summary, touch no GPU
+// --load-meshes     with --print-config: also load and place each mesh (host only)
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+
+#include "sim_api.hpp"
+
+static const char* solver_name(Config::XPBDObjectSolverTypeEnum t)
+{
+    switch (t) {
+    case Config::XPBDObjectSolverTypeEnum::GAUSS_SEIDEL: return "Gauss-Seidel";
+    case Config::XPBDObjectSolverTypeEnum::JACOBI: return "Jacobi";
+    case Config::XPBDObjectSolverTypeEnum::PARALLEL_JACOBI: return "Parallel-Jacobi";
+    case Config::XPBDObjectSolverTypeEnum::GPU_GAUSS_SEIDEL: return "GPU-Gauss-Seidel";
+    }
+    return "?";
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s CONFIG.yaml [--beam] [--steps N] [--dump-positions FILE] "
+                             "[--dump-order FILE] [--dump-face-rank FILE] [--print-config]\n", argv[0]);
+        return 2;
+    }
+    std::string cfg_path = argv[1], dump_pos, dump_order, dump_rank;
+    bool beam = false, print_only = false, load_meshes = false;
+    long steps = -1;
+    for (int i = 2; i < argc; i++) {
+        const std::string a = argv[i];
+        if (a == "--beam") beam = true;
+        else if (a == "--print-config") print_only = true;
+        else if (a == "--load-meshes") load_meshes = true;
+        else if (a == "--steps" && i + 1 < argc) steps = std::atol(argv[++i]);
+        else if (a == "--dump-positions" && i + 1 < argc) dump_pos = argv[++i];
+        else if (a == "--dump-order" && i + 1 < argc) dump_order = argv[++i];
+        else if (a == "--dump-face-rank" && i + 1 < argc) dump_rank = argv[++i];
+        else {
+            std::fprintf(stderr, "unknown argument %s\n", a.c_str());
+            return 2;
+        }
+    }
+    try {
+        const Config::SimulationConfig cfg = Config::SimulationConfig::from_file(cfg_path);
+        if (print_only) {
+            std::printf("{\"name\": \"%s\", \"time_step\": %.9g, \"end_time\": %.9g, \"g_accel\": %.9g, "
+                        "\"collision_rate\": %.9g, \"objects\": [",
+                        cfg.name.c_str(), cfg.time_step, cfg.end_time, cfg.g_accel, cfg.collision_rate);
+            for (size_t i = 0; i < cfg.xpbd_objects.size(); i++) {
+                const auto& o = cfg.xpbd_objects[i];
+                std::printf("%s{\"name\": \"%s\", \"solver_type\": \"%s\", \"iters\": %d, \"E\": %.9g, \"nu\": %.9g, "
+                            "\"density\": %.9g, \"collisions\": %s, \"source\": \"%s\", \"position\": [%.9g, %.9g, %.9g]",
+                            i ? ", " : "", o.name.c_str(), solver_name(o.solver_type), o.num_solver_iters,
+                            o.material.E, o.material.nu, o.material.density, o.collisions ? "true" : "false",
+                            o.grid ? "grid" : o.filename.c_str(), o.position[0], o.position[1], o.position[2]);
+                if (load_meshes) {
+                    // host-side mesh set-up only (no device call)
+                    Sim::XPBDMeshObject m(o, cfg.time_step, cfg.g_accel);
+                    const Sim::AABB b = m.boundingBox();
+                    m.descriptor();
+                    std::printf(", \"vertices\": %d, \"elements\": %d, \"surface_faces\": %d, "
+                                "\"bbox\": [[%.9g, %.9g, %.9g], [%.9g, %.9g, %.9g]]",
+                                m.numVertices(), m.numElements(), m.numFaces(), b.min[0], b.min[1], b.min[2], b.max[0],
+                                b.max[1], b.max[2]);
+                }
+                std::printf("}");
+            }
+            std::printf("], \"rigid_objects\": %zu}\n", cfg.rigid_objects.size());
+            return 0;
+        }
+        std::unique_ptr<Sim::Simulation> sim =
+            beam ? std::make_unique<Sim::BeamSimulation>(cfg) : std::make_unique<Sim::Simulation>(cfg);
+        sim->setup();
+        sim->run(steps);
+        std::cout << sim->logText();
+        std::fprintf(stderr, "%lld steps, t = %.6g s\n", (long long)sim->steps(), sim->time());
+        if (!sim->objects().empty()) {
+            Sim::XPBDMeshObject& o = *sim->objects()[0];
+            if (!dump_pos.empty()) {
+                std::ofstream f(dump_pos, std::ios::binary);
+                f.write(reinterpret_cast<const char*>(o.vertices().data()), sizeof(double) * o.vertices().size());
+            }
+            if (!dump_order.empty()) {
+                std::vector<int32_t> order(o.numElements());
+                const int rc = xpbd_get_tet_order(o.solver().handle(), order.data());
+                if (rc != XPBD_OK) throw Sim::SimError(rc, "xpbd_get_tet_order");
+                std::ofstream f(dump_order, std::ios::binary);
+                f.write(reinterpret_cast<const char*>(order.data()), sizeof(int32_t) * order.size());
+            }
+            if (!dump_rank.empty()) {
+                std::vector<int32_t> rank(o.numFaces());
+                if (!rank.empty()) {
+                    const int rc = xpbd_get_face_rank(o.solver().handle(), rank.data());
+                    if (rc != XPBD_OK) throw Sim::SimError(rc, "xpbd_get_face_rank");
+                }
+                std::ofstream f(dump_rank, std::ios::binary);
+                f.write(reinterpret_cast<const char*>(rank.data()), sizeof(int32_t) * rank.size());
+            }
+        }
+    } catch (const Sim::SimError& e) {
+        std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
+        return 1;
+    }
+    return 0;
+}
