@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--learned-steps", type=int, default=50, help="steps of the learned-vs-plain comparison (0: off)")
     ap.add_argument("--learned-iters", type=int, default=3)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step eagerly (kernel timing mode) instead of replaying HIP graphs")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="MiB moved once by a plain device copy (scale for rocprof HBM byte counters)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xpbd-tissue-sim_amd", "weights", "warmstart.npz"))
@@ -189,6 +191,8 @@ def main():
     T, V = s.nt, s.nv
     iters = cfg["iterations"]
 
+    if a.no_graph:
+        s.set_kernel_timing(True)
     s.step(a.warmup)
     s.sync()
     t0 = time.perf_counter()
@@ -231,9 +235,10 @@ def main():
         "residual_after_run": res,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "element projection (all colour launches of one iteration)",
+                     "kernel": ("k_gs_cluster (the colour launches of one iteration)" if cfg["solver"] == "gs"
+                                else "k_jacobi_patch + k_jacobi_shared (one iteration)"),
                      "us_per_iteration": us_per_iter, "algorithmic_bytes_per_iteration": bytes_iter,
-                     "launches_timed": nlaunch},
+                     "iterations_timed": nit, "graph_replay": not a.no_graph},
     }
     if a.learned_steps > 0 and a.workload != "batch":
         s.close()

@@ -197,9 +197,12 @@ int xpbd_get_contacts(xpbd_handle* h, int32_t cap, int32_t* count, int32_t* face
 /* ---- timing --------------------------------------------------------------- */
 
 /* Kernel timing mode: when on, xpbd_step launches without the HIP graph and
- * brackets every projection-kernel launch with HIP events on the handle's
- * stream; xpbd_kernel_stats returns the summed milliseconds and the number of
- * solver iterations timed since the last reset. */
+ * brackets the element-projection launches of every solver iteration (all
+ * colour launches of a Gauss-Seidel iteration, back to back; the patch and
+ * shared-vertex launches of a Jacobi iteration) with one pair of HIP events on
+ * the handle's stream; xpbd_kernel_stats returns the summed milliseconds, the
+ * number of solver iterations and the number of timed spans since the last
+ * reset. */
 int xpbd_set_kernel_timing(xpbd_handle* h, int32_t on);
 int xpbd_kernel_stats(xpbd_handle* h, double* proj_ms, int64_t* iterations, int64_t* launches);
 

@@ -550,16 +550,17 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
     for (int it = 0; it < iters; it++) {
         const bool first = it == 0;
         if (h->params.solver == XPBD_SOLVER_GS_COLORED) {
+            // one timed span covers the colour launches of the iteration (back to back on the stream)
+            if (timed) timed_begin(h);
             for (int c = 0; c < h->ncolors; c++) {
-                if (timed) timed_begin(h);
                 if (h->variant == XPBD_VARIANT_CLUSTER)
                     e = launch_gs_cluster(s, ta, h->d_cverts, h->d_cstart, h->d_tloc, h->d_pos, h->color_off[c],
                                           h->color_off[c + 1], first);
                 else
                     e = launch_gs_color(s, ta, h->d_pos, h->color_off[c], h->color_off[c + 1], first);
                 if (e != hipSuccess) return hip_err(h, e, "gs_color launch");
-                if (timed) timed_end(h);
             }
+            if (timed) timed_end(h);
             if (con) {
                 e = launch_contacts_gs(s, h->d_faces, h->d_fcol_off, h->nfcolors, h->d_cpos, h->d_clist, h->d_rest,
                                        h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);

@@ -2,7 +2,7 @@
 // as fast as possible and writes the per-step log.
 //
 //   xpbd_sim CONFIG.yaml [--beam] [--steps N] [--dump-positions FILE]
-//            [--dump-order FILE] [--dump-face-rank FILE] [--print-config]
+//            [--dump-order FILE] [--dump-face-rank FILE] [--dump-rest FILE] [--print-config]
 //
 // --beam            BeamSimulation set-up (fix the minimum-Y side, log tip
 //                   deflection and constraint residual)
@@ -10,6 +10,7 @@
 // --dump-positions  final vertex positions of object 0, raw float64 (V x 3)
 // --dump-order      the device's element order of object 0, raw int32 (T)
 // --dump-face-rank  the device's contact visiting rank of object 0's faces, raw int32
+// --dump-rest       rest positions of object 0 after placement, raw float64 (V x 3)
 // --print-config    parse the config, print a summary, touch no GPU
 // --load-meshes     with --print-config: also load and place each mesh (host only)
 #include <cstdio>
@@ -35,10 +36,10 @@ int main(int argc, char** argv)
 {
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s CONFIG.yaml [--beam] [--steps N] [--dump-positions FILE] "
-                             "[--dump-order FILE] [--dump-face-rank FILE] [--print-config]\n", argv[0]);
+                             "[--dump-order FILE] [--dump-face-rank FILE] [--dump-rest FILE] [--print-config]\n", argv[0]);
         return 2;
     }
-    std::string cfg_path = argv[1], dump_pos, dump_order, dump_rank;
+    std::string cfg_path = argv[1], dump_pos, dump_order, dump_rank, dump_rest;
     bool beam = false, print_only = false, load_meshes = false;
     long steps = -1;
     for (int i = 2; i < argc; i++) {
@@ -50,6 +51,7 @@ int main(int argc, char** argv)
         else if (a == "--dump-positions" && i + 1 < argc) dump_pos = argv[++i];
         else if (a == "--dump-order" && i + 1 < argc) dump_order = argv[++i];
         else if (a == "--dump-face-rank" && i + 1 < argc) dump_rank = argv[++i];
+        else if (a == "--dump-rest" && i + 1 < argc) dump_rest = argv[++i];
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -94,6 +96,11 @@ int main(int argc, char** argv)
             if (!dump_pos.empty()) {
                 std::ofstream f(dump_pos, std::ios::binary);
                 f.write(reinterpret_cast<const char*>(o.vertices().data()), sizeof(double) * o.vertices().size());
+            }
+            if (!dump_rest.empty()) {
+                std::ofstream f(dump_rest, std::ios::binary);
+                f.write(reinterpret_cast<const char*>(o.restVertices().data()),
+                        sizeof(double) * o.restVertices().size());
             }
             if (!dump_order.empty()) {
                 std::vector<int32_t> order(o.numElements());
