@@ -25,6 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "xpbd-tissue-sim_amd"))
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {"cluster": "k_gs_cluster x colours", "quad": "k_gs_cluster_quad x colours",
+                "simple": "k_gs_color x colours / k_jacobi_project", "patch": "k_jacobi_patch + k_jacobi_shared"}
 
 
 def parse():
@@ -35,7 +37,7 @@ def parse():
     ap.add_argument("--workload", default="block", choices=["block", "beam", "cube", "batch"])
     ap.add_argument("--solver", default=None, choices=["gs", "jacobi"])
     ap.add_argument("--iters", type=int, default=None)
-    ap.add_argument("--variant", default="auto", choices=["auto", "simple", "cluster", "patch"])
+    ap.add_argument("--variant", default="auto", choices=["auto", "simple", "cluster", "patch", "quad"])
     ap.add_argument("--nondeterministic", action="store_true", help="Jacobi: LDS float atomics")
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps in the CPU baseline sample")
@@ -77,18 +79,47 @@ def build_workload(name, solver, iters, variant="auto", det=1):
                "solver": solver or "gs", "iterations": iters or 5, "dt": 1e-3}
         return s, [mesh], cfg, None
     meshes, sdfs = W.batch(256)
-    s = X.Solver(meshes, dt=1e-3, iters=iters or 10, solver=solver or "jacobi", collision_every=1, **kw)
+    s = X.Solver(meshes, dt=1e-3, iters=iters or 10, solver=solver or "gs", collision_every=1, **kw)
     for i, (kind, pos, quat, prm) in enumerate(sdfs):
         s.set_sdf(kind, pos, quat, prm, instance=i)
     cfg = {"workload": "256 cubes of 4..24 cells per axis x6 tets, box contacts", "instances": 256,
-           "tets": s.nt, "vertices": s.nv, "solver": solver or "jacobi", "iterations": iters or 10, "dt": 1e-3}
-    return s, meshes, cfg, None
+           "tets": s.nt, "vertices": s.nv, "solver": solver or "gs", "iterations": iters or 10, "dt": 1e-3}
+    return s, meshes, cfg, list(sdfs)
+
+
+def cpu_baseline_batch(meshes, cfg, sdfs, nsteps, stride=16):
+    """FP64 oracle, one thread, on every stride-th instance of the batch; the
+    reference steps objects one after another (Simulation.cpp:246-255), so the
+    whole batch costs the sum over instances, extrapolated by element count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    idx = list(range(0, len(meshes), stride))
+    el = 0.0
+    for i in idx:
+        o = orc.Sim(meshes[i], dt=cfg["dt"], iters=cfg["iterations"], mode=cfg["solver"])
+        kind, pos, quat, prm = sdfs[i]
+        o.set_sdf(kind, pos, quat, prm)
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            o.step(1, detect=True)
+        el += time.perf_counter() - t0
+    t_sample = sum(meshes[i].nt for i in idx)
+    t_all = sum(m.nt for m in meshes)
+    el_all = el * t_all / t_sample
+    return {"value": nsteps / el_all, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"{nsteps} steps of {len(idx)} of the {len(meshes)} instances (every {stride}th, "
+                      f"{t_sample} of {t_all} tets), {cfg['iterations']} {cfg['solver']} iterations, box contacts, "
+                      f"one host core, FP64 oracle; whole-batch time extrapolated by element count",
+            "tet_proj_per_s": nsteps * cfg["iterations"] * t_sample / el,
+            "host": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
 
 
 def cpu_baseline(meshes, cfg, sdf, nsteps):
     """FP64 oracle, one thread, on the first mesh of the workload (bounded sample)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
+    if isinstance(sdf, list):
+        return cpu_baseline_batch(meshes, cfg, sdf, nsteps)
     mesh = meshes[0]
     o = orc.Sim(mesh, dt=cfg["dt"], iters=cfg["iterations"], mode=cfg["solver"])
     if sdf is not None:
@@ -185,6 +216,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_each_way": 4 * n}), file=sys.stderr)
     s, meshes, cfg, sdf = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
+    if a.variant == "auto":
+        # time the variants that share this handle's layout and keep the fastest
+        cfg["autotuned"] = list(s.autotune())
     cfg["variant"] = a.variant
     cfg["deterministic"] = not a.nondeterministic
     cfg["colors"] = s.num_colors()
@@ -235,8 +269,7 @@ def main():
         "residual_after_run": res,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("k_gs_cluster (the colour launches of one iteration)" if cfg["solver"] == "gs"
-                                else "k_jacobi_patch + k_jacobi_shared (one iteration)"),
+                     "kernel": KERNEL_NAMES.get(s.variant()[0], s.variant()[0]) + " (one solver iteration)",
                      "us_per_iteration": us_per_iter, "algorithmic_bytes_per_iteration": bytes_iter,
                      "iterations_timed": nit, "graph_replay": not a.no_graph},
     }
@@ -244,7 +277,7 @@ def main():
         s.close()
         out["learned"] = learned_vs_plain(a, a.weights)
     if not a.no_cpu:
-        n = a.cpu_steps if a.cpu_steps is not None else (5 if a.workload == "block" else 20)
+        n = a.cpu_steps if a.cpu_steps is not None else {"block": 10, "batch": 40}.get(a.workload, 20)
         out["cpu_baseline"] = cpu_baseline(meshes, cfg, sdf, n)
     print(json.dumps(out))
 

@@ -45,11 +45,14 @@ extern "C" {
  * per tet colour; Jacobi: float atomics).  CLUSTER (Gauss-Seidel): runs of
  * consecutive tets sharing at most 8 vertices form clusters, clusters are
  * coloured, one thread projects its cluster's tets in order.  PATCH (Jacobi):
- * workgroups stage a patch of tets and their vertices in LDS. */
+ * workgroups stage a patch of tets and their vertices in LDS.  CLUSTER_QUAD
+ * (Gauss-Seidel): the CLUSTER order, four lanes per cluster (one per vertex of
+ * the element being projected). */
 #define XPBD_VARIANT_AUTO 0
 #define XPBD_VARIANT_SIMPLE 1
 #define XPBD_VARIANT_CLUSTER 2
 #define XPBD_VARIANT_PATCH 3
+#define XPBD_VARIANT_CLUSTER_QUAD 4
 
 /* Signed-distance field kinds (src/geometry/SphereSDF.cpp, BoxSDF.cpp). */
 #define XPBD_SDF_NONE 0
@@ -218,9 +221,17 @@ int xpbd_batch_num_instances(xpbd_batch* b, int32_t* out);
 int xpbd_batch_set_sdf(xpbd_batch* b, int32_t instance, int32_t kind, const double* pose7,
                        const double* params3);
 int xpbd_batch_get_positions(xpbd_batch* b, int32_t instance, double* out);
-/* Time the kernel variants and launch shapes on this batch and keep the
- * fastest; results are cached in cache_path (may be NULL) keyed by shape. */
+/* Time the kernel variants and launch shapes that share this handle's device
+ * layout on its own problem (Gauss-Seidel with clusters: the one-thread
+ * cluster kernel and the quad kernel at 64/128/256 threads per workgroup),
+ * restore the state, and keep the fastest.  The choice is appended to
+ * cache_path (may be NULL) as "shape-key variant block ms-per-iteration" and
+ * reused from there for the same shape.  No reference counterpart: the
+ * reference has no kernel variants. */
 int xpbd_autotune(xpbd_batch* b, const char* cache_path);
+/* The kernel variant in use (XPBD_VARIANT_*) and, for CLUSTER_QUAD, the
+ * workgroup size (0 otherwise). */
+int xpbd_get_variant(xpbd_handle* h, int32_t* variant, int32_t* block);
 /* Batch handles are also plain handles: xpbd_step, xpbd_destroy, ... apply. */
 
 /* ---- host-side mesh utilities (no GPU needed) ------------------------------ */

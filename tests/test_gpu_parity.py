@@ -27,7 +27,7 @@ def _pair(mesh, solver, iters, dt=1e-3, g=9.81, variant="auto", det=1):
 
 # every kernel variant: GS one-thread-per-tet colours, GS clusters; Jacobi
 # atomics, Jacobi LDS patches (float-atomic and fixed-order accumulation)
-VARIANTS = [("gs", "simple", 1), ("gs", "cluster", 1), ("jacobi", "simple", 0), ("jacobi", "patch", 0),
+VARIANTS = [("gs", "simple", 1), ("gs", "cluster", 1), ("gs", "quad", 1), ("jacobi", "simple", 0), ("jacobi", "patch", 0),
             ("jacobi", "patch", 1)]
 
 
@@ -41,7 +41,7 @@ def test_cube_100_steps(solver, variant, det):
     assert errs.max() <= TOL
 
 
-@pytest.mark.parametrize("solver,E,variant", [("gs", 1e4, "cluster"), ("gs", 1e4, "simple"),
+@pytest.mark.parametrize("solver,E,variant", [("gs", 1e4, "cluster"), ("gs", 1e4, "simple"), ("gs", 1e4, "quad"),
                                                ("jacobi", 1e3, "patch"), ("jacobi", 1e3, "simple")])
 def test_stiff_beam_100_steps(solver, E, variant):
     """nu = 0.49 beam.  Gauss-Seidel at the beam configuration's E = 1e4; the
@@ -55,7 +55,7 @@ def test_stiff_beam_100_steps(solver, E, variant):
     assert errs.max() <= TOL
 
 
-@pytest.mark.parametrize("solver,E_range,variant", [("gs", (1e3, 5e4), "cluster"), ("gs", (1e3, 5e4), "simple"),
+@pytest.mark.parametrize("solver,E_range,variant", [("gs", (1e3, 5e4), "cluster"), ("gs", (1e3, 5e4), "simple"), ("gs", (1e3, 5e4), "quad"),
                                                      ("jacobi", (3e2, 1e3), "patch"), ("jacobi", (3e2, 1e3), "simple")])
 def test_heterogeneous_block_with_sphere(solver, E_range, variant):
     """Per-tet E (log-uniform) and nu, base pinned, sphere pressed down at 0.1 m/s,
@@ -145,3 +145,30 @@ def test_cluster_order_is_a_valid_gauss_seidel_order():
     order = gpu.tet_order()
     assert sorted(order) == list(range(mesh.nt))
     assert gpu.num_colors() <= 12          # cells of a grid need 8 colours
+
+
+def test_autotune_keeps_results_and_caches_its_choice(tmp_path):
+    """The tuner's trial iterations are undone, its choice shares the cluster
+    order (same results up to FP32 summation order), and a cached choice is
+    reused without timing."""
+    mesh = W.cube(8, seed=21)
+    cache = str(tmp_path / "tune.txt")
+    a = X.Solver(mesh, dt=1e-3, iters=10, solver="gs", variant="cluster")
+    b = X.Solver(mesh, dt=1e-3, iters=10, solver="gs", variant="cluster")
+    a.step(3)
+    b.step(3)
+    choice = a.autotune(cache)
+    assert choice[0] in ("cluster", "quad")
+    assert np.array_equal(a.positions(), b.positions())  # trial iterations left no trace
+    a.step(20)
+    b.step(20)
+    err = rel_err(a.positions(), b.positions())
+    print(f"autotune chose {choice}; rel diff to the untuned run after 20 steps {err:.2e}")
+    assert err <= 1e-6
+    with open(cache) as f:
+        lines = f.read().splitlines()
+    assert len(lines) == 1 and lines[0].startswith("gs:")
+    c = X.Solver(mesh, dt=1e-3, iters=10, solver="gs", variant="cluster")
+    assert c.autotune(cache) == choice
+    with open(cache) as f:
+        assert len(f.read().splitlines()) == 1  # reused, not re-timed

@@ -84,7 +84,6 @@ PatchArrays patch_arrays(xpbd_handle* h)
     pa.partial = h->d_partial;
     pa.shared = h->d_shared;
     pa.soff = h->d_soff;
-    pa.sslot = h->d_sslot;
     pa.npatch = h->patches.npatch;
     pa.maxl = h->patches.maxl;
     pa.nshared = (int)h->patches.shared.size();
@@ -268,15 +267,16 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     int variant = P->variant;
     if (variant == XPBD_VARIANT_AUTO)
         variant = P->solver == XPBD_SOLVER_GS_COLORED ? XPBD_VARIANT_CLUSTER : XPBD_VARIANT_PATCH;
+    const bool cluster = variant == XPBD_VARIANT_CLUSTER || variant == XPBD_VARIANT_CLUSTER_QUAD;
     if ((P->solver == XPBD_SOLVER_GS_COLORED && variant == XPBD_VARIANT_PATCH) ||
-        (P->solver == XPBD_SOLVER_JACOBI && variant == XPBD_VARIANT_CLUSTER) || variant < 0 || variant > 3) {
+        (P->solver == XPBD_SOLVER_JACOBI && cluster) || variant < 0 || variant > XPBD_VARIANT_CLUSTER_QUAD) {
         delete h;
         return set_err(nullptr, XPBD_E_INVALID_ARG, "variant does not apply to this solver");
     }
     h->variant = variant;
     std::vector<uint32_t> tloc_solver;
     std::vector<int32_t> cverts_solver;
-    if (P->solver == XPBD_SOLVER_GS_COLORED && variant == XPBD_VARIANT_CLUSTER) {
+    if (P->solver == XPBD_SOLVER_GS_COLORED && cluster) {
         // clusters of consecutive tets, coloured; solver order = colour, cluster, tet
         std::vector<int32_t> cst, cv, ccol;
         std::vector<uint32_t> tl;
@@ -392,7 +392,6 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
         rc |= dalloc(h, &h->d_partial, (size_t)Pt.nslots);
         rc |= dalloc(h, &h->d_shared, Pt.shared.size());
         rc |= dalloc(h, &h->d_soff, Pt.soff.size());
-        rc |= dalloc(h, &h->d_sslot, Pt.sslot.size());
     }
     if (h->nclusters > 0) {
         rc |= dalloc(h, &h->d_cverts, 2 * (size_t)h->nclusters);
@@ -473,7 +472,6 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
         UP(h->d_inc, Pt.inc.data(), sizeof(uint16_t) * Pt.inc.size());
         if (!Pt.shared.empty()) {
             UP(h->d_shared, Pt.shared.data(), sizeof(int32_t) * Pt.shared.size());
-            UP(h->d_sslot, Pt.sslot.data(), sizeof(int32_t) * Pt.sslot.size());
         }
         UP(h->d_soff, Pt.soff.data(), sizeof(int32_t) * Pt.soff.size());
     }
@@ -556,6 +554,9 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
                 if (h->variant == XPBD_VARIANT_CLUSTER)
                     e = launch_gs_cluster(s, ta, h->d_cverts, h->d_cstart, h->d_tloc, h->d_pos, h->color_off[c],
                                           h->color_off[c + 1], first);
+                else if (h->variant == XPBD_VARIANT_CLUSTER_QUAD)
+                    e = launch_gs_cluster_quad(s, ta, h->d_cverts, h->d_cstart, h->d_tloc, h->d_pos,
+                                               h->color_off[c], h->color_off[c + 1], first, h->quad_block);
                 else
                     e = launch_gs_color(s, ta, h->d_pos, h->color_off[c], h->color_off[c + 1], first);
                 if (e != hipSuccess) return hip_err(h, e, "gs_color launch");

@@ -32,6 +32,8 @@ struct xpbd_handle {
                                                // (tets, or clusters for the cluster variant)
     int ncolors = 1;
     int variant = XPBD_VARIANT_SIMPLE;         // resolved kernel variant
+    int quad_block = 256;                      // CLUSTER_QUAD workgroup size (autotuned)
+    std::string tuned;                         // last autotuner choice, "" if none
     std::vector<int32_t> cl_start;             // cluster variant: tet range of each cluster (solver order)
     int nclusters = 0;
     xpbd::Patches patches;                     // patch variant (host copy)
@@ -64,7 +66,6 @@ struct xpbd_handle {
     float4* d_partial = nullptr;
     int32_t* d_shared = nullptr;
     int32_t* d_soff = nullptr;
-    int32_t* d_sslot = nullptr;
     double* d_part = nullptr;
     int red_blocks = 1;
     double* d_scalar = nullptr;

@@ -39,6 +39,11 @@ hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int 
 // tloc the 4-bit local slots of each tet's vertices.
 hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                              const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first);
+// Same clusters and order, four lanes per cluster (one per vertex of the element
+// being projected); block = threads per workgroup (64, 128 or 256).
+hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
+                                  const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block);
+bool quad_block_supported(int block);
 hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float4* pos, float* acc, int nt,
                                  bool first);
 hipError_t launch_jacobi_apply(hipStream_t s, float4* pos, float* acc, int nv);
@@ -53,8 +58,7 @@ struct PatchArrays {
     const uint16_t* inc;     // per patch local incidence entries
     float4* partial;         // partial sums of shared vertices
     const int* shared;       // shared vertex ids
-    const int* soff;         // shared vertex -> slot range
-    const int* sslot;        // slots
+    const int* soff;         // shared vertex -> its contiguous range of partial slots
     int npatch, maxl, nshared;
 };
 int jacobi_patch_size();

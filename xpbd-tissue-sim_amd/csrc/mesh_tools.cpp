@@ -333,12 +333,17 @@ int build_patches(int nv, int nt, const int32_t* tets, const double* x, const in
             P.pslot[j] = P.nslots;
             vslots[shared_id[g]].push_back(P.nslots++);
         }
+    // renumber the slots so that those of one shared vertex are contiguous (in
+    // patch order): the shared pass then reads partial[soff[j] .. soff[j+1])
     P.soff.assign(1, 0);
-    P.sslot.clear();
+    std::vector<int32_t> renum(P.nslots, -1);
+    int32_t next = 0;
     for (auto& v : vslots) {
-        P.sslot.insert(P.sslot.end(), v.begin(), v.end());
-        P.soff.push_back((int32_t)P.sslot.size());
+        for (int32_t old : v) renum[old] = next++;
+        P.soff.push_back(next);
     }
+    for (int32_t& s : P.pslot)
+        if (s >= 0) s = renum[s];
     // ---- per-patch local incidence lists (local vertex -> tet corners)
     P.inc_off.clear();
     P.inc.assign((size_t)P.npatch * 4 * patch, 0);

@@ -53,12 +53,13 @@ int build_clusters(int nt, const int32_t* tets, std::vector<int32_t>& start, std
 //   pslot[j]       for pverts entry j: -1 if the vertex is interior to the
 //                  patch (all incident tets inside), else its partial slot
 //   tloc[2i..2i+1] the 4 local vertex indices of tet i, 16 bits each
-//   shared/soff/sslot  CSR: shared vertex -> its partial slots (patch order)
+//   shared/soff  shared vertex j owns the contiguous partial slots
+//                [soff[j], soff[j+1]), in patch order
 //   inc_off/inc    per patch local CSR: local vertex -> (tet_local*4 + corner)
 //                  entries, offsets relative to the patch (deterministic mode)
 // Returns the largest local vertex count of any patch.
 struct Patches {
-    std::vector<int32_t> order, pvoff, pverts, pslot, shared, soff, sslot;
+    std::vector<int32_t> order, pvoff, pverts, pslot, shared, soff;  // soff: contiguous slots per shared vertex
     std::vector<uint32_t> tloc;
     std::vector<int32_t> inc_off;  // per patch: nl+1 offsets, stored at pvoff[p] + p
     std::vector<uint16_t> inc;     // per patch: 4*patch entries, stored at p*4*patch

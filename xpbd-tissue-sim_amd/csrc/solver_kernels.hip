@@ -239,6 +239,184 @@ __global__ __launch_bounds__(kClusterBlock) void k_gs_cluster(TetArrays ta, cons
 }
 
 // ---------------------------------------------------------------------------
+// Quad cluster Gauss-Seidel: the cluster order of k_gs_cluster, with four lanes
+// (one DPP quad) per cluster.  For each element, lane j owns the element's
+// vertex j: it forms its row of D = [x0-x3, x1-x3, x2-x3] and its share of
+// F = I + D Q, the quad sums F, and every lane evaluates C_D, C_H and the 2x2
+// system redundantly; lane j then computes the gradient column of its own
+// vertex (vertex 3 through Q's fourth column -(Q_0 + Q_1 + Q_2)), the quad sums
+// the three entries of the system matrix, and lane j corrects its own vertex.
+// A quad sum (x + swap1) + swap2 is the same sum in all four lanes, so the
+// lanes agree bit for bit.  The shorter per-lane chain and four times the
+// lanes per colour are the point: a colour holds only ~1/8 of the clusters.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float dpp_xor1(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_xor2(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float quad_sum(float x)
+{
+    x = x + dpp_xor1(x);
+    return x + dpp_xor2(x);
+}
+__device__ __forceinline__ float quad_lane3(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xFF, 0xF, 0xF, false));
+}
+
+// BLOCK threads per workgroup = BLOCK / 4 clusters; the launch shape is a tuning choice
+template <bool FIRST, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const int4* __restrict__ cverts,
+                                                          const int* __restrict__ cstart,
+                                                          const uint32_t* __restrict__ tloc,
+                                                          float4* __restrict__ pos, int cbeg, int cend)
+{
+    constexpr int kQuadBlock = BLOCK;
+    constexpr int kQuadClusters = BLOCK / 4;
+    constexpr int kQuadStage = kQuadClusters * 8;  // at most 8 tets per cluster
+    constexpr int kQuadCol = kQuadClusters + 1;    // padded column stride (LDS banks)
+    __shared__ float sb[8 * 4 * kQuadCol];   // [slot][xyz, w][cluster]
+    __shared__ float sd[8 * 3 * kQuadCol];   // [slot][xyz][cluster]
+    __shared__ float sq[12][kQuadStage];     // Q (9), alpha tilde (2), gamma
+    __shared__ float sl[2][kQuadStage];      // lambda
+    __shared__ uint32_t stl[kQuadStage];     // local vertex slots
+    const int tid = threadIdx.x;
+    const int cl = tid >> 2, j = tid & 3;
+    const int cb = cbeg + blockIdx.x * kQuadClusters;
+    const int cl_end = min(cend, cb + kQuadClusters);
+    const int tb = cstart[cb];
+    const int n = cstart[cl_end] - tb;
+    for (int i = tid; i < n; i += kQuadBlock) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) sq[k][i] = ta.Q[(size_t)k * ta.qstride + tb + i];
+        const float2 a = ta.at[tb + i];
+        sq[9][i] = a.x;
+        sq[10][i] = a.y;
+        sq[11][i] = ta.gamma[tb + i];
+        if (!FIRST) {
+            const float2 lv = ta.lam[tb + i];
+            sl[0][i] = lv.x;
+            sl[1][i] = lv.y;
+        }
+        stl[i] = tloc[tb + i];
+    }
+    const int c = cb + cl;
+    const bool active = c < cend;
+    int v_lo = -1, v_hi = -1;  // this lane stages and writes back slots j and j + 4
+    if (active) {
+        const int4 va = cverts[2 * (size_t)c], vb = cverts[2 * (size_t)c + 1];
+        v_lo = j == 0 ? va.x : j == 1 ? va.y : j == 2 ? va.z : va.w;
+        v_hi = j == 0 ? vb.x : j == 1 ? vb.y : j == 2 ? vb.z : vb.w;
+        if (v_lo >= 0) {
+            const float4 b = pos[2 * (size_t)v_lo], d = pos[2 * (size_t)v_lo + 1];
+            sb[(j * 4 + 0) * kQuadCol + cl] = b.x;
+            sb[(j * 4 + 1) * kQuadCol + cl] = b.y;
+            sb[(j * 4 + 2) * kQuadCol + cl] = b.z;
+            sb[(j * 4 + 3) * kQuadCol + cl] = b.w;
+            sd[(j * 3 + 0) * kQuadCol + cl] = d.x;
+            sd[(j * 3 + 1) * kQuadCol + cl] = d.y;
+            sd[(j * 3 + 2) * kQuadCol + cl] = d.z;
+        }
+        if (v_hi >= 0) {
+            const int q = j + 4;
+            const float4 b = pos[2 * (size_t)v_hi], d = pos[2 * (size_t)v_hi + 1];
+            sb[(q * 4 + 0) * kQuadCol + cl] = b.x;
+            sb[(q * 4 + 1) * kQuadCol + cl] = b.y;
+            sb[(q * 4 + 2) * kQuadCol + cl] = b.z;
+            sb[(q * 4 + 3) * kQuadCol + cl] = b.w;
+            sd[(q * 3 + 0) * kQuadCol + cl] = d.x;
+            sd[(q * 3 + 1) * kQuadCol + cl] = d.y;
+            sd[(q * 3 + 2) * kQuadCol + cl] = d.z;
+        }
+    }
+    __syncthreads();
+    // a quad is active or inactive as a whole, so DPP reads stay inside active quads
+    if (!active) return;
+    const int t0 = cstart[c] - tb, t1 = cstart[c + 1] - tb;
+    for (int e = t0; e < t1; e++) {
+        const int slot = (int)((stl[e] >> (4 * j)) & 15u);
+        float bb[3], dd[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            bb[k] = sb[(slot * 4 + k) * kQuadCol + cl];
+            dd[k] = sd[(slot * 3 + k) * kQuadCol + cl];
+        }
+        const float w = sb[(slot * 4 + 3) * kQuadCol + cl];
+        float q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) q[k] = sq[k][e];
+        // column j of [Q ; -(Q_0 + Q_1 + Q_2)] (row i of Q multiplies vertex i)
+        float qj[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+            const float q0 = q[3 * cc], q1 = q[3 * cc + 1], q2 = q[3 * cc + 2];
+            qj[cc] = j == 0 ? q0 : j == 1 ? q1 : j == 2 ? q2 : -(q0 + q1 + q2);
+        }
+        // this lane's row of D (zero for vertex 3)
+        float D[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) D[k] = (bb[k] - quad_lane3(bb[k])) + (dd[k] - quad_lane3(dd[k]));
+        const float dsel = j == 3 ? 0.f : 1.f;
+        float F[9];
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) F[3 * cc + r] = quad_sum(dsel * D[r] * qj[cc]) + (r == cc ? 1.f : 0.f);
+        float CD, CH, J;
+        element_constraints<float>(F, sq[11][e], &CD, &CH, &J);
+        const float inv_C = xrcp<float>(CD);
+        const float a = J - 1.f;
+        const float fac = (J >= 1.f) ? xrcp<float>(J) : 1.f - a + a * a;
+        float Fc[9];
+        Fc[0] = F[4] * F[8] - F[5] * F[7];
+        Fc[1] = F[5] * F[6] - F[3] * F[8];
+        Fc[2] = F[3] * F[7] - F[4] * F[6];
+        Fc[3] = F[7] * F[2] - F[8] * F[1];
+        Fc[4] = F[8] * F[0] - F[6] * F[2];
+        Fc[5] = F[6] * F[1] - F[7] * F[0];
+        Fc[6] = F[1] * F[5] - F[2] * F[4];
+        Fc[7] = F[2] * F[3] - F[0] * F[5];
+        Fc[8] = F[0] * F[4] - F[1] * F[3];
+        float gD[3], gH[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            gD[r] = inv_C * (F[r] * qj[0] + F[3 + r] * qj[1] + F[6 + r] * qj[2]);
+            gH[r] = fac * (Fc[r] * qj[0] + Fc[3 + r] * qj[1] + Fc[6 + r] * qj[2]);
+        }
+        const float s00 = quad_sum(w * (gD[0] * gD[0] + gD[1] * gD[1] + gD[2] * gD[2]));
+        const float s01 = quad_sum(w * (gD[0] * gH[0] + gD[1] * gH[1] + gD[2] * gH[2]));
+        const float s11 = quad_sum(w * (gH[0] * gH[0] + gH[1] * gH[1] + gH[2] * gH[2]));
+        const float atD = sq[9][e], atH = sq[10][e];
+        const float l0 = FIRST ? 0.f : sl[0][e], l1 = FIRST ? 0.f : sl[1][e];
+        const float A00 = atD + s00, A11 = atH + s11, A01 = s01;
+        const float b0 = -CD - atD * l0;
+        const float b1 = -CH - atH * l1;
+        const float inv_det = xrcp<float>(A00 * A11 - A01 * A01);
+        const float d0 = (b0 * A11 - b1 * A01) * inv_det;
+        const float d1 = (b1 * A00 - b0 * A01) * inv_det;
+        if (j == 0) ta.lam[tb + e] = make_float2(l0 + d0, l1 + d1);
+#pragma unroll
+        for (int k = 0; k < 3; k++) sd[(slot * 3 + k) * kQuadCol + cl] = dd[k] + w * (gD[k] * d0 + gH[k] * d1);
+        // the next element's reads of other lanes' slots follow these writes in
+        // program order (LDS operations of one wave complete in order)
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (v_lo >= 0)
+        pos[2 * (size_t)v_lo + 1] = make_float4(sd[(j * 3 + 0) * kQuadCol + cl], sd[(j * 3 + 1) * kQuadCol + cl],
+                                                sd[(j * 3 + 2) * kQuadCol + cl], 0.f);
+    if (v_hi >= 0) {
+        const int qs = j + 4;
+        pos[2 * (size_t)v_hi + 1] = make_float4(sd[(qs * 3 + 0) * kQuadCol + cl], sd[(qs * 3 + 1) * kQuadCol + cl],
+                                                sd[(qs * 3 + 2) * kQuadCol + cl], 0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Jacobi: every projection reads the same positions; corrections are summed
 // into acc (4 floats per vertex) with hardware float atomics, then applied.
 // ---------------------------------------------------------------------------
@@ -405,7 +583,6 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
 
 template <bool CON>
 __global__ __launch_bounds__(kBlock) void k_jacobi_shared(const int* __restrict__ shared, const int* __restrict__ soff,
-                                                         const int* __restrict__ sslot,
                                                          const float4* __restrict__ partial,
                                                          float4* __restrict__ cacc, float4* __restrict__ pos, int ns)
 {
@@ -413,8 +590,9 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_shared(const int* __restrict_
     if (j >= ns) return;
     const int g = shared[j];
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    // the vertex's slots are contiguous, in patch order (fixed summation order)
     for (int k = soff[j]; k < soff[j + 1]; k++) {
-        const float4 v = partial[sslot[k]];
+        const float4 v = partial[k];
         a0 += v.x;
         a1 += v.y;
         a2 += v.z;
@@ -923,6 +1101,32 @@ hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cve
     return hipGetLastError();
 }
 
+template <int BLOCK>
+static void launch_quad_shape(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
+                              const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first)
+{
+    const dim3 g(nblk(cend - cbeg, BLOCK / 4));
+    if (first)
+        hipLaunchKernelGGL((k_gs_cluster_quad<true, BLOCK>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc, pos,
+                           cbeg, cend);
+    else
+        hipLaunchKernelGGL((k_gs_cluster_quad<false, BLOCK>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc, pos,
+                           cbeg, cend);
+}
+
+bool quad_block_supported(int block) { return block == 64 || block == 128 || block == 256; }
+
+hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
+                                  const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block)
+{
+    if (cend <= cbeg) return hipSuccess;
+    if (block == 64) launch_quad_shape<64>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first);
+    else if (block == 128) launch_quad_shape<128>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first);
+    else if (block == 256) launch_quad_shape<256>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float4* pos, float* acc, int nt,
                                  bool first)
 {
@@ -964,10 +1168,10 @@ hipError_t launch_jacobi_patch(hipStream_t s, const TetArrays& ta, const PatchAr
     if (e != hipSuccess || pa.nshared <= 0) return e;
     if (con)
         hipLaunchKernelGGL(k_jacobi_shared<true>, dim3(nblk(pa.nshared, kBlock)), dim3(kBlock), 0, s, pa.shared,
-                           pa.soff, pa.sslot, pa.partial, cacc, pos, pa.nshared);
+                           pa.soff, pa.partial, cacc, pos, pa.nshared);
     else
         hipLaunchKernelGGL(k_jacobi_shared<false>, dim3(nblk(pa.nshared, kBlock)), dim3(kBlock), 0, s, pa.shared,
-                           pa.soff, pa.sslot, pa.partial, cacc, pos, pa.nshared);
+                           pa.soff, pa.partial, cacc, pos, pa.nshared);
     return hipGetLastError();
 }
 

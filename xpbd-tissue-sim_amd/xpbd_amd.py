@@ -92,6 +92,7 @@ _sig("xpbd_batch_num_instances", [_vp, _ip])
 _sig("xpbd_batch_set_sdf", [_vp, C.c_int32, C.c_int32, _dp, _dp])
 _sig("xpbd_batch_get_positions", [_vp, C.c_int32, _dp])
 _sig("xpbd_autotune", [_vp, C.c_char_p])
+_sig("xpbd_get_variant", [_vp, _ip, _ip])
 _sig("xpbd_mesh_grid", [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, _dp, _dp, _ip])
 _sig("xpbd_mesh_surface", [C.c_int32, C.c_int32, _ip, _ip, _ip])
 _sig("xpbd_color_items", [C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _ip])
@@ -222,7 +223,7 @@ class Mesh:
 SOLVERS = {"jacobi": XPBD_SOLVER_JACOBI, "gs": XPBD_SOLVER_GS_COLORED}
 
 
-VARIANTS = {"auto": 0, "simple": 1, "cluster": 2, "patch": 3}
+VARIANTS = {"auto": 0, "simple": 1, "cluster": 2, "patch": 3, "quad": 4}
 
 
 def make_params(dt=1e-3, g=9.81, iters=10, solver="gs", deterministic=1, collision_every=1,
@@ -385,6 +386,14 @@ class Solver:
 
     def autotune(self, cache_path: Optional[str] = None):
         self._c(lib.xpbd_autotune(self.h, None if cache_path is None else cache_path.encode()))
+        return self.variant()
+
+    def variant(self):
+        """(variant name, workgroup size or 0) of the kernels in use."""
+        v, b = C.c_int32(), C.c_int32()
+        self._c(lib.xpbd_get_variant(self.h, C.byref(v), C.byref(b)))
+        names = {val: k for k, val in VARIANTS.items()}
+        return names.get(v.value, str(v.value)), b.value
 
     # learned warm start
     def attach(self, net: Optional["Network"]):
