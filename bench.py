@@ -246,9 +246,11 @@ def main():
     achieved = bytes_iter / (us_per_iter * 1e-6) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{a.workload}_{cfg['solver']}.json")
+    traffic_kernel = None
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_iteration")
+            tj = json.load(f)
+        traffic, traffic_kernel = tj.get("bytes_per_iteration"), tj.get("kernel")
 
     res = s.residual()
     out = {
@@ -271,7 +273,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": KERNEL_NAMES.get(s.variant()[0], s.variant()[0]) + " (one solver iteration)",
                      "us_per_iteration": us_per_iter, "algorithmic_bytes_per_iteration": bytes_iter,
-                     "iterations_timed": nit, "graph_replay": not a.no_graph},
+                     "iterations_timed": nit, "graph_replay": not a.no_graph,
+                     "traffic_kernel": traffic_kernel},
     }
     if a.learned_steps > 0 and a.workload != "batch":
         s.close()

@@ -13,6 +13,14 @@
 // --dump-rest       rest positions of object 0 after placement, raw float64 (V x 3)
 // --print-config    parse the config, print a summary, touch no GPU
 // --load-meshes     with --print-config: also load and place each mesh (host only)
+// --solver-type S   override every object's solver-type (a config solver-type name)
+// --eager           launch every step without the HIP graph (kernel timing mode);
+//                   needed by profilers that cannot follow graph launches
+// --copy-calibration MB
+//                   first copy MB mebibytes device-to-device once: a transfer of
+//                   known size to scale hardware byte counters against
+#include <hip/hip_runtime.h>
+
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -36,17 +44,23 @@ int main(int argc, char** argv)
 {
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s CONFIG.yaml [--beam] [--steps N] [--dump-positions FILE] "
-                             "[--dump-order FILE] [--dump-face-rank FILE] [--dump-rest FILE] [--print-config]\n", argv[0]);
+                             "[--dump-order FILE] [--dump-face-rank FILE] [--dump-rest FILE] [--eager] [--copy-calibration MB] "
+                             "[--print-config]\n", argv[0]);
         return 2;
     }
     std::string cfg_path = argv[1], dump_pos, dump_order, dump_rank, dump_rest;
-    bool beam = false, print_only = false, load_meshes = false;
+    bool beam = false, print_only = false, load_meshes = false, eager = false;
+    double copy_mb = 0;
+    std::string solver_override;
     long steps = -1;
     for (int i = 2; i < argc; i++) {
         const std::string a = argv[i];
         if (a == "--beam") beam = true;
         else if (a == "--print-config") print_only = true;
         else if (a == "--load-meshes") load_meshes = true;
+        else if (a == "--eager") eager = true;
+        else if (a == "--solver-type" && i + 1 < argc) solver_override = argv[++i];
+        else if (a == "--copy-calibration" && i + 1 < argc) copy_mb = std::atof(argv[++i]);
         else if (a == "--steps" && i + 1 < argc) steps = std::atol(argv[++i]);
         else if (a == "--dump-positions" && i + 1 < argc) dump_pos = argv[++i];
         else if (a == "--dump-order" && i + 1 < argc) dump_order = argv[++i];
@@ -58,7 +72,9 @@ int main(int argc, char** argv)
         }
     }
     try {
-        const Config::SimulationConfig cfg = Config::SimulationConfig::from_file(cfg_path);
+        Config::SimulationConfig cfg = Config::SimulationConfig::from_file(cfg_path);
+        if (!solver_override.empty())
+            for (auto& o : cfg.xpbd_objects) o.solver_type = Config::solver_type_from_string(solver_override);
         if (print_only) {
             std::printf("{\"name\": \"%s\", \"time_step\": %.9g, \"end_time\": %.9g, \"g_accel\": %.9g, "
                         "\"collision_rate\": %.9g, \"objects\": [",
@@ -87,7 +103,23 @@ int main(int argc, char** argv)
         }
         std::unique_ptr<Sim::Simulation> sim =
             beam ? std::make_unique<Sim::BeamSimulation>(cfg) : std::make_unique<Sim::Simulation>(cfg);
+        if (copy_mb > 0) {
+            const size_t bytes = (size_t)(copy_mb * 1048576.0);
+            void *a = nullptr, *b = nullptr;
+            if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+                hipMemset(a, 0, bytes) != hipSuccess || hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
+                hipDeviceSynchronize() != hipSuccess)
+                throw Sim::SimError(XPBD_E_HIP, "calibration copy failed");
+            (void)hipFree(a);
+            (void)hipFree(b);
+            std::fprintf(stderr, "calibration copy: %zu bytes device to device\n", bytes);
+        }
         sim->setup();
+        if (eager)
+            for (auto& o : sim->objects()) {
+                const int rc = xpbd_set_kernel_timing(o->solver().handle(), 1);
+                if (rc != XPBD_OK) throw Sim::SimError(rc, "xpbd_set_kernel_timing");
+            }
         sim->run(steps);
         std::cout << sim->logText();
         std::fprintf(stderr, "%lld steps, t = %.6g s\n", (long long)sim->steps(), sim->time());
