@@ -289,55 +289,91 @@ __global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const i
     const int cl = tid >> 2, j = tid & 3;
     const int cb = cbeg + blockIdx.x * kQuadClusters;
     const int cl_end = min(cend, cb + kQuadClusters);
-    const int tb = cstart[cb];
-    const int n = cstart[cl_end] - tb;
-    for (int i = tid; i < n; i += kQuadBlock) {
-#pragma unroll
-        for (int k = 0; k < 9; k++) sq[k][i] = ta.Q[(size_t)k * ta.qstride + tb + i];
-        const float2 a = ta.at[tb + i];
-        sq[9][i] = a.x;
-        sq[10][i] = a.y;
-        sq[11][i] = ta.gamma[tb + i];
-        if (!FIRST) {
-            const float2 lv = ta.lam[tb + i];
-            sl[0][i] = lv.x;
-            sl[1][i] = lv.y;
-        }
-        stl[i] = tloc[tb + i];
-    }
     const int c = cb + cl;
     const bool active = c < cend;
-    int v_lo = -1, v_hi = -1;  // this lane stages and writes back slots j and j + 4
+    // Every global load of the prologue is issued before the first wait: the
+    // cluster table and ranges, the element data of the workgroup's contiguous
+    // tet range (kPer tets per thread, into registers), then the vertex records
+    // (which need the cluster table); the LDS stores follow.
+    int4 va = make_int4(-1, -1, -1, -1), vb = va;
+    int cs0 = 0, cs1 = 0;
     if (active) {
-        const int4 va = cverts[2 * (size_t)c], vb = cverts[2 * (size_t)c + 1];
-        v_lo = j == 0 ? va.x : j == 1 ? va.y : j == 2 ? va.z : va.w;
-        v_hi = j == 0 ? vb.x : j == 1 ? vb.y : j == 2 ? vb.z : vb.w;
-        if (v_lo >= 0) {
-            const float4 b = pos[2 * (size_t)v_lo], d = pos[2 * (size_t)v_lo + 1];
-            sb[(j * 4 + 0) * kQuadCol + cl] = b.x;
-            sb[(j * 4 + 1) * kQuadCol + cl] = b.y;
-            sb[(j * 4 + 2) * kQuadCol + cl] = b.z;
-            sb[(j * 4 + 3) * kQuadCol + cl] = b.w;
-            sd[(j * 3 + 0) * kQuadCol + cl] = d.x;
-            sd[(j * 3 + 1) * kQuadCol + cl] = d.y;
-            sd[(j * 3 + 2) * kQuadCol + cl] = d.z;
+        va = cverts[2 * (size_t)c];
+        vb = cverts[2 * (size_t)c + 1];
+        cs0 = cstart[c];
+        cs1 = cstart[c + 1];
+    }
+    const int tb = cstart[cb];
+    const int n = cstart[cl_end] - tb;
+    constexpr int kPer = kQuadStage / kQuadBlock;
+    float eq[kPer][12], el[kPer][2];
+    uint32_t etl[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int i = tid + r * kQuadBlock;
+        if (i < n) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) eq[r][k] = ta.Q[(size_t)k * ta.qstride + tb + i];
+            const float2 a = ta.at[tb + i];
+            eq[r][9] = a.x;
+            eq[r][10] = a.y;
+            eq[r][11] = ta.gamma[tb + i];
+            if (!FIRST) {
+                const float2 lv = ta.lam[tb + i];
+                el[r][0] = lv.x;
+                el[r][1] = lv.y;
+            }
+            etl[r] = tloc[tb + i];
         }
-        if (v_hi >= 0) {
-            const int q = j + 4;
-            const float4 b = pos[2 * (size_t)v_hi], d = pos[2 * (size_t)v_hi + 1];
-            sb[(q * 4 + 0) * kQuadCol + cl] = b.x;
-            sb[(q * 4 + 1) * kQuadCol + cl] = b.y;
-            sb[(q * 4 + 2) * kQuadCol + cl] = b.z;
-            sb[(q * 4 + 3) * kQuadCol + cl] = b.w;
-            sd[(q * 3 + 0) * kQuadCol + cl] = d.x;
-            sd[(q * 3 + 1) * kQuadCol + cl] = d.y;
-            sd[(q * 3 + 2) * kQuadCol + cl] = d.z;
+    }
+    // this lane stages and writes back slots j and j + 4
+    const int v_lo = j == 0 ? va.x : j == 1 ? va.y : j == 2 ? va.z : va.w;
+    const int v_hi = j == 0 ? vb.x : j == 1 ? vb.y : j == 2 ? vb.z : vb.w;
+    float4 blo = make_float4(0.f, 0.f, 0.f, 0.f), dlo = blo, bhi = blo, dhi = blo;
+    if (v_lo >= 0) {
+        blo = pos[2 * (size_t)v_lo];
+        dlo = pos[2 * (size_t)v_lo + 1];
+    }
+    if (v_hi >= 0) {
+        bhi = pos[2 * (size_t)v_hi];
+        dhi = pos[2 * (size_t)v_hi + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int i = tid + r * kQuadBlock;
+        if (i < n) {
+#pragma unroll
+            for (int k = 0; k < 12; k++) sq[k][i] = eq[r][k];
+            if (!FIRST) {
+                sl[0][i] = el[r][0];
+                sl[1][i] = el[r][1];
+            }
+            stl[i] = etl[r];
         }
+    }
+    if (v_lo >= 0) {
+        sb[(j * 4 + 0) * kQuadCol + cl] = blo.x;
+        sb[(j * 4 + 1) * kQuadCol + cl] = blo.y;
+        sb[(j * 4 + 2) * kQuadCol + cl] = blo.z;
+        sb[(j * 4 + 3) * kQuadCol + cl] = blo.w;
+        sd[(j * 3 + 0) * kQuadCol + cl] = dlo.x;
+        sd[(j * 3 + 1) * kQuadCol + cl] = dlo.y;
+        sd[(j * 3 + 2) * kQuadCol + cl] = dlo.z;
+    }
+    if (v_hi >= 0) {
+        const int q = j + 4;
+        sb[(q * 4 + 0) * kQuadCol + cl] = bhi.x;
+        sb[(q * 4 + 1) * kQuadCol + cl] = bhi.y;
+        sb[(q * 4 + 2) * kQuadCol + cl] = bhi.z;
+        sb[(q * 4 + 3) * kQuadCol + cl] = bhi.w;
+        sd[(q * 3 + 0) * kQuadCol + cl] = dhi.x;
+        sd[(q * 3 + 1) * kQuadCol + cl] = dhi.y;
+        sd[(q * 3 + 2) * kQuadCol + cl] = dhi.z;
     }
     __syncthreads();
     // a quad is active or inactive as a whole, so DPP reads stay inside active quads
     if (!active) return;
-    const int t0 = cstart[c] - tb, t1 = cstart[c + 1] - tb;
+    const int t0 = cs0 - tb, t1 = cs1 - tb;
     for (int e = t0; e < t1; e++) {
         const int slot = (int)((stl[e] >> (4 * j)) & 15u);
         float bb[3], dd[3];
