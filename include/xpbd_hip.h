@@ -93,6 +93,15 @@ typedef struct xpbd_params {
     int32_t deterministic;    /* Jacobi only: 1 = fixed-order accumulation */
     int32_t collision_every;  /* re-detect contacts every k steps; 0 = never */
     int32_t variant;          /* XPBD_VARIANT_* */
+    /* First-order XPBD (FirstOrderXPBDMeshObject; XPBDMeshObject.cpp:277-348,
+     * 450-457; CombinedNeohookeanConstraintProjector.cpp:16-96): the per-vertex
+     * damping B_i = V_i b [E_i / (1 + nu_i)] replaces the mass as constraint
+     * inertia, alpha~ = alpha / dt, and the inertial move is z -= g m_i dt / B_i.
+     * Zero-initialised params give the second-order solver. */
+    int32_t first_order;          /* 1 = first-order XPBD */
+    double damping_multiplier;    /* b (config damping-multiplier) */
+    int32_t adjust_damping;       /* 1 = scale B by E_i / (1 + nu_i) (adjust-damping-to-material) */
+    int32_t reserved;             /* must be 0 */
 } xpbd_params;
 
 /* ---- object life cycle ------------------------------------------------- */

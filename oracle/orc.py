@@ -67,6 +67,8 @@ _sig("orc_project_contact", [_dp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int)
 _sig("orc_step_corrected", [C.c_void_p, C.c_int, _dp, C.c_int])
 _sig("orc_step_dual_warm", [C.c_void_p, C.c_int, _dp, C.c_int])
 _sig("orc_step_learned", [C.c_void_p, C.c_int, _dp, C.c_int])
+_sig("orc_set_first_order", [C.c_void_p, C.c_double, C.c_int])
+_sig("orc_get_inverse_inertia", [C.c_void_p, _dp])
 
 
 def project_contact(x9, w3, cp, cn, bary, lam=0.0):
@@ -179,6 +181,13 @@ class Sim:
 
     def set_mode(self, mode, iters):
         lib.orc_set_mode(self.s, GS if mode == "gs" else JACOBI, iters)
+
+    def set_first_order(self, damping_multiplier=1.0, adjust_to_material=False):
+        """Switch to first-order XPBD (damping B as constraint inertia, alpha / dt)."""
+        lib.orc_set_first_order(self.s, float(damping_multiplier), int(adjust_to_material))
+
+    def inverse_inertia(self):
+        return self._get(lib.orc_get_inverse_inertia, (self.nv,))
 
     def set_sdf(self, kind, pos, quat=(0, 0, 0, 1), params=(0, 0, 0)):
         lib.orc_set_sdf(self.s, kind, p(_d(pos)), p(_d(quat)), p(_d(params)))

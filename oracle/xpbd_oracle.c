@@ -188,9 +188,12 @@ ORC_API void orc_deformation_gradient(const double* x12, const double* Q, double
 static double g_relax = 1.0;
 ORC_API void orc_set_relaxation(double omega) { g_relax = omega; }
 
+/* ts is the time scale of the compliance: alpha~ = alpha / ts with ts = dt^2
+ * (second order, CombinedNeohookeanConstraintProjector.cpp:117) or ts = dt
+ * (first order, CombinedNeohookeanConstraintProjector.cpp:35). */
 static void project_combined(const double* const p[4], const double* w,
                              const double* Q, double aD, double aH, double gamma,
-                             double dt, double* lam, double* dx)
+                             double ts, double* lam, double* dx)
 {
     double F[9], C[2], g[24];
     compute_F(p[0], p[1], p[2], p[3], Q, F);
@@ -199,7 +202,7 @@ static void project_combined(const double* const p[4], const double* w,
     C[1] = hyd_eval(F, gamma);
     hyd_grad(F, Q, g + 12);
 
-    const double at[2] = { aD / (dt * dt), aH / (dt * dt) };
+    const double at[2] = { aD / ts, aH / ts };
     /* A(ci, cj) accumulated from alpha_tilde on the diagonal, vertex by vertex */
     double A00 = at[0], A01 = 0, A11 = at[1];
     for (int i = 0; i < 4; i++)
@@ -226,7 +229,7 @@ ORC_API void orc_project_tet(const double* x12, const double* w4, const double* 
                              double* lam2, double* dx12)
 {
     const double* p[4] = { x12, x12 + 3, x12 + 6, x12 + 9 };
-    project_combined(p, w4, Q, aD, aH, gamma, dt, lam2, dx12);
+    project_combined(p, w4, Q, aD, aH, gamma, dt * dt, lam2, dx12);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -442,6 +445,10 @@ typedef struct orc_sim {
     double last_res;       /* ||r||_2 at the end of the last solve (before pin reset) */
     double* jbuf;          /* Jacobi coordinate-update buffer (values) */
     int* jidx;             /* Jacobi coordinate-update buffer (coordinate index) */
+    double ts;             /* compliance time scale: dt^2 (second order) or dt (first order) */
+    int first_order;       /* 1: first-order XPBD (damping replaces mass) */
+    double* dz1;           /* first order: per-vertex inertial move -g m dt / B */
+    double *E, *nu;        /* element materials (kept for the first-order damping) */
 } orc_sim;
 
 /* XPBDMeshObject.cpp:102-160 (setup) + 277-331 (masses) + 352-417 (constraints) */
@@ -494,8 +501,67 @@ ORC_API orc_sim* orc_create(int nv, int nt, const double* x0, const double* v0,
         s->aH[e] = 1 / (lam * s->vol[e]);
         s->gam[e] = mu / lam;
     }
+    s->ts = dt * dt;
+    s->E = (double*)malloc(sizeof(double) * nt);
+    s->nu = (double*)malloc(sizeof(double) * nt);
+    memcpy(s->E, E, sizeof(double) * nt);
+    memcpy(s->nu, nu, sizeof(double) * nt);
     (void)bad;
     return s;
+}
+
+/* First-order XPBD (FirstOrderXPBDMeshObject):
+ *   per-vertex damping B_i = V_i b E_i / (1 + nu_i) with adjust, else V_i b,
+ *   V_i = sum of element volume / 4, E_i and nu_i = means over the elements
+ *   attached to the vertex (XPBDMeshObject.cpp:277-348);
+ *   B_i is the constraint inertia (XPBDMeshObjectBase.hpp:125-136), so w_i = 1/B_i;
+ *   alpha~ = alpha / dt (CombinedNeohookeanConstraintProjector.cpp:16-96);
+ *   inertial move z -= g m_i dt / B_i, no velocity term (XPBDMeshObject.cpp:450-457). */
+ORC_API void orc_set_first_order(orc_sim* s, double damping_multiplier, int adjust_to_material)
+{
+    const int nv = s->nv, nt = s->nt;
+    double* V = (double*)calloc(nv, sizeof(double));
+    double* Ev = (double*)calloc(nv, sizeof(double));
+    double* Nv = (double*)calloc(nv, sizeof(double));
+    int* cnt = (int*)calloc(nv, sizeof(int));
+    for (int e = 0; e < nt; e++)
+        for (int j = 0; j < 4; j++) cnt[s->tets[4 * e + j]]++;
+    for (int e = 0; e < nt; e++)
+        for (int j = 0; j < 4; j++) {
+            const int v = s->tets[4 * e + j];
+            V[v] += s->vol[e] / 4.0;
+            Ev[v] += s->E[e] / cnt[v];
+            Nv[v] += s->nu[e] / cnt[v];
+        }
+    free(s->dz1);
+    s->dz1 = (double*)malloc(sizeof(double) * nv);
+    for (int i = 0; i < nv; i++) {
+        const double B = adjust_to_material ? V[i] * damping_multiplier * Ev[i] / (1 + Nv[i])
+                                            : V[i] * damping_multiplier;
+        s->w[i] = 1.0 / B;
+        s->dz1[i] = -s->g * s->m[i] * s->dt / B;
+    }
+    s->ts = s->dt;
+    s->first_order = 1;
+    free(V); free(Ev); free(Nv); free(cnt);
+}
+
+ORC_API void orc_get_inverse_inertia(const orc_sim* s, double* out) { memcpy(out, s->w, sizeof(double) * s->nv); }
+
+/* inertial move of one step, both orders (XPBDMeshObject.cpp:430, 446-470) */
+static void move_inertially(orc_sim* s)
+{
+    const int nv = s->nv;
+    const double dt = s->dt;
+    memcpy(s->xp, s->x, sizeof(double) * 3 * nv);
+    if (s->first_order) {
+        for (int i = 0; i < nv; i++) s->x[3 * i + 2] += s->dz1[i];
+        return;
+    }
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
+    const double dz = -s->g * dt * dt;
+    for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
 }
 
 ORC_API void orc_destroy(orc_sim* s)
@@ -505,7 +571,7 @@ ORC_API void orc_destroy(orc_sim* s)
     free(s->tets); free(s->Q); free(s->vol); free(s->aD); free(s->aH); free(s->gam);
     free(s->lam); free(s->order); free(s->faces); free(s->jbuf); free(s->jidx);
     free(s->c_face); free(s->c_p); free(s->c_n); free(s->c_bary); free(s->c_lam);
-    free(s->face_rank); free(s->corder);
+    free(s->face_rank); free(s->corder); free(s->dz1); free(s->E); free(s->nu);
     free(s);
 }
 
@@ -667,7 +733,7 @@ static void iterate(orc_sim* s)
             const double* p[4] = { s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3] };
             const double w[4] = { s->w[t[0]], s->w[t[1]], s->w[t[2]], s->w[t[3]] };
             double dx[12];
-            project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->dt, s->lam + 2 * e, dx);
+            project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->ts, s->lam + 2 * e, dx);
             for (int i = 0; i < 4; i++)
                 for (int k = 0; k < 3; k++) s->x[3 * t[i] + k] += dx[3 * i + k];
         }
@@ -691,7 +757,7 @@ static void iterate(orc_sim* s)
         const int* t = s->tets + 4 * e;
         const double* p[4] = { s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3] };
         const double w[4] = { s->w[t[0]], s->w[t[1]], s->w[t[2]], s->w[t[3]] };
-        project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->dt, s->lam + 2 * e, s->jbuf + nu);
+        project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->ts, s->lam + 2 * e, s->jbuf + nu);
         for (int i = 0; i < 4; i++)
             for (int k = 0; k < 3; k++) s->jidx[nu + 3 * i + k] = 3 * t[i] + k;
         nu += 12;
@@ -731,11 +797,7 @@ ORC_API void orc_step(orc_sim* s, int detect)
     if (detect) orc_detect(s);
     const int nv = s->nv;
     const double dt = s->dt;
-    memcpy(s->xp, s->x, sizeof(double) * 3 * nv);
-    for (int i = 0; i < nv; i++)
-        for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
-    const double dz = -s->g * dt * dt;
-    for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
+    move_inertially(s);
     memset(s->lam, 0, sizeof(double) * 2 * (size_t)s->nt);
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
     for (int it = 0; it < s->iters; it++) iterate(s);
@@ -863,14 +925,7 @@ ORC_API void orc_step_learned(orc_sim* s, int detect, const double* corr, int it
 }
 
 /* Prediction half of a step only (used to build learned-warm-start features). */
-ORC_API void orc_predict(orc_sim* s)
-{
-    const double dt = s->dt;
-    memcpy(s->xp, s->x, sizeof(double) * 3 * s->nv);
-    for (int i = 0; i < 3 * s->nv; i++) s->x[i] += dt * s->v[i];
-    const double dz = -s->g * dt * dt;
-    for (int i = 0; i < s->nv; i++) s->x[3 * i + 2] += dz;
-}
+ORC_API void orc_predict(orc_sim* s) { move_inertially(s); }
 
 ORC_API void orc_get_x(const orc_sim* s, double* out) { memcpy(out, s->x, sizeof(double) * 3 * s->nv); }
 ORC_API void orc_get_xprev(const orc_sim* s, double* out) { memcpy(out, s->xp, sizeof(double) * 3 * s->nv); }
@@ -895,7 +950,7 @@ ORC_API void orc_get_compliance(const orc_sim* s, double* aD, double* aH, double
  *     Returns ||r||_2; vec (2*nt, optional) receives r. */
 ORC_API double orc_residual(const orc_sim* s, double* vec)
 {
-    const double dt2 = s->dt * s->dt;
+    const double dt2 = s->ts;  /* dt^2, or dt for first order */
     double acc = 0;
     for (int e = 0; e < s->nt; e++) {
         const int* t = s->tets + 4 * e;

@@ -75,11 +75,14 @@ def test_reference_style_config(tmp_path):
     assert c["collision_rate"] == 250 and c["rigid_objects"] == 1
     o = c["objects"][0]
     assert o == {"name": "Cube1", "solver_type": "Parallel-Jacobi", "iters": 7, "E": 6e4, "nu": 0.45,
-                 "density": 1100, "collisions": False, "source": "mesh.msh", "position": [0, 0, 2]}
+                 "density": 1100, "collisions": False, "source": "mesh.msh", "position": [0, 0, 2],
+                 "first_order": False, "damping_multiplier": 1, "adjust_damping": False}
 
 
 @pytest.mark.parametrize("bad,msg", [
-    ('objects:\n  - name: a\n    type: "FirstOrderXPBDMeshObject"\n    grid: [1, 1, 1]\n', "not on the device path"),
+    ('objects:\n  - name: a\n    type: "RigidMeshObject"\n    grid: [1, 1, 1]\n', "not on the device path"),
+    ('objects:\n  - name: a\n    type: "FirstOrderXPBDMeshObject"\n    grid: [1, 1, 1]\n    damping-multiplier: 0\n',
+     "damping-multiplier must be > 0"),
     ('objects:\n  - name: a\n    grid: [1, 1, 1]\n    solver-type: "Newton"\n', "unknown solver-type"),
     ('objects:\n  - name: a\n    grid: [1, 1, 1]\n    residual-policy: "Sometimes"\n', "unknown residual-policy"),
     ('objects:\n  - name: a\n    grid: [1, 1]\n', "grid needs 3"),
@@ -131,3 +134,19 @@ objects:
     centre = [(l + h) / 2 for l, h in zip(lo, hi)]
     assert centre == pytest.approx([1, 2, 3], abs=1e-9)        # symmetric cube: centroid = box centre
     assert o["surface_faces"] > 0 and o["elements"] > 0
+
+
+def test_first_order_object_config(tmp_path):
+    cfg = """
+objects:
+  - name: "fo"
+    type: "FirstOrderXPBDMeshObject"
+    grid: [2, 2, 2]
+    damping-multiplier: 20000
+    adjust-damping-to-material: true
+    num-solver-iters: 3
+"""
+    r = run(cfg, tmp_path, "--print-config")
+    assert r.returncode == 0, r.stderr
+    o = json.loads(r.stdout)["objects"][0]
+    assert o["first_order"] is True and o["damping_multiplier"] == 20000 and o["adjust_damping"] is True

@@ -388,3 +388,100 @@ def test_detection_sphere_on_block():
     assert np.all(top > 0.0599)                                                   # only top faces
     s.set_sdf(orc.SDF_SPHERE, [0.03, 0.03, 0.2], [0, 0, 0, 1], [r, 0, 0])
     assert s.detect() == 0
+
+
+# ---------------------------------------------------------------- first-order XPBD (next row)
+def _first_order_mesh(n=3, seed=5, hetero=True):
+    import workloads as W
+    rng = np.random.default_rng(seed)
+    E, nu = W.heterogeneous(6 * n ** 3, rng) if hetero else (3e3, 0.45)
+    return W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.0, rng=rng)
+
+
+@pytest.mark.parametrize("adjust", [False, True])
+def test_first_order_damping_closed_form(adjust):
+    """B_i = V_i b [E_i / (1 + nu_i)], V_i = sum vol/4, E_i, nu_i = means over attached elements."""
+    m = _first_order_mesh()
+    s = orc.Sim(m, iters=1)
+    b = 2.5e4
+    s.set_first_order(b, adjust)
+    nv = len(m.x0)
+    V, Es, Ns, cnt = np.zeros(nv), np.zeros(nv), np.zeros(nv), np.zeros(nv)
+    for e, t in enumerate(m.tets):
+        X = m.x0[t]
+        vol = abs(np.linalg.det(np.stack([X[0] - X[3], X[1] - X[3], X[2] - X[3]], 1))) / 6
+        for v in t:
+            V[v] += vol / 4
+            Es[v] += m.E[e]
+            Ns[v] += m.nu[e]
+            cnt[v] += 1
+    B = V * b * (Es / cnt) / (1 + Ns / cnt) if adjust else V * b
+    np.testing.assert_allclose(s.inverse_inertia(), 1 / B, rtol=1e-12)
+
+
+def test_first_order_inertial_move_closed_form():
+    """Without projections: z_n = z_0 - n g m_i dt / B_i for free vertices, pins fixed."""
+    m = _first_order_mesh(hetero=False)
+    s = orc.Sim(m, g=9.81, iters=0)
+    s.set_first_order(1e4, False)
+    n, dt = 11, 1e-3
+    s.step(n)
+    w = s.inverse_inertia()
+    free = m.pinned == 0
+    dz = s.x()[:, 2] - m.x0[:, 2]
+    np.testing.assert_allclose(dz[free], -n * 9.81 * s.masses()[free] * dt * w[free], rtol=1e-12)
+    assert np.all(dz[~free] == 0)
+
+
+def test_first_order_rest_stationary():
+    m = _first_order_mesh(hetero=True)
+    for mode in ("gs", "jacobi"):
+        s = orc.Sim(m, g=0.0, iters=4, mode=mode)
+        s.set_first_order(1e4, True)
+        s.step(50)
+        assert np.abs(s.x() - m.x0).max() < 1e-15
+
+
+def test_first_order_single_element_step_matches_autograd_restatement():
+    """One first-order step of one element: the inertial move, then one 2x2 block
+    projection with alpha~ = alpha / dt and inverse inertia 1/B, against torch_ref."""
+    import torch_ref as T
+    import xpbd_amd as X
+    Xr = random_tet(scale=0.01)
+    t = np.array([[0, 1, 2, 3]], np.int32)
+    E, nu, b, dt, g = 5e3, 0.47, 3e4, 1e-3, 9.81
+    m = X.Mesh(x0=Xr, tets=t, E=np.array([E]), nu=np.array([nu]), rho=np.array([1000.0]))
+    s = orc.Sim(m, dt=dt, g=g, iters=1)
+    s.set_first_order(b, True)
+    s.step(1)
+    w = s.inverse_inertia()
+    Q, vol = T.rest(Xr)
+    mu, lam = orc.lame(E, nu)
+    x = Xr.copy()
+    x[:, 2] += -g * s.masses() * dt * w
+    # torch_ref divides alpha by its dt argument squared: pass sqrt(dt) to get alpha / dt
+    dx, _ = T.project(x.reshape(12), w, Q, 1 / (mu * vol), 1 / (lam * vol), mu / lam, math.sqrt(dt), [0.0, 0.0])
+    np.testing.assert_allclose(s.x(), x + dx.reshape(4, 3), rtol=0, atol=1e-15)
+
+
+def test_first_order_low_damping_is_rounding_chaotic():
+    """With low damping the constraint stiffness w |grad C|^2 dominates alpha / dt;
+    at rest grad C_H = sqrt(3) grad C_D, so the 2x2 block systems are nearly
+    singular and the Gauss-Seidel sweep amplifies rounding: the FP64 oracle, from
+    a state perturbed by 1e-9 (relative), ends 30 steps later far from the
+    unperturbed run for the reference's own example damping (E = 6e4, nu = 0.45,
+    damping-multiplier 2e4), while a material-scaled damping stays at 1e-9."""
+    import workloads as W
+    def run(b, adjust, pert):
+        m = W.grid_object(6, 6, 6, 0.01, E=6e4, nu=0.45, pin_axis=2, noise=0.0)
+        s = orc.Sim(m, dt=1e-3, iters=10, mode="gs")
+        s.set_first_order(b, adjust)
+        if pert:
+            s.set_state(m.x0 * (1 + pert * np.random.default_rng(2).standard_normal(m.x0.shape)), None)
+        s.step(30)
+        return s.x()
+    chaotic = rel_err(run(2e4, False, 1e-9), run(2e4, False, 0.0))
+    calm = rel_err(run(2e4, True, 1e-9), run(2e4, True, 0.0))
+    print(f"sensitivity to a 1e-9 perturbation: low damping {chaotic:.2e}, material-scaled {calm:.2e}")
+    assert chaotic > 1e-3
+    assert calm < 1e-7

@@ -162,7 +162,9 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     *out = nullptr;
     if (n <= 0 || !meshes || !P) return set_err(nullptr, XPBD_E_INVALID_ARG, "need n > 0 meshes and params");
     if (!(P->dt > 0) || !std::isfinite(P->dt) || !std::isfinite(P->g) || P->iters < 0 ||
-        (P->solver != XPBD_SOLVER_JACOBI && P->solver != XPBD_SOLVER_GS_COLORED) || P->collision_every < 0)
+        (P->solver != XPBD_SOLVER_JACOBI && P->solver != XPBD_SOLVER_GS_COLORED) || P->collision_every < 0 ||
+        (P->first_order != 0 && P->first_order != 1) ||
+        (P->first_order && !(P->damping_multiplier > 0 && std::isfinite(P->damping_multiplier))))
         return set_err(nullptr, XPBD_E_INVALID_ARG, "params: need dt > 0, finite g, iters >= 0, known solver");
     for (int32_t i = 0; i < n; i++) {
         int rc = validate_desc(&meshes[i]);
@@ -251,9 +253,34 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
             }
     }
     h->tets_host = tets;
+    // constraint inertia: the lumped mass, or for first-order objects the vertex
+    // damping B_i = V_i b [E_i / (1 + nu_i)] with V_i = sum vol / 4 and E_i, nu_i
+    // the means over attached elements (XPBDMeshObject.cpp:277-348,
+    // XPBDMeshObjectBase.hpp:125-136); first order also moves z by -g m dt / B
+    h->inertia = h->mass;
+    if (P->first_order) {
+        std::vector<double> V(nv, 0.0), sE(nv, 0.0), sN(nv, 0.0);
+        std::vector<int32_t> cnt(nv, 0);
+        for (int e = 0; e < nt; e++)
+            for (int j = 0; j < 4; j++) cnt[tets[4 * (size_t)e + j]]++;
+        for (int e = 0; e < nt; e++)
+            for (int j = 0; j < 4; j++) {
+                const int32_t v = tets[4 * (size_t)e + j];
+                V[v] += vol[e] / 4.0;
+                sE[v] += E[e] / cnt[v];
+                sN[v] += nu[e] / cnt[v];
+            }
+        h->dz1.assign(nv, 0.f);
+        for (int v = 0; v < nv; v++) {
+            const double B = P->adjust_damping ? V[v] * P->damping_multiplier * sE[v] / (1 + sN[v])
+                                               : V[v] * P->damping_multiplier;
+            h->inertia[v] = B;
+            h->dz1[v] = (float)(-P->g * h->mass[v] * P->dt / B);
+        }
+    }
     // compliances (DeviatoricConstraint.cpp:13, HydrostaticConstraint.cpp:15-16,
-    // ElasticMaterial.hpp:31-32) and alpha tilde = alpha / dt^2
-    const double dt2 = P->dt * P->dt;
+    // ElasticMaterial.hpp:31-32) and alpha tilde = alpha / dt^2 (first order: alpha / dt)
+    const double dt2 = P->first_order ? P->dt : P->dt * P->dt;
     std::vector<float> atD(nt), atH(nt), gam(nt);
     for (int e = 0; e < nt; e++) {
         const double mu = E[e] / (2 * (1 + nu[e]));
@@ -373,6 +400,7 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     rc |= dalloc(h, &h->d_pos, 2 * (size_t)nv);
     rc |= dalloc(h, &h->d_lo, nv);
     rc |= dalloc(h, &h->d_vel, nv);
+    if (P->first_order) rc |= dalloc(h, &h->d_dz1, nv);
     rc |= dalloc(h, &h->d_step_res, 1);
     rc |= dalloc(h, &h->d_acc, 4 * (size_t)nv);
     rc |= dalloc(h, &h->d_rest, 3 * (size_t)nv);
@@ -421,7 +449,7 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     // host staging in solver order
     std::vector<float4> hu(2 * (size_t)nv), hv(nv);
     for (int i = 0; i < nv; i++) {
-        const double w = h->mass[i] > 0 ? 1.0 / h->mass[i] : 0.0;  // PositionReference.hpp:30-33
+        const double w = h->inertia[i] > 0 ? 1.0 / h->inertia[i] : 0.0;  // PositionReference.hpp:30-33
         hu[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, (float)w);
         hu[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
         hv[i] = make_float4((float)vel[3 * (size_t)i], (float)vel[3 * (size_t)i + 1], (float)vel[3 * (size_t)i + 2],
@@ -456,6 +484,7 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     } while (0)
     UP(h->d_pos, hu.data(), sizeof(float4) * 2 * (size_t)nv);
     UP(h->d_vel, hv.data(), sizeof(float4) * nv);
+    if (P->first_order) UP(h->d_dz1, h->dz1.data(), sizeof(float) * nv);
     UP(h->d_rest, h->rest.data(), sizeof(double) * 3 * nv);
     UP(h->d_tet, ht.data(), sizeof(int4) * nt);
     UP(h->d_Q, hQ.data(), sizeof(float) * 9 * (size_t)nt);
@@ -610,7 +639,9 @@ int enqueue_detect(xpbd_handle* h)
 int enqueue_predict(xpbd_handle* h)
 {
     const double dt = h->params.dt;
-    hipError_t e = launch_predict(h->stream, h->d_pos, h->d_vel, h->nv, (float)dt, (float)(-h->params.g * dt * dt));
+    hipError_t e = h->params.first_order
+                       ? launch_predict_first_order(h->stream, h->d_pos, h->d_dz1, h->nv)
+                       : launch_predict(h->stream, h->d_pos, h->d_vel, h->nv, (float)dt, (float)(-h->params.g * dt * dt));
     return e == hipSuccess ? XPBD_OK : hip_err(h, e, "predict launch");
 }
 
@@ -941,7 +972,8 @@ extern "C" int xpbd_constraint_residual(xpbd_handle* h, double* norm_out, double
 extern "C" int xpbd_strain_energy(xpbd_handle* h, double* out)
 {
     if (!h || !out) return XPBD_E_INVALID_ARG;
-    const double dt2 = h->params.dt * h->params.dt;
+    // alpha = alpha~ * dt^2 (first order: alpha~ * dt)
+    const double dt2 = h->params.first_order ? h->params.dt : h->params.dt * h->params.dt;
     hipError_t e = launch_energy(h->stream, tet_arrays(h), h->d_pos, h->d_lo, h->nt, dt2, h->d_part, h->red_blocks);
     if (e == hipSuccess) e = launch_sum_partials(h->stream, h->d_part, h->red_blocks, h->d_scalar);
     if (e == hipSuccess) e = hipMemcpyAsync(out, h->d_scalar, sizeof(double), hipMemcpyDeviceToHost, h->stream);

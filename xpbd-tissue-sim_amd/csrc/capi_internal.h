@@ -24,6 +24,8 @@ struct xpbd_handle {
     std::vector<int64_t> v_off, t_off, f_off;  // per-instance offsets in the concatenated arrays
     std::vector<double> rest;                  // 3*nv rest positions (FP64, host copy)
     std::vector<double> mass;                  // nv lumped masses
+    std::vector<double> inertia;               // constraint inertia: mass, or damping B (first order)
+    std::vector<float> dz1;                    // first order: inertial move -g m dt / B per vertex
     std::vector<uint8_t> pinned;
     std::vector<int32_t> tets_host;            // 4*nt vertex ids in the caller's element order
     std::vector<float> vmat;                   // 2*nv: mean log10(E) and mean nu of incident elements
@@ -47,6 +49,7 @@ struct xpbd_handle {
     float4* d_pos = nullptr;     // 2 per vertex: (base displacement, inverse mass), (in-step motion, 0)
     float4* d_lo = nullptr;      // low-order part of the base displacement
     float4* d_vel = nullptr;     // velocity xyz, pinned flag w
+    float* d_dz1 = nullptr;      // first order: inertial move per vertex
     float* d_acc = nullptr;      // Jacobi accumulator (4 per vertex)
     double* d_rest = nullptr;
     int4* d_tet = nullptr;

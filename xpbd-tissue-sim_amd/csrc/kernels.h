@@ -31,6 +31,8 @@ struct TetArrays {
 // pos[2i+1] = (in-step motion xyz, 0); lo[i] = low-order part of the base;
 // vel[i] = (velocity xyz, pinned flag).
 hipError_t launch_predict(hipStream_t s, float4* pos, const float4* vel, int nv, float dt, float dz);
+// first-order inertial move: delta = (0, 0, dz1[i])
+hipError_t launch_predict_first_order(hipStream_t s, float4* pos, const float* dz1, int nv);
 hipError_t launch_finish(hipStream_t s, float4* pos, float4* lo, float4* vel, int nv, float dt,
                          unsigned long long* step_counter);
 hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int begin, int end, bool first);

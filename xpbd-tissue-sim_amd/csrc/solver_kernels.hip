@@ -64,6 +64,16 @@ __global__ __launch_bounds__(kBlock) void k_predict(float4* __restrict__ pos, co
     pos[2 * (size_t)i + 1] = make_float4(dt * v.x, dt * v.y, dt * v.z + dz, 0.f);
 }
 
+// first-order inertial move (XPBDMeshObject.cpp:450-457): z -= g m_i dt / B_i,
+// no velocity term
+__global__ __launch_bounds__(kBlock) void k_predict_first_order(float4* __restrict__ pos,
+                                                               const float* __restrict__ dz1, int nv)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nv) return;
+    pos[2 * (size_t)i + 1] = make_float4(0.f, 0.f, dz1[i], 0.f);
+}
+
 // ---------------------------------------------------------------------------
 // finish: pinned vertices back to x_prev (XPBDMeshObject.cpp:497-503), then
 // v = (x - x_prev) / dt (XPBDMeshObject.cpp:507-535) = delta / dt, and the
@@ -1105,6 +1115,12 @@ hipError_t launch_predict(hipStream_t s, float4* pos, const float4* vel, int nv,
 {
     if (nv <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_predict, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos, vel, nv, dt, dz);
+    return hipGetLastError();
+}
+
+hipError_t launch_predict_first_order(hipStream_t s, float4* pos, const float* dz1, int nv)
+{
+    hipLaunchKernelGGL(k_predict_first_order, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos, dz1, nv);
     return hipGetLastError();
 }
 

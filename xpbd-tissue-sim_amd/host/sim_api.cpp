@@ -48,9 +48,17 @@ XPBDMeshObjectConfig XPBDMeshObjectConfig::from_node(const yaml_lite::Node& n)
     auto B = [](const yaml_lite::Node& x) { return x.as_bool(); };
     opt(n, "name", c.name, S);
     opt(n, "type", c.type, S);
-    if (c.type != "XPBDMeshObject")
+    if (c.type == "FirstOrderXPBDMeshObject") {
+        c.first_order = true;
+        opt(n, "damping-multiplier", c.damping_multiplier, D);
+        opt(n, "adjust-damping-to-material", c.adjust_damping_to_material, B);
+        if (!(c.damping_multiplier > 0))
+            throw Sim::SimError(XPBD_E_INVALID_ARG, "object '" + c.name + "': damping-multiplier must be > 0");
+    } else if (c.type != "XPBDMeshObject") {
         throw Sim::SimError(XPBD_E_UNSUPPORTED, "object '" + c.name + "': type " + c.type +
-                                                    " is not on the device path (XPBDMeshObject only)");
+                                                    " is not on the device path (XPBDMeshObject, "
+                                                    "FirstOrderXPBDMeshObject)");
+    }
     opt(n, "filename", c.filename, S);
     if (const yaml_lite::Node* g = n.find("grid")) {
         const std::vector<double> v = g->as_doubles();
@@ -180,6 +188,9 @@ void XPBDGPUSolver::setup()
     p.deterministic = 1;
     p.collision_every = 0;  // detection is driven by Simulation::timeStep
     p.variant = XPBD_VARIANT_AUTO;
+    p.first_order = _obj->config().first_order ? 1 : 0;
+    p.damping_multiplier = _obj->config().damping_multiplier;
+    p.adjust_damping = _obj->config().adjust_damping_to_material ? 1 : 0;
     const int rc = xpbd_create(&d, &p, &_h);
     if (rc != XPBD_OK) throw Sim::SimError(rc, std::string("xpbd_create: ") + xpbd_last_error(nullptr));
     if (_policy != XPBDSolverResidualPolicyEnum::NEVER) check(xpbd_set_residual_policy(_h, 1));

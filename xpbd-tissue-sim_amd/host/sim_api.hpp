@@ -72,6 +72,10 @@ struct XPBDMeshObjectConfig {
     XPBDSolverResidualPolicyEnum residual_policy = XPBDSolverResidualPolicyEnum::NEVER;
     // vertices with the minimum coordinate along this axis are fixed (-1: none)
     int fix_min_axis = -1;
+    // type "FirstOrderXPBDMeshObject" (FirstOrderXPBDMeshObjectConfig.hpp:18-59)
+    bool first_order = false;
+    double damping_multiplier = 1.0;        // damping-multiplier
+    bool adjust_damping_to_material = false;  // adjust-damping-to-material
 
     static XPBDMeshObjectConfig from_node(const yaml_lite::Node& n);
 };

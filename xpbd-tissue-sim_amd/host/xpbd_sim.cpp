@@ -82,10 +82,13 @@ int main(int argc, char** argv)
             for (size_t i = 0; i < cfg.xpbd_objects.size(); i++) {
                 const auto& o = cfg.xpbd_objects[i];
                 std::printf("%s{\"name\": \"%s\", \"solver_type\": \"%s\", \"iters\": %d, \"E\": %.9g, \"nu\": %.9g, "
-                            "\"density\": %.9g, \"collisions\": %s, \"source\": \"%s\", \"position\": [%.9g, %.9g, %.9g]",
+                            "\"density\": %.9g, \"collisions\": %s, \"source\": \"%s\", \"position\": [%.9g, %.9g, %.9g], "
+                            "\"first_order\": %s, \"damping_multiplier\": %.9g, \"adjust_damping\": %s",
                             i ? ", " : "", o.name.c_str(), solver_name(o.solver_type), o.num_solver_iters,
                             o.material.E, o.material.nu, o.material.density, o.collisions ? "true" : "false",
-                            o.grid ? "grid" : o.filename.c_str(), o.position[0], o.position[1], o.position[2]);
+                            o.grid ? "grid" : o.filename.c_str(), o.position[0], o.position[1], o.position[2],
+                            o.first_order ? "true" : "false", o.damping_multiplier,
+                            o.adjust_damping_to_material ? "true" : "false");
                 if (load_meshes) {
                     // host-side mesh set-up only (no device call)
                     Sim::XPBDMeshObject m(o, cfg.time_step, cfg.g_accel);

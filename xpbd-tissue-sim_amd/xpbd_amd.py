@@ -52,7 +52,9 @@ class MeshDesc(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("g", C.c_double), ("iters", C.c_int32), ("solver", C.c_int32),
-                ("deterministic", C.c_int32), ("collision_every", C.c_int32), ("variant", C.c_int32)]
+                ("deterministic", C.c_int32), ("collision_every", C.c_int32), ("variant", C.c_int32),
+                ("first_order", C.c_int32), ("damping_multiplier", C.c_double), ("adjust_damping", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 def _sig(name, args, res=C.c_int):
@@ -227,12 +229,13 @@ VARIANTS = {"auto": 0, "simple": 1, "cluster": 2, "patch": 3, "quad": 4}
 
 
 def make_params(dt=1e-3, g=9.81, iters=10, solver="gs", deterministic=1, collision_every=1,
-                variant="auto") -> Params:
+                variant="auto", first_order=False, damping_multiplier=1.0, adjust_damping=False) -> Params:
     p = Params()
     p.dt, p.g, p.iters = dt, g, iters
     p.solver = SOLVERS[solver] if isinstance(solver, str) else solver
     p.deterministic, p.collision_every = deterministic, collision_every
     p.variant = VARIANTS[variant] if isinstance(variant, str) else variant
+    p.first_order, p.damping_multiplier, p.adjust_damping = int(first_order), damping_multiplier, int(adjust_damping)
     return p
 
 
@@ -240,10 +243,11 @@ class Solver:
     """A device solver for one object (or a batch of objects)."""
 
     def __init__(self, mesh, dt=1e-3, g=9.81, iters=10, solver="gs", deterministic=1, collision_every=1,
-                 variant="auto"):
+                 variant="auto", first_order=False, damping_multiplier=1.0, adjust_damping=False):
         meshes = mesh if isinstance(mesh, (list, tuple)) else [mesh]
         self.meshes = list(meshes)
-        self.params = make_params(dt, g, iters, solver, deterministic, collision_every, variant)
+        self.params = make_params(dt, g, iters, solver, deterministic, collision_every, variant, first_order,
+                                  damping_multiplier, adjust_damping)
         descs = (MeshDesc * len(meshes))(*[m.desc() for m in meshes])
         h = _vp()
         if len(meshes) == 1 and not isinstance(mesh, (list, tuple)):
