@@ -101,7 +101,13 @@ typedef struct xpbd_params {
     int32_t first_order;          /* 1 = first-order XPBD */
     double damping_multiplier;    /* b (config damping-multiplier) */
     int32_t adjust_damping;       /* 1 = scale B by E_i / (1 + nu_i) (adjust-damping-to-material) */
-    int32_t reserved;             /* must be 0 */
+    /* Local collision iterations (num-local-collision-iters,
+     * XPBDMeshObjectConfig.hpp:68; XPBDMeshObject.cpp:476-483, 764-825): after
+     * the global solve, this many extra passes over (a) every element attached to
+     * a vertex of a contact whose multiplier is non-zero, once per such
+     * (contact, face vertex) pair, then (b) every contact, with the multipliers
+     * kept.  0 = none. */
+    int32_t local_collision_iters;
 } xpbd_params;
 
 /* ---- object life cycle ------------------------------------------------- */
@@ -207,6 +213,19 @@ int xpbd_get_contacts(xpbd_handle* h, int32_t cap, int32_t* count, int32_t* face
                       double* n, double* bary);
 
 /* ---- timing --------------------------------------------------------------- */
+
+/* Attachment constraints: vertex vertices[a] is pulled to the world point
+ * targets[3a..3a+2] every iteration (alpha = 0, after the elastic and contact
+ * projectors).  Replaces XPBDMeshObject_::addAttachmentConstraint /
+ * clearAttachmentConstraints (src/simobject/XPBDMeshObject.cpp:255-275) with
+ * AttachmentConstraint (include/solver/constraint/AttachmentConstraint.hpp:31-71)
+ * and ConstraintProjector::project (ConstraintProjector.hpp:111-172):
+ * C = |x - p|, alpha = 0.  targets[3a..3a+2] is the reference's attached point
+ * plus offset.  Below |x - p| = 1e-12 no correction is made (the reference's
+ * substitute gradient there moves the vertex by ~3e11 |x - p|).  The call
+ * replaces the whole list (n = 0 clears); targets may be updated every step
+ * with the same vertices.  A vertex may carry one attachment. */
+int xpbd_set_attachments(xpbd_handle* h, int32_t n, const int32_t* vertices, const double* targets);
 
 /* Kernel timing mode: when on, xpbd_step launches without the HIP graph and
  * brackets the element-projection launches of every solver iteration (all

@@ -69,6 +69,11 @@ _sig("orc_step_dual_warm", [C.c_void_p, C.c_int, _dp, C.c_int])
 _sig("orc_step_learned", [C.c_void_p, C.c_int, _dp, C.c_int])
 _sig("orc_set_first_order", [C.c_void_p, C.c_double, C.c_int])
 _sig("orc_get_inverse_inertia", [C.c_void_p, _dp])
+_sig("orc_set_attachments", [C.c_void_p, C.c_int, _ip, _dp])
+_sig("orc_set_local_collision_iters", [C.c_void_p, C.c_int], None)
+_sig("orc_last_local_list_size", [C.c_void_p], C.c_int)
+_sig("orc_local_list_size", [C.c_void_p], C.c_int)
+_sig("orc_get_contact_lambdas", [C.c_void_p, _dp], None)
 
 
 def project_contact(x9, w3, cp, cn, bary, lam=0.0):
@@ -186,8 +191,34 @@ class Sim:
         """Switch to first-order XPBD (damping B as constraint inertia, alpha / dt)."""
         lib.orc_set_first_order(self.s, float(damping_multiplier), int(adjust_to_material))
 
+    def set_attachments(self, verts, targets):
+        """Attach vertices verts[i] to the points targets[i] (world coordinates)."""
+        v = np.ascontiguousarray(verts, np.int32)
+        t = _d(targets).reshape(-1)
+        self._att = (v, t)
+        lib.orc_set_attachments(self.s, len(v), p(v, C.c_int32), p(t))
+
     def inverse_inertia(self):
         return self._get(lib.orc_get_inverse_inertia, (self.nv,))
+
+    def set_local_collision_iters(self, n):
+        """Extra passes over the elements around active contacts, then all
+        contacts, after the global solve (num-local-collision-iters)."""
+        lib.orc_set_local_collision_iters(self.s, int(n))
+
+    def last_local_list_size(self):
+        """Element projections per local pass in the last step (repeats included)."""
+        return lib.orc_last_local_list_size(self.s)
+
+    def local_list_size(self):
+        """The same count for the current contact multipliers, by brute force."""
+        return lib.orc_local_list_size(self.s)
+
+    def contact_lambdas(self):
+        n = lib.orc_num_contacts(self.s)
+        out = np.zeros(max(n, 1))
+        lib.orc_get_contact_lambdas(self.s, p(out))
+        return out[:n]
 
     def set_sdf(self, kind, pos, quat=(0, 0, 0, 1), params=(0, 0, 0)):
         lib.orc_set_sdf(self.s, kind, p(_d(pos)), p(_d(quat)), p(_d(params)))

@@ -449,6 +449,13 @@ typedef struct orc_sim {
     int first_order;       /* 1: first-order XPBD (damping replaces mass) */
     double* dz1;           /* first order: per-vertex inertial move -g m dt / B */
     double *E, *nu;        /* element materials (kept for the first-order damping) */
+    int na;                /* attachments: vertex, target point, multiplier */
+    int* att_v;
+    double *att_p, *att_lam;
+    int local_iters;       /* num-local-collision-iters */
+    int order_set;         /* s->order was set by orc_set_order (device order) */
+    int *v2e_off, *v2e;    /* elements attached to each vertex, ascending (TetMesh.cpp:88-96) */
+    int last_local_n;      /* element projections per local pass in the last step */
 } orc_sim;
 
 /* XPBDMeshObject.cpp:102-160 (setup) + 277-331 (masses) + 352-417 (constraints) */
@@ -572,12 +579,15 @@ ORC_API void orc_destroy(orc_sim* s)
     free(s->lam); free(s->order); free(s->faces); free(s->jbuf); free(s->jidx);
     free(s->c_face); free(s->c_p); free(s->c_n); free(s->c_bary); free(s->c_lam);
     free(s->face_rank); free(s->corder); free(s->dz1); free(s->E); free(s->nu);
+    free(s->att_v); free(s->att_p); free(s->att_lam);
+    free(s->v2e_off); free(s->v2e);
     free(s);
 }
 
 ORC_API void orc_set_order(orc_sim* s, const int* order)
 {
     memcpy(s->order, order, sizeof(int) * s->nt);
+    s->order_set = 1;
 }
 
 ORC_API void orc_set_mode(orc_sim* s, int mode, int iters)
@@ -723,6 +733,45 @@ ORC_API void orc_get_contacts(const orc_sim* s, int* face, double* p, double* n,
  * Jacobi: include/solver/xpbd_solver/XPBDJacobiSolver.hpp:79-114
  *         (all projectors read the same positions, updates buffered in projector
  *          order and summed into x afterwards, no relaxation) */
+/* Attachment of vertex att_v[a] to the point att_p[a] = attached point + offset
+ * (AttachmentConstraint.hpp:33-88 with ConstraintProjector.hpp:111-172,
+ * alpha = 0): C = |x - p|, gradient (x - p) / |x - p|.  Below |x - p| = 1e-12
+ * the reference substitutes a 1e-12 gradient (AttachmentConstraint.hpp:76-81),
+ * which with C = |x - p| moves the vertex by ~3e11 |x - p| along -(1, 1, 1);
+ * its GPU twin (GPUAttachmentConstraint.cuh:74-86) divides by zero there.  The
+ * constraint is met at that distance, so this makes no correction. */
+static void project_attachment(orc_sim* s, int a, double* dx3)
+{
+    const double* x = s->x + 3 * s->att_v[a];
+    const double* p = s->att_p + 3 * a;
+    const double d[3] = { x[0] - p[0], x[1] - p[1], x[2] - p[2] };
+    const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (dist < 1e-12) {
+        dx3[0] = dx3[1] = dx3[2] = 0.0;
+        return;
+    }
+    double g[3];
+    for (int k = 0; k < 3; k++) g[k] = d[k] / dist;
+    const double w = s->w[s->att_v[a]];
+    const double lhs = w * (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    const double dlam = (-dist - 0.0 * s->att_lam[a]) / lhs;
+    s->att_lam[a] += dlam;
+    for (int k = 0; k < 3; k++) dx3[k] = w * g[k] * dlam;
+}
+
+ORC_API void orc_set_attachments(orc_sim* s, int n, const int* verts, const double* targets)
+{
+    free(s->att_v); free(s->att_p); free(s->att_lam);
+    s->na = n;
+    s->att_v = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+    s->att_p = (double*)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+    s->att_lam = (double*)calloc(n > 0 ? n : 1, sizeof(double));
+    if (n > 0) {
+        memcpy(s->att_v, verts, sizeof(int) * n);
+        memcpy(s->att_p, targets, sizeof(double) * 3 * n);
+    }
+}
+
 static void iterate(orc_sim* s)
 {
     const int nt = s->nt;
@@ -746,6 +795,12 @@ static void iterate(orc_sim* s)
             if (project_contact(p, w, s->c_p + 3 * c, s->c_n + 3 * c, s->c_bary + 3 * c, s->c_lam + c, dx))
                 for (int i = 0; i < 3; i++)
                     for (int k = 0; k < 3; k++) s->x[3 * f[i] + k] += dx[3 * i + k];
+        }
+        /* attachments follow the collision projectors (type-list order, XPBDTypedefs.hpp:56) */
+        for (int a = 0; a < s->na; a++) {
+            double dx[3];
+            project_attachment(s, a, dx);
+            for (int k = 0; k < 3; k++) s->x[3 * s->att_v[a] + k] += dx[k];
         }
         return;
     }
@@ -777,10 +832,151 @@ static void iterate(orc_sim* s)
             }
         }
     }
+    double* abuf = NULL;
+    if (s->na) {
+        abuf = (double*)malloc(sizeof(double) * 3 * s->na);
+        for (int a = 0; a < s->na; a++) project_attachment(s, a, abuf + 3 * a);
+    }
     for (size_t u = 0; u < nu; u++) s->x[s->jidx[u]] += s->jbuf[u];
     for (int u = 0; u < ncu; u++) s->x[cidx[u]] += cbuf[u];
+    for (int a = 0; a < s->na; a++)
+        for (int k = 0; k < 3; k++) s->x[3 * s->att_v[a] + k] += abuf[3 * a + k];
     free(cbuf);
     free(cidx);
+    free(abuf);
+}
+
+/* Local collision iterations (XPBDMeshObject.cpp:476-483 with the gather at
+ * XPBDMeshObject.cpp:764-825 and XPBDSolver.hpp:106-121), after the global solve:
+ * the projector list holds, for every contact whose multiplier is non-zero
+ * (insertion order), for each of its three face vertices, every element attached
+ * to that vertex (ascending element index) -- an element appears once per such
+ * (contact, vertex) pair -- followed by every contact.  Each of local_iters
+ * passes projects that list without resetting any multiplier: Gauss-Seidel
+ * applies each projection at once (XPBDGaussSeidelSolver.hpp:44-53), Jacobi
+ * buffers all of them and sums them afterwards (XPBDJacobiSolver.hpp:119-145).
+ * The reference visits the element list in gather order; when a device order
+ * is set (orc_set_order) the list is visited in that order instead, repeats of
+ * an element back to back, which is how the colour-ordered device sequences it. */
+static const int* g_elem_rank;
+static int cmp_elem(const void* a, const void* b)
+{
+    const int ra = g_elem_rank[*(const int*)a], rb = g_elem_rank[*(const int*)b];
+    return (ra > rb) - (ra < rb);
+}
+
+static void local_collision_iterations(orc_sim* s)
+{
+    s->last_local_n = 0;
+    if (s->local_iters <= 0 || s->nc == 0) return;
+    const int nv = s->nv, nt = s->nt;
+    if (!s->v2e_off) {
+        s->v2e_off = (int*)calloc((size_t)nv + 1, sizeof(int));
+        s->v2e = (int*)malloc(sizeof(int) * 4 * (size_t)(nt > 0 ? nt : 1));
+        for (int e = 0; e < nt; e++)
+            for (int j = 0; j < 4; j++) s->v2e_off[s->tets[4 * e + j] + 1]++;
+        for (int v = 0; v < nv; v++) s->v2e_off[v + 1] += s->v2e_off[v];
+        int* fill = (int*)malloc(sizeof(int) * (size_t)(nv > 0 ? nv : 1));
+        memcpy(fill, s->v2e_off, sizeof(int) * (size_t)nv);
+        for (int e = 0; e < nt; e++)
+            for (int j = 0; j < 4; j++) s->v2e[fill[s->tets[4 * e + j]]++] = e;
+        free(fill);
+    }
+    size_t n = 0, cap = 64;
+    int* list = (int*)malloc(sizeof(int) * cap);
+    for (int c = 0; c < s->nc; c++) {
+        if (s->c_lam[c] == 0) continue;
+        const int* f = s->faces + 3 * s->c_face[c];
+        for (int i = 0; i < 3; i++)
+            for (int k = s->v2e_off[f[i]]; k < s->v2e_off[f[i] + 1]; k++) {
+                if (n == cap) {
+                    cap *= 2;
+                    list = (int*)realloc(list, sizeof(int) * cap);
+                }
+                list[n++] = s->v2e[k];
+            }
+    }
+    s->last_local_n = (int)n;
+    if (s->order_set && n > 1) {
+        int* rank = (int*)malloc(sizeof(int) * (size_t)nt);
+        for (int i = 0; i < nt; i++) rank[s->order[i]] = i;
+        g_elem_rank = rank;
+        qsort(list, n, sizeof(int), cmp_elem);  /* equal keys are the same element */
+        free(rank);
+    }
+    double* buf = s->mode == ORC_JACOBI ? (double*)malloc(sizeof(double) * 12 * (n > 0 ? n : 1)) : NULL;
+    double* cbuf = s->mode == ORC_JACOBI ? (double*)malloc(sizeof(double) * 9 * (size_t)s->nc) : NULL;
+    int* cact = s->mode == ORC_JACOBI ? (int*)malloc(sizeof(int) * (size_t)s->nc) : NULL;
+    for (int it = 0; it < s->local_iters; it++) {
+        for (size_t u = 0; u < n; u++) {
+            const int e = list[u];
+            const int* t = s->tets + 4 * e;
+            const double* p[4] = { s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3] };
+            const double w[4] = { s->w[t[0]], s->w[t[1]], s->w[t[2]], s->w[t[3]] };
+            double dx[12];
+            project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->ts, s->lam + 2 * e,
+                             buf ? buf + 12 * u : dx);
+            if (!buf)
+                for (int i = 0; i < 4; i++)
+                    for (int k = 0; k < 3; k++) s->x[3 * t[i] + k] += dx[3 * i + k];
+        }
+        for (int ci = 0; ci < s->nc; ci++) {
+            const int c = buf ? ci : s->corder[ci];
+            const int* f = s->faces + 3 * s->c_face[c];
+            const double* p[3] = { s->x + 3 * f[0], s->x + 3 * f[1], s->x + 3 * f[2] };
+            const double w[3] = { s->w[f[0]], s->w[f[1]], s->w[f[2]] };
+            double dx[9];
+            const int act = project_contact(p, w, s->c_p + 3 * c, s->c_n + 3 * c, s->c_bary + 3 * c, s->c_lam + c,
+                                            buf ? cbuf + 9 * c : dx);
+            if (buf) cact[c] = act;
+            else if (act)
+                for (int i = 0; i < 3; i++)
+                    for (int k = 0; k < 3; k++) s->x[3 * f[i] + k] += dx[3 * i + k];
+        }
+        if (buf) {
+            for (size_t u = 0; u < n; u++) {
+                const int* t = s->tets + 4 * list[u];
+                for (int i = 0; i < 4; i++)
+                    for (int k = 0; k < 3; k++) s->x[3 * t[i] + k] += buf[12 * u + 3 * i + k];
+            }
+            for (int c = 0; c < s->nc; c++) {
+                if (!cact[c]) continue;
+                const int* f = s->faces + 3 * s->c_face[c];
+                for (int i = 0; i < 3; i++)
+                    for (int k = 0; k < 3; k++) s->x[3 * f[i] + k] += cbuf[9 * c + 3 * i + k];
+            }
+        }
+    }
+    free(list);
+    free(buf);
+    free(cbuf);
+    free(cact);
+}
+
+ORC_API void orc_set_local_collision_iters(orc_sim* s, int n) { s->local_iters = n > 0 ? n : 0; }
+ORC_API int orc_last_local_list_size(const orc_sim* s) { return s->last_local_n; }
+ORC_API void orc_get_contact_lambdas(const orc_sim* s, double* out)
+{
+    memcpy(out, s->c_lam, sizeof(double) * (size_t)s->nc);
+}
+
+/* Number of element projections in one local pass (the gathered list length,
+ * repeats included), counted by brute force over all elements, for checks of
+ * the gather. */
+ORC_API int orc_local_list_size(const orc_sim* s)
+{
+    int n = 0;
+    for (int c = 0; c < s->nc; c++) {
+        if (s->c_lam[c] == 0) continue;
+        const int* f = s->faces + 3 * s->c_face[c];
+        for (int i = 0; i < 3; i++) {
+            int cnt = 0;
+            for (int e = 0; e < s->nt; e++)
+                for (int j = 0; j < 4; j++) cnt += s->tets[4 * e + j] == f[i];
+            n += cnt;
+        }
+    }
+    return n;
 }
 
 /* The per-object part of one time step, with contacts optionally re-detected
@@ -799,10 +995,14 @@ ORC_API void orc_step(orc_sim* s, int detect)
     const double dt = s->dt;
     move_inertially(s);
     memset(s->lam, 0, sizeof(double) * 2 * (size_t)s->nt);
+    for (int a = 0; a < s->na; a++) s->att_lam[a] = 0;
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
     for (int it = 0; it < s->iters; it++) iterate(s);
-    /* EVERY_SUBSTEP residual is taken here, before the pin reset (XPBDSolver.hpp:91-96) */
+    /* EVERY_SUBSTEP residual is taken here, at the end of the global solve,
+     * before the local collision iterations and the pin reset (XPBDSolver.hpp:91-96,
+     * XPBDMeshObject.cpp:473-503) */
     s->last_res = orc_residual(s, NULL);
+    local_collision_iterations(s);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])
             for (int k = 0; k < 3; k++) s->x[3 * i + k] = s->xp[3 * i + k];
@@ -829,6 +1029,7 @@ ORC_API void orc_step_corrected(orc_sim* s, int detect, const double* corr, int 
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
     for (int it = 0; it < iters; it++) iterate(s);
     s->last_res = orc_residual(s, NULL);
+    local_collision_iterations(s);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])
             for (int k = 0; k < 3; k++) s->x[3 * i + k] = s->xp[3 * i + k];
@@ -873,6 +1074,7 @@ ORC_API void orc_step_dual_warm(orc_sim* s, int detect, const double* lam0, int 
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
     for (int it = 0; it < iters; it++) iterate(s);
     s->last_res = orc_residual(s, NULL);
+    local_collision_iterations(s);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])
             for (int k = 0; k < 3; k++) s->x[3 * i + k] = s->xp[3 * i + k];
@@ -918,6 +1120,7 @@ ORC_API void orc_step_learned(orc_sim* s, int detect, const double* corr, int it
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
     for (int it = 0; it < iters; it++) iterate(s);
     s->last_res = orc_residual(s, NULL);
+    local_collision_iterations(s);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])
             for (int k = 0; k < 3; k++) s->x[3 * i + k] = s->xp[3 * i + k];

@@ -86,3 +86,38 @@ def test_block_with_sphere_on_reference_collision_cadence(tmp_path):
           f"{detections} detections in {steps} steps")
     assert 0 < detections < steps
     assert np.array_equal(x, xg)
+
+
+def test_grasping_simulation_matches_oracle(tmp_path):
+    """Headless GraspingSimulation: the non-fixed surface vertices within the
+    grasp radius of the cursor are attached with their offsets from it, and the
+    cursor lifts at 0.1 m/s, moving before every step; against the oracle with
+    the same attachments and targets."""
+    steps, radius, vel, dt = 100, 0.015, np.array([0.02, 0.0, 0.1]), 1e-3
+    rest, att = tmp_path / "rest.bin", tmp_path / "att.bin"
+    x, order, _, _ = drive("grasp_block.yaml", tmp_path, steps, "--grasp", str(radius), "--cursor-velocity",
+                           ",".join(map(str, vel)), "--dump-rest", str(rest), "--dump-attachments", str(att))
+    xr = np.fromfile(rest, np.float64).reshape(-1, 3)
+    verts = np.fromfile(att, np.int32)
+    _, t = X.grid_mesh(8, 8, 8, 0.01)
+    pinned = (np.abs(xr[:, 2] - xr[:, 2].min()) < 1e-9).astype(np.uint8)
+    # selection rule, by brute force: surface, not fixed, within the radius of the
+    # top-face centre, vertex order
+    lo, hi = xr.min(0), xr.max(0)
+    cursor = np.array([(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, hi[2]])
+    surf = np.zeros(len(xr), bool)
+    surf[X.surface_faces(len(xr), t).reshape(-1)] = True
+    expect = np.where(surf & (pinned == 0) & (np.linalg.norm(xr - cursor, axis=1) <= radius))[0]
+    assert len(expect) > 0 and np.array_equal(verts, expect)
+    mesh = X.Mesh(x0=xr, tets=t, E=np.full(len(t), 5e3), nu=np.full(len(t), 0.45), rho=np.full(len(t), 1000.0),
+                  pinned=pinned)
+    o = orc.Sim(mesh, dt=dt, g=9.81, iters=10, mode="gs")
+    o.set_order(order)
+    for n in range(steps):
+        o.set_attachments(verts, xr[verts] + vel * dt * (n + 1))
+        o.step(1)
+    err = rel_err(x, o.x())
+    lift = x[verts, 2].mean() - xr[verts, 2].mean()
+    print(f"grasp via xpbd_sim: {len(verts)} vertices, rel err {err:.2e}, mean lift {lift:.4f} m")
+    assert lift > 0.009
+    assert err <= 1e-5

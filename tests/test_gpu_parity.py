@@ -202,3 +202,110 @@ def test_first_order_100_steps(solver, variant, E_range, b, adjust):
     assert errs.max() <= TOL
     # energy uses alpha = alpha~ dt for first order
     assert gpu.energy() == pytest.approx(o.energy(), rel=1e-5)
+
+
+@pytest.mark.parametrize("solver,variant,det", [("gs", "quad", 1), ("gs", "cluster", 1), ("gs", "simple", 1),
+                                                ("jacobi", "patch", 0), ("jacobi", "patch", 1), ("jacobi", "simple", 0)])
+def test_attachments_pull_100_steps(solver, variant, det):
+    """Three top vertices of a base-pinned block grasped and pulled along a moving
+    path (targets reset every step, AttachmentConstraint.hpp:31-71), against the
+    oracle's attachment projector."""
+    rng = np.random.default_rng(5)
+    n = 6
+    E, nu = W.heterogeneous(6 * n ** 3, rng, E_range=(1e3, 5e3))
+    mesh = W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.0, rng=rng)
+    top = np.where(np.abs(mesh.x0[:, 2] - n * 0.01) < 1e-9)[0]
+    grasp = top[[0, len(top) // 2, len(top) - 1]]
+    gpu, o = _pair(mesh, solver, 10 if solver == "jacobi" else 5, variant=variant, det=det)
+    errs = []
+    for k in range(100):
+        tgt = mesh.x0[grasp] + np.array([0.004, -0.002, 0.01]) * min(k + 1, 50) / 50
+        gpu.set_attachments(grasp, tgt)
+        o.set_attachments(grasp, tgt)
+        gpu.step(1)
+        o.step(1)
+        errs.append(rel_err(gpu.positions(), o.x()))
+    errs = np.array(errs)
+    moved = np.max(np.abs(o.x() - mesh.x0))
+    print(f"attachments {solver}/{variant}/det{det}: max rel err {errs.max():.3e}, max displacement {moved:.3e} m, "
+          f"grasp gap {np.max(np.abs(gpu.positions()[grasp] - tgt)):.3e} m")
+    assert moved > 5e-3
+    assert errs.max() <= TOL
+
+
+def test_attachments_reject_bad_lists_and_clear():
+    mesh = W.cube(4, seed=2)
+    s = X.Solver(mesh, dt=1e-3, iters=5, solver="gs")
+    t = mesh.x0[[1, 2]]
+    with pytest.raises(RuntimeError, match="out of range"):
+        s.set_attachments([1, mesh.x0.shape[0]], t)
+    with pytest.raises(RuntimeError, match="one attachment"):
+        s.set_attachments([1, 1], t)
+    with pytest.raises(RuntimeError):
+        s.set_attachments([1, 2], np.array([[0.0, 0.0, np.nan], [0.0, 0.0, 0.0]]))
+    # grow, shrink, clear: cleared solver matches a solver that never had attachments
+    s.set_attachments([1, 2], t)
+    s.set_attachments(np.arange(10), mesh.x0[:10])
+    s.set_attachments([3], mesh.x0[[3]])
+    s.set_attachments([], np.zeros((0, 3)))
+    ref = X.Solver(mesh, dt=1e-3, iters=5, solver="gs")
+    s.step(20)
+    ref.step(20)
+    np.testing.assert_array_equal(s.positions(), ref.positions())
+
+
+@pytest.mark.parametrize("solver,variant,E_range,det,g,off,min_local",
+                         [("gs", "quad", (1e3, 5e4), 1, 9.81, 0.0, 50), ("gs", "cluster", (1e3, 5e4), 1, 9.81, 0.0, 50),
+                          ("gs", "simple", (1e3, 5e4), 1, 9.81, 0.0, 50),
+                          ("jacobi", "patch", (3e2, 1e3), 0, 3.0, 0.0013, 5),
+                          ("jacobi", "patch", (3e2, 1e3), 1, 3.0, 0.0013, 5),
+                          ("jacobi", "simple", (3e2, 1e3), 0, 3.0, 0.0013, 5)])
+def test_local_collision_iterations_100_steps(solver, variant, E_range, det, g, off, min_local):
+    """num-local-collision-iters = 5: after each global solve, five passes over the
+    elements around contacts with a non-zero multiplier (repeats kept) and then
+    all contacts (XPBDMeshObject.cpp:476-483, 764-825), against the oracle's
+    local passes; sphere pressed into a base-pinned block, contacts re-detected
+    every step.
+
+    Jacobi is held to a set-up with few contact steps: under sustained sphere
+    contact the reference's Jacobi (contacts at alpha = 0 summed with the elastic
+    corrections, no relaxation) amplifies rounding, and the device and the FP64
+    oracle drift apart by ~1e-3 over 100 steps with or without local passes
+    (E 3e2-5e3 without gravity; the error grows smoothly, 3e-8 at step 25 to
+    2e-3 at step 99).  Off the block's axis, with weak gravity, it stays at ~3e-6."""
+    n = 8
+    rng = np.random.default_rng(5)
+    E, nu = W.heterogeneous(6 * n ** 3, rng, E_range=E_range)
+    mesh = W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.0, faces=True, rng=rng)
+    gpu = X.Solver(mesh, dt=1e-3, g=g, iters=10, solver=solver, variant=variant, deterministic=det, collision_every=1,
+                   local_collision_iters=5)
+    o = orc.Sim(mesh, dt=1e-3, g=g, iters=10, mode="gs" if solver == "gs" else "jacobi")
+    o.set_local_collision_iters(5)
+    r, L = 0.02, 0.01 * n
+    pos0, quat, prm, vel = (L / 2 + off, L / 2 + 0.37 * off, L + r - 0.002), (0, 0, 0, 1), (r, 0, 0), (0, 0, -0.1)
+    gpu.set_sdf(X.SDF_SPHERE, pos0, quat, prm)
+    gpu.set_sdf_velocity(vel)
+    if solver == "gs":
+        o.set_order(gpu.tet_order())
+        o.set_contact_order(gpu.face_rank())
+    counts, local = [], []
+    errs = []
+    for k in range(100):
+        o.set_sdf(orc.SDF_SPHERE, np.asarray(pos0) + np.asarray(vel) * k * 1e-3, quat, prm)
+        gpu.step(1)
+        o.step(1, detect=True)
+        counts.append(len(gpu.contacts()[0]))
+        local.append(o.last_local_list_size())
+        errs.append(rel_err(gpu.positions(), o.x()))
+    errs = np.array(errs)
+    print(f"local {solver}/{variant}/det{det}: max rel err {errs.max():.3e}, contacts max {max(counts)}, "
+          f"local projections per pass max {max(local)}, steps with local passes {sum(x > 0 for x in local)}")
+    assert sum(x > 0 for x in local) >= min_local
+    assert errs.max() <= TOL
+    # the local passes change the result
+    plain = X.Solver(mesh, dt=1e-3, g=g, iters=10, solver=solver, variant=variant, deterministic=det,
+                     collision_every=1)
+    plain.set_sdf(X.SDF_SPHERE, pos0, quat, prm)
+    plain.set_sdf_velocity(vel)
+    plain.step(100)
+    assert np.abs(plain.positions() - gpu.positions()).max() > 1e-7

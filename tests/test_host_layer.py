@@ -55,6 +55,7 @@ objects:
     position: [0, 0, 2]
     velocity: [0, 0, 0.0]
     num-solver-iters: 7
+    num-local-collision-iters: 3
     solver-type: "Parallel-Jacobi"
     # solver-type: "Jacobi"
     residual-policy: "Every-Substep"
@@ -74,7 +75,8 @@ def test_reference_style_config(tmp_path):
     assert c["name"] == "Style_Test" and c["g_accel"] == 9.5 and c["time_step"] == 5e-4
     assert c["collision_rate"] == 250 and c["rigid_objects"] == 1
     o = c["objects"][0]
-    assert o == {"name": "Cube1", "solver_type": "Parallel-Jacobi", "iters": 7, "E": 6e4, "nu": 0.45,
+    assert o == {"name": "Cube1", "solver_type": "Parallel-Jacobi", "iters": 7, "local_collision_iters": 3,
+                 "E": 6e4, "nu": 0.45,
                  "density": 1100, "collisions": False, "source": "mesh.msh", "position": [0, 0, 2],
                  "first_order": False, "damping_multiplier": 1, "adjust_damping": False}
 

@@ -485,3 +485,114 @@ def test_first_order_low_damping_is_rounding_chaotic():
     print(f"sensitivity to a 1e-9 perturbation: low damping {chaotic:.2e}, material-scaled {calm:.2e}")
     assert chaotic > 1e-3
     assert calm < 1e-7
+
+
+# ---------------------------------------------------------------- attachments (next row)
+def test_attachment_gs_projection_reaches_the_target_exactly():
+    """alpha = 0 and C = |x - p|: one Gauss-Seidel attachment projection moves the
+    vertex onto its target (the elastic projectors run first in the same iteration)."""
+    import workloads as W
+    m = W.grid_object(3, 3, 3, 0.01, E=3e3, nu=0.45, pin_axis=2, noise=0.0)
+    top = np.where(np.abs(m.x0[:, 2] - 0.03) < 1e-9)[0][[0, 5, 9]]
+    tgt = m.x0[top] + np.array([[0.001, -0.002, 0.003], [0.0, 0.0, 0.002], [-0.001, 0.0, 0.0]])
+    s = orc.Sim(m, iters=1, mode="gs")
+    s.set_attachments(top, tgt)
+    s.step(1)
+    np.testing.assert_allclose(s.x()[top], tgt, rtol=0, atol=1e-15)
+
+
+def test_attachment_at_target_leaves_the_vertex_alone():
+    """Below |x - p| = 1e-12 the constraint is met and nothing moves."""
+    import workloads as W
+    m = W.grid_object(2, 2, 2, 0.01, E=3e3, nu=0.45, pin_axis=None, noise=0.0)
+    s = orc.Sim(m, g=0.0, iters=3, mode="gs")
+    s.set_attachments([4], m.x0[[4]])
+    s.step(5)
+    np.testing.assert_allclose(s.x(), m.x0, rtol=0, atol=1e-15)
+
+
+def test_attachment_jacobi_is_the_plain_sum_of_both_corrections():
+    """Jacobi: the attachment correction is computed from the pre-iteration
+    positions and summed with the elastic corrections (XPBDJacobiSolver.hpp:79-114)."""
+    import workloads as W
+    m = W.grid_object(2, 2, 2, 0.01, E=3e3, nu=0.45, pin_axis=None, noise=0.0)
+    v, tgt = 7, m.x0[7] + np.array([0.0, 0.0, 0.001])
+    a = orc.Sim(m, g=0.0, iters=1, mode="jacobi")
+    a.set_attachments([v], [tgt])
+    a.step(1)
+    b = orc.Sim(m, g=0.0, iters=1, mode="jacobi")
+    b.step(1)
+    # at rest the elastic corrections vanish, so the sum is the attachment alone
+    expect = m.x0.copy()
+    expect[v] = tgt
+    np.testing.assert_allclose(b.x(), m.x0, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(a.x(), expect, rtol=0, atol=1e-15)
+
+
+# ---------------------------------------------------------------- local collision iterations (next row)
+def _local_case(mode, k, iters=10):
+    import workloads as W
+    n = 6
+    rng = np.random.default_rng(5)
+    E, nu = W.heterogeneous(6 * n ** 3, rng, E_range=(1e3, 5e3))
+    m = W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.0, faces=True, rng=rng)
+    r, L = 0.02, 0.01 * n
+    s = orc.Sim(m, iters=iters, mode=mode)
+    s.set_sdf(orc.SDF_SPHERE, (L / 2, L / 2, L + r - 0.002), (0, 0, 0, 1), (r, 0, 0))
+    s.set_local_collision_iters(k)
+    s.step(1, detect=True)
+    return m, s
+
+
+def _local_sets(m, plain):
+    """Brute force from the state after the global solve (a twin without local
+    passes): the contacts with a non-zero multiplier, the element list (with
+    repeats) the reference gathers for them, and the vertices any local pass
+    can move."""
+    face, _, _, _ = plain.contacts()
+    active = face[plain.contact_lambdas() != 0]
+    count = sum(int(np.any(m.tets == v, axis=1).sum()) for f in active for v in m.faces[f])
+    listed = np.where(np.isin(m.tets, m.faces[active].reshape(-1)).any(1))[0]
+    touched = np.union1d(m.tets[listed].reshape(-1), m.faces[face].reshape(-1))
+    return active, count, listed, touched
+
+
+@pytest.mark.parametrize("mode", ["gs", "jacobi"])
+def test_local_collision_gather_counts_each_contact_vertex_element_pair(mode):
+    """XPBDMeshObject.cpp:776-806: one entry per (active contact, face vertex,
+    attached element), repeats kept; counted here by scanning all elements."""
+    m, plain = _local_case(mode, 0)
+    _, s = _local_case(mode, 1)
+    active, count, _, _ = _local_sets(m, plain)
+    assert len(active) > 0
+    assert s.last_local_list_size() == count == plain.local_list_size()
+
+
+@pytest.mark.parametrize("mode,k", [("gs", 10), ("jacobi", 1000)])
+def test_local_collision_passes_reach_the_fixed_point_of_the_listed_constraints(mode, k):
+    """Repeated local passes converge to the XPBD fixed point of the listed
+    elements (r = C + alpha~ lambda = 0, with lambda carried over from the global
+    solve, not reset) while elements no local projection touches keep exactly
+    their state after the global solve."""
+    m, plain = _local_case(mode, 0)
+    _, s = _local_case(mode, k)
+    _, _, listed, touched = _local_sets(m, plain)
+    _, r0 = plain.residual(vec=True)
+    _, r = s.residual(vec=True)
+    assert np.abs(r0[listed]).max() > 1e-9
+    assert np.abs(r[listed]).max() <= 1e-12
+    far = ~np.isin(m.tets, touched).any(1)
+    assert far.sum() > 0
+    np.testing.assert_array_equal(r[far], r0[far])
+
+
+def test_local_collision_passes_do_nothing_without_contacts():
+    import workloads as W
+    m = W.grid_object(3, 3, 3, 0.01, E=3e3, nu=0.45, pin_axis=2, noise=0.0, faces=True)
+    a = orc.Sim(m, iters=5, mode="gs")
+    b = orc.Sim(m, iters=5, mode="gs")
+    b.set_local_collision_iters(4)
+    a.step(10)
+    b.step(10)
+    np.testing.assert_array_equal(a.x(), b.x())
+    assert b.last_local_list_size() == 0

@@ -164,8 +164,10 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     if (!(P->dt > 0) || !std::isfinite(P->dt) || !std::isfinite(P->g) || P->iters < 0 ||
         (P->solver != XPBD_SOLVER_JACOBI && P->solver != XPBD_SOLVER_GS_COLORED) || P->collision_every < 0 ||
         (P->first_order != 0 && P->first_order != 1) ||
-        (P->first_order && !(P->damping_multiplier > 0 && std::isfinite(P->damping_multiplier))))
-        return set_err(nullptr, XPBD_E_INVALID_ARG, "params: need dt > 0, finite g, iters >= 0, known solver");
+        (P->first_order && !(P->damping_multiplier > 0 && std::isfinite(P->damping_multiplier))) ||
+        P->local_collision_iters < 0)
+        return set_err(nullptr, XPBD_E_INVALID_ARG,
+                       "params: need dt > 0, finite g, iters >= 0, local_collision_iters >= 0, known solver");
     for (int32_t i = 0; i < n; i++) {
         int rc = validate_desc(&meshes[i]);
         if (rc) return rc;
@@ -441,6 +443,23 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     rc |= dalloc(h, &h->d_cn, 3 * (size_t)nf);
     rc |= dalloc(h, &h->d_cb, 3 * (size_t)nf);
     rc |= dalloc(h, &h->d_clam, nf);
+    // local collision iterations: elements attached to each vertex, as solver positions
+    std::vector<int32_t> vt_off, vt_pos;
+    if (P->local_collision_iters > 0 && nf > 0) {
+        vt_off.assign((size_t)nv + 1, 0);
+        vt_pos.resize(4 * (size_t)nt);
+        for (int e = 0; e < nt; e++)
+            for (int j = 0; j < 4; j++) vt_off[tets[4 * (size_t)e + j] + 1]++;
+        for (int v = 0; v < nv; v++) vt_off[v + 1] += vt_off[v];
+        std::vector<int32_t> fill(vt_off.begin(), vt_off.end() - 1);
+        for (int i = 0; i < nt; i++) {
+            const int e = h->order[i];
+            for (int j = 0; j < 4; j++) vt_pos[fill[tets[4 * (size_t)e + j]]++] = i;
+        }
+        rc |= dalloc(h, &h->d_vt_off, vt_off.size());
+        rc |= dalloc(h, &h->d_vt_pos, vt_pos.size());
+        rc |= dalloc(h, &h->d_mult, nt);
+    }
     if (rc) {
         std::string msg = h->err;
         xpbd_destroy(h);
@@ -511,6 +530,10 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     }
     if (nf > 0) UP(h->d_faces, hf.data(), sizeof(int4) * nf);
     UP(h->d_fcol_off, h->fcol_off.data(), sizeof(int32_t) * h->fcol_off.size());
+    if (h->d_vt_off) {
+        UP(h->d_vt_off, vt_off.data(), sizeof(int32_t) * vt_off.size());
+        UP(h->d_vt_pos, vt_pos.data(), sizeof(int32_t) * vt_pos.size());
+    }
 #undef UP
     // zero-initialised device state (fill kernels, then wait)
     launch_fill_zero(h->stream, h->d_acc, sizeof(float) * 4 * (size_t)nv);
@@ -596,7 +619,18 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
                                        h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
                 if (e != hipSuccess) return hip_err(h, e, "contacts_gs launch");
             }
+            // attachments follow the collision projectors (type-list order, XPBDTypedefs.hpp:56)
+            if (h->natt > 0) {
+                e = launch_attach_gs(s, h->d_att_v, h->d_att_p, h->d_att_lam, h->d_rest, h->d_pos, h->d_lo, h->natt,
+                                     first);
+                if (e != hipSuccess) return hip_err(h, e, "attach_gs launch");
+            }
         } else if (h->variant == XPBD_VARIANT_PATCH) {
+            if (h->natt > 0) {
+                e = launch_attach_jacobi(s, h->d_att_v, h->d_att_p, h->d_att_lam, h->d_rest, h->d_pos, h->d_lo,
+                                         h->d_acc, h->natt, first);
+                if (e != hipSuccess) return hip_err(h, e, "attach_jacobi launch");
+            }
             if (con) {
                 // contact corrections read the pre-iteration positions, so they run first
                 e = launch_contacts_jacobi(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
@@ -605,7 +639,7 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
             }
             if (timed) timed_begin(h);
             e = launch_jacobi_patch(s, ta, patch_arrays(h), h->d_pos, reinterpret_cast<float4*>(h->d_acc), h->nt,
-                                    first, h->params.deterministic != 0, con);
+                                    first, h->params.deterministic != 0, con || h->natt > 0);
             if (e != hipSuccess) return hip_err(h, e, "jacobi_patch launch");
             if (timed) timed_end(h);
         } else {
@@ -618,10 +652,57 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
                                            h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
                 if (e != hipSuccess) return hip_err(h, e, "contacts_jacobi launch");
             }
+            if (h->natt > 0) {
+                e = launch_attach_jacobi(s, h->d_att_v, h->d_att_p, h->d_att_lam, h->d_rest, h->d_pos, h->d_lo,
+                                         h->d_acc, h->natt, first);
+                if (e != hipSuccess) return hip_err(h, e, "attach_jacobi launch");
+            }
             e = launch_jacobi_apply(s, h->d_pos, h->d_acc, h->nv);
             if (e != hipSuccess) return hip_err(h, e, "jacobi_apply launch");
         }
         if (timed) h->timed_iterations++;
+    }
+    return XPBD_OK;
+}
+
+// Local collision iterations after the global solve (XPBDMeshObject.cpp:476-483,
+// 764-825): count the element projections of a pass from the contacts whose
+// multiplier is non-zero, then local_collision_iters passes of (elements,
+// contacts) with the multipliers kept.  Gauss-Seidel visits the elements in the
+// global solve's order, an element's repeats back to back; Jacobi sums every
+// projection of a pass from the same positions.  Attachments are not part of it.
+int enqueue_local_collision(xpbd_handle* h)
+{
+    const int n = h->params.local_collision_iters;
+    if (n <= 0 || !h->d_mult || !contacts_enabled(h)) return XPBD_OK;
+    hipStream_t s = h->stream;
+    const TetArrays ta = tet_arrays(h);
+    hipError_t e = launch_fill_zero(s, h->d_mult, sizeof(int32_t) * (size_t)h->nt);
+    if (e == hipSuccess)
+        e = launch_local_count(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_clam, h->d_vt_off, h->d_vt_pos,
+                               h->d_mult);
+    if (e != hipSuccess) return hip_err(h, e, "local collision count launch");
+    const bool gs = h->params.solver == XPBD_SOLVER_GS_COLORED;
+    const bool cluster = h->variant == XPBD_VARIANT_CLUSTER || h->variant == XPBD_VARIANT_CLUSTER_QUAD;
+    for (int it = 0; it < n; it++) {
+        if (gs) {
+            for (int c = 0; c < h->ncolors; c++) {
+                e = cluster ? launch_gs_local_cluster(s, ta, h->d_cstart, h->d_mult, h->d_pos, h->color_off[c],
+                                                      h->color_off[c + 1])
+                            : launch_gs_local_color(s, ta, h->d_mult, h->d_pos, h->color_off[c], h->color_off[c + 1]);
+                if (e != hipSuccess) return hip_err(h, e, "local gs launch");
+            }
+            e = launch_contacts_gs(s, h->d_faces, h->d_fcol_off, h->nfcolors, h->d_cpos, h->d_clist, h->d_rest,
+                                   h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, false);
+            if (e != hipSuccess) return hip_err(h, e, "local contacts_gs launch");
+        } else {
+            e = launch_jacobi_local(s, ta, h->d_mult, h->d_pos, h->d_acc, h->nt);
+            if (e == hipSuccess)
+                e = launch_contacts_jacobi(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
+                                           h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, false);
+            if (e == hipSuccess) e = launch_jacobi_apply(s, h->d_pos, h->d_acc, h->nv);
+            if (e != hipSuccess) return hip_err(h, e, "local jacobi launch");
+        }
     }
     return XPBD_OK;
 }
@@ -659,6 +740,7 @@ static int enqueue_step(xpbd_handle* h, bool detect, bool timed)
     if ((rc = enqueue_predict(h))) return rc;
     if ((rc = enqueue_iterations(h, h->params.iters, timed))) return rc;
     if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+    if ((rc = enqueue_local_collision(h))) return rc;
     return enqueue_finish(h);
 }
 
@@ -1057,6 +1139,38 @@ extern "C" int xpbd_batch_set_sdf(xpbd_batch* b, int32_t instance, int32_t kind,
     if (!b) return XPBD_E_INVALID_ARG;
     if (instance < 0 || instance >= b->ninst) return set_err(b, XPBD_E_INVALID_ARG, "instance out of range");
     return put_sdf(b, instance, kind, pose7, params3);
+}
+
+// Attachments (XPBDMeshObject_::addAttachmentConstraint / clearAttachmentConstraints,
+// XPBDMeshObject.cpp:255-275): replaces the whole list; n = 0 clears it.
+extern "C" int xpbd_set_attachments(xpbd_handle* h, int32_t n, const int32_t* vertices, const double* targets)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (n < 0 || (n > 0 && (!vertices || !targets))) return set_err(h, XPBD_E_INVALID_ARG, "attachments: bad arguments");
+    std::vector<uint8_t> seen(h->nv, 0);
+    for (int32_t a = 0; a < n; a++) {
+        if (vertices[a] < 0 || vertices[a] >= h->nv) return set_err(h, XPBD_E_INVALID_ARG, "attachment vertex out of range");
+        if (seen[vertices[a]]) return set_err(h, XPBD_E_INVALID_ARG, "a vertex may carry one attachment");
+        seen[vertices[a]] = 1;
+        for (int k = 0; k < 3; k++)
+            if (!std::isfinite(targets[3 * (size_t)a + k])) return set_err(h, XPBD_E_NONFINITE, "attachment target");
+    }
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_err(h, e, "attachments");
+    if (n > h->att_cap) {
+        // grow: the old buffers are released with the handle
+        if (dalloc(h, &h->d_att_v, n) || dalloc(h, &h->d_att_p, 3 * (size_t)n) || dalloc(h, &h->d_att_lam, n))
+            return XPBD_E_NOMEM;
+        h->att_cap = n;
+    }
+    if (n > 0) {
+        e = hipMemcpy(h->d_att_v, vertices, sizeof(int32_t) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->d_att_p, targets, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_err(h, e, "attachments upload");
+    }
+    if (n != h->natt) invalidate_graphs(h);  // launches capture the count
+    h->natt = n;
+    return XPBD_OK;
 }
 
 extern "C" int xpbd_set_sdf_velocity(xpbd_handle* h, const double* vel3)

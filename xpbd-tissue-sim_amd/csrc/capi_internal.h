@@ -85,6 +85,15 @@ struct xpbd_handle {
     double* d_cn = nullptr;
     double* d_cb = nullptr;
     double* d_clam = nullptr;
+    int natt = 0, att_cap = 0;   // attachments (vertex, FP64 target point, multiplier)
+    int32_t* d_att_v = nullptr;
+    double* d_att_p = nullptr;
+    double* d_att_lam = nullptr;
+    // local collision iterations: CSR vertex -> solver positions of its elements,
+    // per-element projection counts of a local pass
+    int32_t* d_vt_off = nullptr;
+    int32_t* d_vt_pos = nullptr;
+    int32_t* d_mult = nullptr;
     std::vector<void*> allocs;
 
     // stepping
@@ -118,6 +127,7 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed);
 int enqueue_detect(xpbd_handle* h);
 int enqueue_predict(xpbd_handle* h);
 int enqueue_finish(xpbd_handle* h);
+int enqueue_local_collision(xpbd_handle* h);
 bool detect_this_step(const xpbd_handle* h);
 int run_steps(xpbd_handle* h, int32_t nsteps);
 void timed_begin(xpbd_handle* h);

@@ -95,4 +95,25 @@ hipError_t launch_contacts_jacobi(hipStream_t s, const int4* faces, int nf, cons
                                   const double* rest, const float4* pos, const float4* lo, float* acc,
                                   const double* cp, const double* cn, const double* cb, double* clam, bool first);
 
+// Attachments: vertex av[a] pulled to the point ap[3a..3a+2] (alpha = 0).
+// Gauss-Seidel applies in place; Jacobi accumulates into acc (4 floats per vertex).
+hipError_t launch_attach_gs(hipStream_t s, const int* av, const double* ap, double* alam, const double* rest,
+                            float4* pos, const float4* lo, int n, bool first);
+hipError_t launch_attach_jacobi(hipStream_t s, const int* av, const double* ap, double* alam, const double* rest,
+                                const float4* pos, const float4* lo, float* acc, int n, bool first);
+
+// Local collision iterations.  vt_off / vt_pos: CSR of the solver positions of
+// the elements attached to each vertex.  mult (nt ints, zeroed by the caller)
+// receives, per solver position, how many times a local pass projects it.
+hipError_t launch_local_count(hipStream_t s, const int4* faces, int nf, const int* cpos, const int* clist,
+                              const double* clam, const int* vt_off, const int* vt_pos, int* mult);
+// Gauss-Seidel local pass over one colour: elements [begin, end) (one element
+// per thread) or clusters [cbeg, cend) (one cluster per thread).
+hipError_t launch_gs_local_color(hipStream_t s, const TetArrays& ta, const int* mult, float4* pos, int begin, int end);
+hipError_t launch_gs_local_cluster(hipStream_t s, const TetArrays& ta, const int* cstart, const int* mult, float4* pos,
+                                   int cbeg, int cend);
+// Jacobi local pass: corrections summed into acc (4 floats per vertex).
+hipError_t launch_jacobi_local(hipStream_t s, const TetArrays& ta, const int* mult, const float4* pos, float* acc,
+                               int nt);
+
 }  // namespace xpbd

@@ -994,6 +994,7 @@ extern "C" int xpbd_collect_training(xpbd_handle* h, int32_t nsteps, int32_t cap
         if (e != hipSuccess) return hip_err(h, e, "collect features");
         if ((rc = enqueue_iterations(h, h->params.iters, false))) return rc;
         if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+        if ((rc = enqueue_local_collision(h))) return rc;
         if ((rc = enqueue_store_cprev(h)) || (rc = enqueue_finish(h))) return rc;
         e = hipMemcpyAsync(corr.data(), L->d_cprev, sizeof(float4) * nv, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1068,6 +1069,7 @@ extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_resid
         L->fallbacks++;
     }
     if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+    if ((rc = enqueue_local_collision(h))) return rc;
     if ((rc = enqueue_store_cprev(h)) || (rc = enqueue_finish(h))) return rc;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_err(h, e, "learned step");

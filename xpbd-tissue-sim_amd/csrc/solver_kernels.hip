@@ -1107,9 +1107,231 @@ __global__ __launch_bounds__(kBlock) void k_contacts_jacobi(const int4* __restri
     clam[j] = lam;
 }
 
+// ---------------------------------------------------------------------------
+// Attachments (AttachmentConstraint.hpp:31-71 with ConstraintProjector.hpp:111-172,
+// alpha = 0): vertex v pulled to the point p, C = |x - p|, gradient
+// (x - p) / |x - p|; no correction below |x - p| = 1e-12, where the constraint
+// is met (the reference's 1e-12 substitute gradient, AttachmentConstraint.hpp:76-81,
+// would move the vertex by ~3e11 |x - p|).  FP64 on the FP32 state, like the
+// contacts.  Attached vertices are distinct, so one thread per attachment is
+// the sequential order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void attach_dx(const double* x, const double* p, float w, double* lam, double* dx)
+{
+    const double d[3] = { x[0] - p[0], x[1] - p[1], x[2] - p[2] };
+    const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (dist < 1e-12) {
+        dx[0] = dx[1] = dx[2] = 0.0;
+        return;
+    }
+    double g[3];
+    for (int k = 0; k < 3; k++) g[k] = d[k] / dist;
+    const double lhs = (double)w * (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    const double dlam = -dist / lhs;
+    *lam += dlam;
+    for (int k = 0; k < 3; k++) dx[k] = (double)w * g[k] * dlam;
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_attach_gs(const int* __restrict__ av, const double* __restrict__ ap,
+                                                     double* __restrict__ alam, const double* __restrict__ rest,
+                                                     float4* __restrict__ pos, const float4* __restrict__ lo, int n)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= n) return;
+    const int v = av[a];
+    double x[3], dx[3];
+    world_pos(rest, pos, lo, v, true, x);
+    double lam = FIRST ? 0.0 : alam[a];
+    attach_dx(x, ap + 3 * (size_t)a, pos[2 * (size_t)v].w, &lam, dx);
+    alam[a] = lam;
+    float4 d = pos[2 * (size_t)v + 1];
+    d.x += (float)dx[0];
+    d.y += (float)dx[1];
+    d.z += (float)dx[2];
+    pos[2 * (size_t)v + 1] = d;
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_attach_jacobi(const int* __restrict__ av, const double* __restrict__ ap,
+                                                         double* __restrict__ alam, const double* __restrict__ rest,
+                                                         const float4* __restrict__ pos, const float4* __restrict__ lo,
+                                                         float* __restrict__ acc, int n)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= n) return;
+    const int v = av[a];
+    double x[3], dx[3];
+    world_pos(rest, pos, lo, v, true, x);
+    double lam = FIRST ? 0.0 : alam[a];
+    attach_dx(x, ap + 3 * (size_t)a, pos[2 * (size_t)v].w, &lam, dx);
+    alam[a] = lam;
+    for (int k = 0; k < 3; k++) unsafeAtomicAdd(acc + 4 * (size_t)v + k, (float)dx[k]);
+}
+
+// ---------------------------------------------------------------------------
+// Local collision iterations (XPBDMeshObject.cpp:476-483, 764-825).  After the
+// global solve, mult[p] counts the (contact with non-zero multiplier, face
+// vertex) pairs whose vertex belongs to the element at solver position p; a
+// local pass projects every such element mult[p] times in a row (the
+// reference's list holds it that many times), in the global solve's order,
+// without resetting multipliers, then all contacts.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_local_count(const int4* __restrict__ faces, int nf,
+                                                       const int* __restrict__ cpos, const int* __restrict__ clist,
+                                                       const double* __restrict__ clam, const int* __restrict__ vt_off,
+                                                       const int* __restrict__ vt_pos, int* __restrict__ mult)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= cpos[nf]) return;
+    const int j = clist[k];
+    if (clam[j] == 0.0) return;
+    const int4 f = faces[j];
+    const int vid[3] = { f.x, f.y, f.z };
+    for (int i = 0; i < 3; i++)
+        for (int p = vt_off[vid[i]]; p < vt_off[vid[i] + 1]; p++) atomicAdd(mult + vt_pos[p], 1);
+}
+
+// Gauss-Seidel: project element e m times, each projection seeing the last one's
+// positions and multipliers, then write its four vertices back.
+__device__ __forceinline__ void project_repeated(const TetArrays& ta, int e, int m, float4* pos)
+{
+    const int4 t = ta.tet[e];
+    Gather g;
+    gather4(pos, t, g);
+    float q[9];
+    float2 at;
+    float gm;
+    load_tet(ta, e, q, at, gm);
+    const float2 lv = ta.lam[e];
+    float l[2] = { lv.x, lv.y };
+    for (int r = 0; r < m; r++) {
+        float dx[12];
+        project_tet_split<float>(g.b, g.d, g.w, q, at.x, at.y, gm, l, dx);
+#pragma unroll
+        for (int k = 0; k < 12; k++) g.d[k] += dx[k];
+    }
+    ta.lam[e] = make_float2(l[0], l[1]);
+    const int vid[4] = { t.x, t.y, t.z, t.w };
+#pragma unroll
+    for (int i = 0; i < 4; i++) pos[2 * (size_t)vid[i] + 1] = make_float4(g.d[3 * i], g.d[3 * i + 1], g.d[3 * i + 2], 0.f);
+}
+
+// one thread per element of a colour (elements of a colour share no vertex)
+__global__ __launch_bounds__(kBlock) void k_gs_local_color(TetArrays ta, const int* __restrict__ mult,
+                                                          float4* __restrict__ pos, int begin, int end)
+{
+    const int e = begin + blockIdx.x * kBlock + threadIdx.x;
+    if (e >= end) return;
+    const int m = mult[e];
+    if (m > 0) project_repeated(ta, e, m, pos);
+}
+
+// one thread per cluster of a colour, its elements in order (clusters of a
+// colour share no vertex; a thread re-reads what it wrote itself)
+__global__ __launch_bounds__(kBlock) void k_gs_local_cluster(TetArrays ta, const int* __restrict__ cstart,
+                                                            const int* __restrict__ mult, float4* __restrict__ pos,
+                                                            int cbeg, int cend)
+{
+    const int c = cbeg + blockIdx.x * kBlock + threadIdx.x;
+    if (c >= cend) return;
+    const int e1 = cstart[c + 1];
+    for (int e = cstart[c]; e < e1; e++) {
+        const int m = mult[e];
+        if (m > 0) project_repeated(ta, e, m, pos);
+    }
+}
+
+// Jacobi: the m projections of element e all read the pass's starting positions
+// (the multiplier advances between them); their sum goes into acc.
+__global__ __launch_bounds__(kBlock) void k_jacobi_local(TetArrays ta, const int* __restrict__ mult,
+                                                        const float4* __restrict__ pos, float* __restrict__ acc, int nt)
+{
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nt) return;
+    const int m = mult[e];
+    if (m == 0) return;
+    const int4 t = ta.tet[e];
+    Gather g;
+    gather4(pos, t, g);
+    float q[9];
+    float2 at;
+    float gm;
+    load_tet(ta, e, q, at, gm);
+    const float2 lv = ta.lam[e];
+    float l[2] = { lv.x, lv.y };
+    float sum[12] = {};
+    for (int r = 0; r < m; r++) {
+        float dx[12];
+        project_tet_split<float>(g.b, g.d, g.w, q, at.x, at.y, gm, l, dx);
+#pragma unroll
+        for (int k = 0; k < 12; k++) sum[k] += dx[k];
+    }
+    ta.lam[e] = make_float2(l[0], l[1]);
+    const int vid[4] = { t.x, t.y, t.z, t.w };
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) unsafeAtomicAdd(acc + 4 * (size_t)vid[i] + k, sum[3 * i + k]);
+}
+
 inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
+
+hipError_t launch_attach_gs(hipStream_t s, const int* av, const double* ap, double* alam, const double* rest,
+                            float4* pos, const float4* lo, int n, bool first)
+{
+    if (n <= 0) return hipSuccess;
+    if (first) hipLaunchKernelGGL(k_attach_gs<true>, dim3(nblk(n, kBlock)), dim3(kBlock), 0, s, av, ap, alam, rest, pos, lo, n);
+    else hipLaunchKernelGGL(k_attach_gs<false>, dim3(nblk(n, kBlock)), dim3(kBlock), 0, s, av, ap, alam, rest, pos, lo, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_attach_jacobi(hipStream_t s, const int* av, const double* ap, double* alam, const double* rest,
+                                const float4* pos, const float4* lo, float* acc, int n, bool first)
+{
+    if (n <= 0) return hipSuccess;
+    if (first)
+        hipLaunchKernelGGL(k_attach_jacobi<true>, dim3(nblk(n, kBlock)), dim3(kBlock), 0, s, av, ap, alam, rest, pos, lo,
+                           acc, n);
+    else
+        hipLaunchKernelGGL(k_attach_jacobi<false>, dim3(nblk(n, kBlock)), dim3(kBlock), 0, s, av, ap, alam, rest, pos,
+                           lo, acc, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_local_count(hipStream_t s, const int4* faces, int nf, const int* cpos, const int* clist,
+                              const double* clam, const int* vt_off, const int* vt_pos, int* mult)
+{
+    if (nf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_local_count, dim3(nblk(nf, kBlock)), dim3(kBlock), 0, s, faces, nf, cpos, clist, clam, vt_off,
+                       vt_pos, mult);
+    return hipGetLastError();
+}
+
+hipError_t launch_gs_local_color(hipStream_t s, const TetArrays& ta, const int* mult, float4* pos, int begin, int end)
+{
+    if (end <= begin) return hipSuccess;
+    hipLaunchKernelGGL(k_gs_local_color, dim3(nblk(end - begin, kBlock)), dim3(kBlock), 0, s, ta, mult, pos, begin, end);
+    return hipGetLastError();
+}
+
+hipError_t launch_gs_local_cluster(hipStream_t s, const TetArrays& ta, const int* cstart, const int* mult, float4* pos,
+                                   int cbeg, int cend)
+{
+    if (cend <= cbeg) return hipSuccess;
+    hipLaunchKernelGGL(k_gs_local_cluster, dim3(nblk(cend - cbeg, kBlock)), dim3(kBlock), 0, s, ta, cstart, mult, pos,
+                       cbeg, cend);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_local(hipStream_t s, const TetArrays& ta, const int* mult, const float4* pos, float* acc, int nt)
+{
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jacobi_local, dim3(nblk(nt, kBlock)), dim3(kBlock), 0, s, ta, mult, pos, acc, nt);
+    return hipGetLastError();
+}
 
 hipError_t launch_predict(hipStream_t s, float4* pos, const float4* vel, int nv, float dt, float dz)
 {

@@ -81,6 +81,8 @@ XPBDMeshObjectConfig XPBDMeshObjectConfig::from_node(const yaml_lite::Node& n)
     }
     opt(n, "collisions", c.collisions, B);
     if (const yaml_lite::Node* m = n.find("num-solver-iters")) c.num_solver_iters = (int)m->as_int();
+    if (const yaml_lite::Node* m = n.find("num-local-collision-iters"))
+        c.num_local_collision_iters = (int)m->as_int();
     if (const yaml_lite::Node* m = n.find("solver-type")) c.solver_type = solver_type_from_string(m->as_string());
     opt(n, "constraint-type", c.constraint_type, S);
     if (c.constraint_type != "Stable-Neohookean-Combined")
@@ -95,6 +97,8 @@ XPBDMeshObjectConfig XPBDMeshObjectConfig::from_node(const yaml_lite::Node& n)
     if (c.filename.empty() && !c.grid)
         throw Sim::SimError(XPBD_E_INVALID_ARG, "object '" + c.name + "' needs a filename or a grid");
     if (c.num_solver_iters < 0) throw Sim::SimError(XPBD_E_INVALID_ARG, "num-solver-iters < 0");
+    if (c.num_local_collision_iters < 0)
+        throw Sim::SimError(XPBD_E_INVALID_ARG, "num-local-collision-iters < 0");
     return c;
 }
 
@@ -191,6 +195,7 @@ void XPBDGPUSolver::setup()
     p.first_order = _obj->config().first_order ? 1 : 0;
     p.damping_multiplier = _obj->config().damping_multiplier;
     p.adjust_damping = _obj->config().adjust_damping_to_material ? 1 : 0;
+    p.local_collision_iters = _obj->config().collisions ? _obj->config().num_local_collision_iters : 0;
     const int rc = xpbd_create(&d, &p, &_h);
     if (rc != XPBD_OK) throw Sim::SimError(rc, std::string("xpbd_create: ") + xpbd_last_error(nullptr));
     if (_policy != XPBDSolverResidualPolicyEnum::NEVER) check(xpbd_set_residual_policy(_h, 1));
@@ -229,6 +234,12 @@ void XPBDGPUSolver::setCollider(int32_t kind, const Sim::Vec3r& pos, const std::
 }
 
 void XPBDGPUSolver::detectCollisions() { check(xpbd_detect_contacts(_h)); }
+
+void XPBDGPUSolver::setAttachments(const std::vector<int32_t>& vertices, const std::vector<double>& targets)
+{
+    if (targets.size() != 3 * vertices.size()) throw Sim::SimError(XPBD_E_INVALID_ARG, "attachments: 3 target values per vertex");
+    check(xpbd_set_attachments(_h, (int32_t)vertices.size(), vertices.data(), targets.data()));
+}
 
 std::vector<double> XPBDGPUSolver::positions() const
 {
@@ -391,11 +402,51 @@ void XPBDMeshObject::setup()
 
 void XPBDMeshObject::update()
 {
+    if (!_att.empty() || _att_sent) {
+        // targets follow the attached points, read once per step like the
+        // reference's projector reads *attach_pos_ptr
+        std::vector<int32_t> v(_att.size());
+        std::vector<double> t(3 * _att.size());
+        for (size_t a = 0; a < _att.size(); a++) {
+            v[a] = _att[a].v;
+            for (int k = 0; k < 3; k++) t[3 * a + k] = (*_att[a].pos)[k] + _att[a].offset[k];
+        }
+        _solver->setAttachments(v, t);
+        _att_sent = !_att.empty();
+    }
     _solver->solve();
     _x = _solver->positions();
 }
 
 void XPBDMeshObject::velocityUpdate() {}
+
+void XPBDMeshObject::addAttachmentConstraint(int v_ind, const Vec3r* attach_pos_ptr, const Vec3r& attachment_offset)
+{
+    if (v_ind < 0 || v_ind >= numVertices()) throw SimError(XPBD_E_INVALID_ARG, "attachment vertex out of range");
+    if (!attach_pos_ptr) throw SimError(XPBD_E_INVALID_ARG, "attachment position pointer is null");
+    for (const Attachment& a : _att)
+        if (a.v == v_ind) throw SimError(XPBD_E_INVALID_ARG, "a vertex may carry one attachment");
+    _att.push_back({ v_ind, attach_pos_ptr, attachment_offset });
+}
+
+void XPBDMeshObject::clearAttachmentConstraints() { _att.clear(); }
+
+bool XPBDMeshObject::vertexOnSurface(int index)
+{
+    if (_surface.empty()) {
+        std::vector<int32_t> f = _faces;
+        if (f.empty()) {
+            int32_t nf = 0;
+            f.resize(12 * (size_t)numElements());
+            const int rc = xpbd_mesh_surface(numVertices(), numElements(), _tets.data(), &nf, f.data());
+            if (rc != XPBD_OK) throw SimError(rc, "surface extraction failed");
+            f.resize(3 * (size_t)nf);
+        }
+        _surface.assign(numVertices(), 0);
+        for (int32_t v : f) _surface[v] = 1;
+    }
+    return _surface.at(index) != 0;
+}
 
 void XPBDMeshObject::fixVertex(int index)
 {
@@ -552,6 +603,49 @@ void BeamSimulation::setup()
         if (p->config().residual_policy != Config::XPBDSolverResidualPolicyEnum::NEVER)
             addOutput(p->name() + "_Constraint-Residual", [p]() { return p->lastConstraintResidualNorm(); });
     }
+}
+
+}  // namespace Sim
+
+namespace Sim {
+
+void GraspingSimulation::moveCursor(const Vec3r& dp)
+{
+    for (int k = 0; k < 3; k++) _cursor[k] += dp[k];
+}
+
+void GraspingSimulation::timeStep()
+{
+    const double dt = _cfg.time_step;
+    moveCursor({ _cursor_vel[0] * dt, _cursor_vel[1] * dt, _cursor_vel[2] * dt });
+    Simulation::timeStep();
+}
+
+void GraspingSimulation::toggleGrasping()
+{
+    // GraspingSimulation.cpp:177-318: a clean slate, then every non-fixed surface
+    // vertex within the radius, vertex order, capped per object
+    for (auto& o : _objects) o->clearAttachmentConstraints();
+    _grasped.clear();
+    if (!_grasping) {
+        for (size_t oi = 0; oi < _objects.size(); oi++) {
+            XPBDMeshObject& o = *_objects[oi];
+            int attached = 0;
+            for (int v = 0; v < o.numVertices(); v++) {
+                if (o.vertexFixed(v) || !o.vertexOnSurface(v)) continue;
+                const Vec3r x = o.vertex(v);
+                const double d = std::sqrt((x[0] - _cursor[0]) * (x[0] - _cursor[0]) +
+                                           (x[1] - _cursor[1]) * (x[1] - _cursor[1]) +
+                                           (x[2] - _cursor[2]) * (x[2] - _cursor[2]));
+                if (d <= _grasp_radius && attached < kMaxAttachPerObject) {
+                    o.addAttachmentConstraint(v, &_cursor, { x[0] - _cursor[0], x[1] - _cursor[1], x[2] - _cursor[2] });
+                    _grasped.emplace_back((int)oi, v);
+                    attached++;
+                }
+            }
+        }
+    }
+    _grasping = !_grasping;
 }
 
 }  // namespace Sim

@@ -67,6 +67,7 @@ struct XPBDMeshObjectConfig {
     ElasticMaterialConfig material;
     bool collisions = false;
     int num_solver_iters = 1;
+    int num_local_collision_iters = 0;  // num-local-collision-iters (XPBDMeshObjectConfig.hpp:68)
     XPBDObjectSolverTypeEnum solver_type = XPBDObjectSolverTypeEnum::GAUSS_SEIDEL;
     std::string constraint_type = "Stable-Neohookean-Combined";
     XPBDSolverResidualPolicyEnum residual_policy = XPBDSolverResidualPolicyEnum::NEVER;
@@ -126,6 +127,8 @@ public:
     void setCollider(int32_t kind, const Sim::Vec3r& pos, const std::array<double, 4>& quat,
                      const Sim::Vec3r& params, const Sim::Vec3r& vel);
     void detectCollisions();
+    // replaces the device attachment list: vertices[a] pulled to targets[3a..3a+2]
+    void setAttachments(const std::vector<int32_t>& vertices, const std::vector<double>& targets);
     std::vector<double> positions() const;
     std::vector<double> velocities() const;
     std::vector<double> masses() const;
@@ -165,6 +168,14 @@ public:
     double lastConstraintResidualNorm() const;
     double totalStrainEnergy() const;
     AABB boundingBox() const;
+    // Attachment (grasp) constraints, XPBDMeshObject.cpp:254-275: vertex v_ind is
+    // pulled to *attach_pos_ptr + attachment_offset, where *attach_pos_ptr is read
+    // at every update (the caller keeps it alive and moves it, e.g. a cursor).
+    void addAttachmentConstraint(int v_ind, const Vec3r* attach_pos_ptr, const Vec3r& attachment_offset);
+    void clearAttachmentConstraints();
+    int numAttachmentConstraints() const { return (int)_att.size(); }
+    // vertex of a boundary face of the tetrahedral mesh (Mesh.cpp:16-25)
+    bool vertexOnSurface(int index);
     std::vector<int> getVerticesWithAxisValue(int axis, double value, double tol = 1e-9) const;
     std::vector<int> getVerticesWithY(double y) const { return getVerticesWithAxisValue(1, y); }
     int getClosestVertex(const Vec3r& p) const;
@@ -187,6 +198,14 @@ private:
     std::vector<double> _x0, _x, _v0, _E, _nu, _rho, _masses;
     std::vector<int32_t> _tets, _faces;
     std::vector<uint8_t> _fixed;
+    struct Attachment {
+        int v;
+        const Vec3r* pos;
+        Vec3r offset;
+    };
+    std::vector<Attachment> _att;
+    bool _att_sent = false;  // the device holds a non-empty list
+    std::vector<uint8_t> _surface;  // surface vertex flags, computed on first use
     std::unique_ptr<Solver::XPBDGPUSolver> _solver;
     friend class Solver::XPBDGPUSolver;
 };
@@ -197,7 +216,7 @@ public:
     virtual ~Simulation() = default;
     virtual void setup();
     void run(int64_t max_steps = -1);  // as fast as possible, until end-time or max_steps
-    void timeStep();
+    virtual void timeStep();
     double time() const { return _time; }
     int64_t steps() const { return _steps; }
     std::vector<std::unique_ptr<XPBDMeshObject>>& objects() { return _objects; }
@@ -224,6 +243,36 @@ public:
 
 protected:
     void configureObject(XPBDMeshObject& obj) override;
+};
+
+// Grasping without the input devices (GraspingSimulation.cpp:26-60, 150-320): a
+// cursor point and a grasp radius.  toggleGrasping() attaches every non-fixed
+// surface vertex within the radius of the cursor (at most 256 per object) with
+// its offset from the cursor, so the attachments start satisfied, or releases
+// them; moveCursor(dp) moves the point the attachments follow.  The cursor
+// moves by cursorVelocity() * dt at the start of every step.
+class GraspingSimulation : public Simulation {
+public:
+    using Simulation::Simulation;
+    static constexpr int kMaxAttachPerObject = 256;
+    void timeStep() override;
+    void setCursor(const Vec3r& p) { _cursor = p; }
+    const Vec3r& cursorPosition() const { return _cursor; }
+    void setCursorVelocity(const Vec3r& v) { _cursor_vel = v; }
+    const Vec3r& cursorVelocity() const { return _cursor_vel; }
+    void setGraspRadius(double r) { _grasp_radius = r; }
+    double graspRadius() const { return _grasp_radius; }
+    void moveCursor(const Vec3r& dp);
+    void toggleGrasping();
+    bool grasping() const { return _grasping; }
+    // (object index, vertex) of every attachment made by the last grasp
+    const std::vector<std::pair<int, int>>& graspedVertices() const { return _grasped; }
+
+private:
+    Vec3r _cursor{ 0, 0, 0 }, _cursor_vel{ 0, 0, 0 };
+    double _grasp_radius = 0.1;  // grasp-radius default (GraspingSimulationConfig.hpp:25)
+    bool _grasping = false;
+    std::vector<std::pair<int, int>> _grasped;
 };
 
 }  // namespace Sim

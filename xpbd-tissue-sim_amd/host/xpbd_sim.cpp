@@ -3,6 +3,7 @@
 //
 //   xpbd_sim CONFIG.yaml [--beam] [--steps N] [--dump-positions FILE]
 //            [--dump-order FILE] [--dump-face-rank FILE] [--dump-rest FILE] [--print-config]
+//            [--grasp R [--cursor X,Y,Z] [--cursor-velocity VX,VY,VZ] [--dump-attachments FILE]]
 //
 // --beam            BeamSimulation set-up (fix the minimum-Y side, log tip
 //                   deflection and constraint residual)
@@ -19,6 +20,13 @@
 // --copy-calibration MB
 //                   first copy MB mebibytes device-to-device once: a transfer of
 //                   known size to scale hardware byte counters against
+// --grasp R         GraspingSimulation: after set-up, grasp every non-fixed surface
+//                   vertex within R of the cursor
+// --cursor X,Y,Z    cursor start (default: centre of object 0's top bounding-box face)
+// --cursor-velocity VX,VY,VZ
+//                   cursor motion per second of simulated time while grasping
+// --dump-attachments
+//                   object 0's grasped vertex ids, raw int32
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -45,10 +53,17 @@ int main(int argc, char** argv)
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s CONFIG.yaml [--beam] [--steps N] [--dump-positions FILE] "
                              "[--dump-order FILE] [--dump-face-rank FILE] [--dump-rest FILE] [--eager] [--copy-calibration MB] "
-                             "[--print-config]\n", argv[0]);
+                             "[--print-config] [--grasp R] [--cursor X,Y,Z] [--cursor-velocity VX,VY,VZ] "
+                             "[--dump-attachments FILE]\n", argv[0]);
         return 2;
     }
-    std::string cfg_path = argv[1], dump_pos, dump_order, dump_rank, dump_rest;
+    std::string cfg_path = argv[1], dump_pos, dump_order, dump_rank, dump_rest, dump_att;
+    double grasp_radius = -1;
+    bool have_cursor = false;
+    Sim::Vec3r cursor{ 0, 0, 0 }, cursor_vel{ 0, 0, 0 };
+    auto triple = [](const char* t, Sim::Vec3r& out) {
+        return std::sscanf(t, "%lf,%lf,%lf", &out[0], &out[1], &out[2]) == 3;
+    };
     bool beam = false, print_only = false, load_meshes = false, eager = false;
     double copy_mb = 0;
     std::string solver_override;
@@ -66,6 +81,10 @@ int main(int argc, char** argv)
         else if (a == "--dump-order" && i + 1 < argc) dump_order = argv[++i];
         else if (a == "--dump-face-rank" && i + 1 < argc) dump_rank = argv[++i];
         else if (a == "--dump-rest" && i + 1 < argc) dump_rest = argv[++i];
+        else if (a == "--dump-attachments" && i + 1 < argc) dump_att = argv[++i];
+        else if (a == "--grasp" && i + 1 < argc) grasp_radius = std::atof(argv[++i]);
+        else if (a == "--cursor" && i + 1 < argc && triple(argv[i + 1], cursor)) have_cursor = true, i++;
+        else if (a == "--cursor-velocity" && i + 1 < argc && triple(argv[i + 1], cursor_vel)) i++;
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -81,11 +100,12 @@ int main(int argc, char** argv)
                         cfg.name.c_str(), cfg.time_step, cfg.end_time, cfg.g_accel, cfg.collision_rate);
             for (size_t i = 0; i < cfg.xpbd_objects.size(); i++) {
                 const auto& o = cfg.xpbd_objects[i];
-                std::printf("%s{\"name\": \"%s\", \"solver_type\": \"%s\", \"iters\": %d, \"E\": %.9g, \"nu\": %.9g, "
+                std::printf("%s{\"name\": \"%s\", \"solver_type\": \"%s\", \"iters\": %d, \"local_collision_iters\": %d, "
+                        "\"E\": %.9g, \"nu\": %.9g, "
                             "\"density\": %.9g, \"collisions\": %s, \"source\": \"%s\", \"position\": [%.9g, %.9g, %.9g], "
                             "\"first_order\": %s, \"damping_multiplier\": %.9g, \"adjust_damping\": %s",
                             i ? ", " : "", o.name.c_str(), solver_name(o.solver_type), o.num_solver_iters,
-                            o.material.E, o.material.nu, o.material.density, o.collisions ? "true" : "false",
+                            o.num_local_collision_iters, o.material.E, o.material.nu, o.material.density, o.collisions ? "true" : "false",
                             o.grid ? "grid" : o.filename.c_str(), o.position[0], o.position[1], o.position[2],
                             o.first_order ? "true" : "false", o.damping_multiplier,
                             o.adjust_damping_to_material ? "true" : "false");
@@ -104,8 +124,15 @@ int main(int argc, char** argv)
             std::printf("], \"rigid_objects\": %zu}\n", cfg.rigid_objects.size());
             return 0;
         }
-        std::unique_ptr<Sim::Simulation> sim =
-            beam ? std::make_unique<Sim::BeamSimulation>(cfg) : std::make_unique<Sim::Simulation>(cfg);
+        std::unique_ptr<Sim::Simulation> sim;
+        Sim::GraspingSimulation* grasp = nullptr;
+        if (beam) sim = std::make_unique<Sim::BeamSimulation>(cfg);
+        else if (grasp_radius > 0) {
+            auto g = std::make_unique<Sim::GraspingSimulation>(cfg);
+            grasp = g.get();
+            sim = std::move(g);
+        } else
+            sim = std::make_unique<Sim::Simulation>(cfg);
         if (copy_mb > 0) {
             const size_t bytes = (size_t)(copy_mb * 1048576.0);
             void *a = nullptr, *b = nullptr;
@@ -118,6 +145,17 @@ int main(int argc, char** argv)
             std::fprintf(stderr, "calibration copy: %zu bytes device to device\n", bytes);
         }
         sim->setup();
+        if (grasp) {
+            if (!have_cursor && !sim->objects().empty()) {
+                const Sim::AABB b = sim->objects()[0]->boundingBox();
+                cursor = { (b.min[0] + b.max[0]) / 2, (b.min[1] + b.max[1]) / 2, b.max[2] };
+            }
+            grasp->setGraspRadius(grasp_radius);
+            grasp->setCursor(cursor);
+            grasp->setCursorVelocity(cursor_vel);
+            grasp->toggleGrasping();
+            std::fprintf(stderr, "grasped %zu vertices\n", grasp->graspedVertices().size());
+        }
         if (eager)
             for (auto& o : sim->objects()) {
                 const int rc = xpbd_set_kernel_timing(o->solver().handle(), 1);
@@ -131,6 +169,14 @@ int main(int argc, char** argv)
             if (!dump_pos.empty()) {
                 std::ofstream f(dump_pos, std::ios::binary);
                 f.write(reinterpret_cast<const char*>(o.vertices().data()), sizeof(double) * o.vertices().size());
+            }
+            if (!dump_att.empty()) {
+                std::vector<int32_t> v;
+                if (grasp)
+                    for (const auto& p : grasp->graspedVertices())
+                        if (p.first == 0) v.push_back(p.second);
+                std::ofstream f(dump_att, std::ios::binary);
+                f.write(reinterpret_cast<const char*>(v.data()), sizeof(int32_t) * v.size());
             }
             if (!dump_rest.empty()) {
                 std::ofstream f(dump_rest, std::ios::binary);

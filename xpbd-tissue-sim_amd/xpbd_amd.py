@@ -54,7 +54,7 @@ class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("g", C.c_double), ("iters", C.c_int32), ("solver", C.c_int32),
                 ("deterministic", C.c_int32), ("collision_every", C.c_int32), ("variant", C.c_int32),
                 ("first_order", C.c_int32), ("damping_multiplier", C.c_double), ("adjust_damping", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("local_collision_iters", C.c_int32)]
 
 
 def _sig(name, args, res=C.c_int):
@@ -95,6 +95,7 @@ _sig("xpbd_batch_set_sdf", [_vp, C.c_int32, C.c_int32, _dp, _dp])
 _sig("xpbd_batch_get_positions", [_vp, C.c_int32, _dp])
 _sig("xpbd_autotune", [_vp, C.c_char_p])
 _sig("xpbd_get_variant", [_vp, _ip, _ip])
+_sig("xpbd_set_attachments", [_vp, C.c_int32, _ip, _dp])
 _sig("xpbd_mesh_grid", [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, _dp, _dp, _ip])
 _sig("xpbd_mesh_surface", [C.c_int32, C.c_int32, _ip, _ip, _ip])
 _sig("xpbd_color_items", [C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _ip])
@@ -229,13 +230,15 @@ VARIANTS = {"auto": 0, "simple": 1, "cluster": 2, "patch": 3, "quad": 4}
 
 
 def make_params(dt=1e-3, g=9.81, iters=10, solver="gs", deterministic=1, collision_every=1,
-                variant="auto", first_order=False, damping_multiplier=1.0, adjust_damping=False) -> Params:
+                variant="auto", first_order=False, damping_multiplier=1.0, adjust_damping=False,
+                local_collision_iters=0) -> Params:
     p = Params()
     p.dt, p.g, p.iters = dt, g, iters
     p.solver = SOLVERS[solver] if isinstance(solver, str) else solver
     p.deterministic, p.collision_every = deterministic, collision_every
     p.variant = VARIANTS[variant] if isinstance(variant, str) else variant
     p.first_order, p.damping_multiplier, p.adjust_damping = int(first_order), damping_multiplier, int(adjust_damping)
+    p.local_collision_iters = int(local_collision_iters)
     return p
 
 
@@ -243,11 +246,12 @@ class Solver:
     """A device solver for one object (or a batch of objects)."""
 
     def __init__(self, mesh, dt=1e-3, g=9.81, iters=10, solver="gs", deterministic=1, collision_every=1,
-                 variant="auto", first_order=False, damping_multiplier=1.0, adjust_damping=False):
+                 variant="auto", first_order=False, damping_multiplier=1.0, adjust_damping=False,
+                 local_collision_iters=0):
         meshes = mesh if isinstance(mesh, (list, tuple)) else [mesh]
         self.meshes = list(meshes)
         self.params = make_params(dt, g, iters, solver, deterministic, collision_every, variant, first_order,
-                                  damping_multiplier, adjust_damping)
+                                  damping_multiplier, adjust_damping, local_collision_iters)
         descs = (MeshDesc * len(meshes))(*[m.desc() for m in meshes])
         h = _vp()
         if len(meshes) == 1 and not isinstance(mesh, (list, tuple)):
@@ -354,6 +358,12 @@ class Solver:
             self._c(lib.xpbd_set_sdf(self.h, kind, ptr(pose, C.c_double), ptr(prm, C.c_double)))
         else:
             self._c(lib.xpbd_batch_set_sdf(self.h, instance, kind, ptr(pose, C.c_double), ptr(prm, C.c_double)))
+
+    def set_attachments(self, vertices, targets):
+        """Pull vertices[i] to the world points targets[i] (replaces the list; empty clears)."""
+        v = np.ascontiguousarray(vertices, np.int32).reshape(-1)
+        t = np.ascontiguousarray(targets, np.float64).reshape(-1)
+        self._c(lib.xpbd_set_attachments(self.h, len(v), ptr(v, C.c_int32), ptr(t, C.c_double)))
 
     def set_sdf_velocity(self, vel):
         v = np.asarray(vel, np.float64)
