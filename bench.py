@@ -218,7 +218,8 @@ def main():
     s, meshes, cfg, sdf = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
     if a.variant == "auto":
         # time the variants that share this handle's layout and keep the fastest
-        cfg["autotuned"] = list(s.autotune())
+        s.autotune()
+        cfg["autotuned"] = list(s.launch_config())
     cfg["variant"] = a.variant
     cfg["deterministic"] = not a.nondeterministic
     cfg["colors"] = s.num_colors()

@@ -5,8 +5,10 @@
 // Gauss-Seidel handles built with clusters can switch between the one-thread
 // cluster kernel and the quad kernel at 64, 128 or 256 threads per workgroup
 // without rebuilding anything (same clusters, colours and order, so the same
-// results up to FP32 summation order within an element).  Other handles have
-// a single candidate; the call then only records it.
+// results up to FP32 summation order within an element).  Cluster and patch
+// kernels are also timed with and without the XCD-contiguous workgroup mapping
+// (which does not change results).  Other handles have a single candidate; the
+// call then only records it.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -25,13 +27,14 @@ namespace {
 struct Candidate {
     int variant;
     int block;
+    int xcd;
 };
 
 std::string shape_key(const xpbd_handle* h)
 {
     std::ostringstream k;
     k << (h->params.solver == XPBD_SOLVER_GS_COLORED ? "gs" : "jacobi") << ":v" << h->nv << ":t" << h->nt << ":c"
-      << h->ncolors << ":k" << h->nclusters << ":i" << h->ninst;
+      << h->ncolors << ":k" << h->nclusters << ":i" << h->ninst << ":x";
     return k.str();
 }
 
@@ -44,9 +47,9 @@ bool read_cache(const char* path, const std::string& key, Candidate* out)
     while (std::getline(f, line)) {
         std::istringstream is(line);
         std::string k;
-        int v = 0, b = 0;
-        if (is >> k >> v >> b && k == key) {
-            *out = { v, b };
+        int v = 0, b = 0, x = 0;
+        if (is >> k >> v >> b >> x && k == key) {
+            *out = { v, b, x };
             found = true;  // the last entry for a key wins
         }
     }
@@ -57,7 +60,7 @@ void write_cache(const char* path, const std::string& key, const Candidate& c, d
 {
     if (!path) return;
     std::ofstream f(path, std::ios::app);
-    f << key << " " << c.variant << " " << c.block << " " << ms_per_iter << "\n";
+    f << key << " " << c.variant << " " << c.block << " " << c.xcd << " " << ms_per_iter << "\n";
 }
 
 bool valid(const xpbd_handle* h, const Candidate& c)
@@ -69,13 +72,16 @@ bool valid(const xpbd_handle* h, const Candidate& c)
 
 void apply(xpbd_handle* h, const Candidate& c)
 {
-    if (h->variant != c.variant || (c.variant == XPBD_VARIANT_CLUSTER_QUAD && h->quad_block != c.block)) {
+    if (h->variant != c.variant || (c.variant == XPBD_VARIANT_CLUSTER_QUAD && h->quad_block != c.block) ||
+        h->xcd_map != (c.xcd != 0)) {
         h->variant = c.variant;
         if (c.variant == XPBD_VARIANT_CLUSTER_QUAD) h->quad_block = c.block;
+        h->xcd_map = c.xcd != 0;
         invalidate_graphs(h);
     }
     char buf[64];
-    std::snprintf(buf, sizeof buf, "variant %d block %d", c.variant, c.variant == XPBD_VARIANT_CLUSTER_QUAD ? c.block : 0);
+    std::snprintf(buf, sizeof buf, "variant %d block %d xcd %d", c.variant,
+                  c.variant == XPBD_VARIANT_CLUSTER_QUAD ? c.block : 0, c.xcd);
     h->tuned = buf;
 }
 
@@ -91,11 +97,17 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
         return XPBD_OK;
     }
     std::vector<Candidate> cand;
-    if (h->params.solver == XPBD_SOLVER_GS_COLORED && h->nclusters > 0)
-        cand = { { XPBD_VARIANT_CLUSTER, 0 }, { XPBD_VARIANT_CLUSTER_QUAD, 64 }, { XPBD_VARIANT_CLUSTER_QUAD, 128 },
-                 { XPBD_VARIANT_CLUSTER_QUAD, 256 } };
-    else
-        cand = { { h->variant, h->quad_block } };
+    if (h->params.solver == XPBD_SOLVER_GS_COLORED && h->nclusters > 0) {
+        for (int x = 1; x >= 0; x--)
+            for (const Candidate& b : { Candidate{ XPBD_VARIANT_CLUSTER, 0, x }, Candidate{ XPBD_VARIANT_CLUSTER_QUAD, 64, x },
+                                        Candidate{ XPBD_VARIANT_CLUSTER_QUAD, 128, x },
+                                        Candidate{ XPBD_VARIANT_CLUSTER_QUAD, 256, x } })
+                cand.push_back(b);
+    } else if (h->variant == XPBD_VARIANT_PATCH) {
+        cand = { { h->variant, 0, 1 }, { h->variant, 0, 0 } };
+    } else {
+        cand = { { h->variant, h->quad_block, h->xcd_map ? 1 : 0 } };
+    }
     if (cand.size() == 1) {
         apply(h, cand[0]);
         write_cache(cache_path, key, cand[0], 0.0);
@@ -114,6 +126,7 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
     if (e == hipSuccess) e = hipMemcpyAsync(save_lam, h->d_lam, lam_bytes, hipMemcpyDeviceToDevice, s);
     int rc = e == hipSuccess ? XPBD_OK : hip_err(h, e, "autotune set-up");
     const int saved_variant = h->variant, saved_block = h->quad_block;
+    const bool saved_xcd = h->xcd_map;
     double best_ms = 1e300;
     Candidate best = cand[0];
     const int iters = h->params.iters > 0 ? h->params.iters : 1;
@@ -121,6 +134,7 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
         if (rc) break;
         h->variant = cd.variant;
         if (cd.variant == XPBD_VARIANT_CLUSTER_QUAD) h->quad_block = cd.block;
+        h->xcd_map = cd.xcd != 0;
         double ms_best = 1e300;
         for (int rep = 0; rep < 4 && !rc; rep++) {  // first repetition warms up
             (void)hipEventRecord(e0, s);
@@ -150,16 +164,18 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
     if (save_lam) (void)hipFree(save_lam);
     h->variant = saved_variant;
     h->quad_block = saved_block;
+    h->xcd_map = saved_xcd;
     if (rc) return rc;
     apply(h, best);
     write_cache(cache_path, key, best, best_ms / iters);
     return XPBD_OK;
 }
 
-extern "C" int xpbd_get_variant(xpbd_handle* h, int32_t* variant, int32_t* block)
+extern "C" int xpbd_get_variant(xpbd_handle* h, int32_t* variant, int32_t* block, int32_t* xcd)
 {
     if (!h || !variant) return XPBD_E_INVALID_ARG;
     *variant = h->variant;
     if (block) *block = h->variant == XPBD_VARIANT_CLUSTER_QUAD ? h->quad_block : 0;
+    if (xcd) *xcd = h->xcd_map ? 1 : 0;
     return XPBD_OK;
 }

@@ -605,10 +605,11 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
             for (int c = 0; c < h->ncolors; c++) {
                 if (h->variant == XPBD_VARIANT_CLUSTER)
                     e = launch_gs_cluster(s, ta, h->d_cverts, h->d_cstart, h->d_tloc, h->d_pos, h->color_off[c],
-                                          h->color_off[c + 1], first);
+                                          h->color_off[c + 1], first, h->xcd_map);
                 else if (h->variant == XPBD_VARIANT_CLUSTER_QUAD)
                     e = launch_gs_cluster_quad(s, ta, h->d_cverts, h->d_cstart, h->d_tloc, h->d_pos,
-                                               h->color_off[c], h->color_off[c + 1], first, h->quad_block);
+                                               h->color_off[c], h->color_off[c + 1], first, h->quad_block,
+                                               h->xcd_map);
                 else
                     e = launch_gs_color(s, ta, h->d_pos, h->color_off[c], h->color_off[c + 1], first);
                 if (e != hipSuccess) return hip_err(h, e, "gs_color launch");
@@ -639,7 +640,7 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
             }
             if (timed) timed_begin(h);
             e = launch_jacobi_patch(s, ta, patch_arrays(h), h->d_pos, reinterpret_cast<float4*>(h->d_acc), h->nt,
-                                    first, h->params.deterministic != 0, con || h->natt > 0);
+                                    first, h->params.deterministic != 0, con || h->natt > 0, h->xcd_map);
             if (e != hipSuccess) return hip_err(h, e, "jacobi_patch launch");
             if (timed) timed_end(h);
         } else {

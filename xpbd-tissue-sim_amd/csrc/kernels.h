@@ -38,13 +38,15 @@ hipError_t launch_finish(hipStream_t s, float4* pos, float4* lo, float4* vel, in
 hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int begin, int end, bool first);
 // Cluster Gauss-Seidel over clusters [cbeg, cend) of one colour: cverts holds 8
 // vertex ids per cluster (-1 padded), cstart the tet range of each cluster,
-// tloc the 4-bit local slots of each tet's vertices.
+// tloc the 4-bit local slots of each tet's vertices.  xcd: give each XCD one
+// contiguous eighth of the clusters (see xcd_item).
 hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                             const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first);
+                             const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, bool xcd);
 // Same clusters and order, four lanes per cluster (one per vertex of the element
 // being projected); block = threads per workgroup (64, 128 or 256).
 hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                                  const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block);
+                                  const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block,
+                                  bool xcd);
 bool quad_block_supported(int block);
 hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float4* pos, float* acc, int nt,
                                  bool first);
@@ -68,7 +70,7 @@ size_t jacobi_patch_smem(int maxl, bool det);
 // One Jacobi iteration: patch kernel + shared-vertex kernel.  cacc holds contact
 // corrections (float4 per vertex) when con is set; it is consumed and zeroed.
 hipError_t launch_jacobi_patch(hipStream_t s, const TetArrays& ta, const PatchArrays& pa, float4* pos, float4* cacc,
-                               int nt, bool first, bool det, bool con);
+                               int nt, bool first, bool det, bool con, bool xcd);
 hipError_t launch_fill_zero(hipStream_t s, void* p, size_t bytes);
 
 // FP64 residual r = C + alpha_tilde * lambda per element, block partial sums of

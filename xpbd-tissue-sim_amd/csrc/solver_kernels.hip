@@ -42,6 +42,20 @@ __device__ __forceinline__ void gather4(const float4* __restrict__ pos, const in
     }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8), and each XCD has its own L2.  With remap set, workgroup b takes
+// the work item that gives every XCD one contiguous eighth of the G items:
+// neighbouring clusters or patches (adjacent in solver order) then share an L2,
+// and since every colour's clusters follow the same spatial order, a vertex
+// written in one colour launch is read in the next by the same XCD.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item(int b, int G, bool remap)
+{
+    if (!remap) return b;
+    const int q = G / kXcds, r = G % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 __device__ __forceinline__ void load_tet(const TetArrays& ta, int e, float* q, float2& at, float& g)
 {
 #pragma unroll
@@ -159,7 +173,7 @@ template <bool FIRST>
 __global__ __launch_bounds__(kClusterBlock) void k_gs_cluster(TetArrays ta, const int4* __restrict__ cverts,
                                                              const int* __restrict__ cstart,
                                                              const uint32_t* __restrict__ tloc,
-                                                             float4* __restrict__ pos, int cbeg, int cend)
+                                                             float4* __restrict__ pos, int cbeg, int cend, int xcd)
 {
     __shared__ float sb[8][4][kClusterBlock];   // base xyz, inverse mass (thread-private columns)
     __shared__ float sd[8][3][kClusterBlock];   // in-step motion
@@ -167,7 +181,7 @@ __global__ __launch_bounds__(kClusterBlock) void k_gs_cluster(TetArrays ta, cons
     __shared__ float sl[2][kClusterStage];      // lambda
     __shared__ uint32_t stl[kClusterStage];     // local vertex slots
     const int tid = threadIdx.x;
-    const int cb = cbeg + blockIdx.x * kClusterBlock;
+    const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * kClusterBlock;
     const int cl_end = min(cend, cb + kClusterBlock);
     // the block's clusters are consecutive, so their tets form one contiguous
     // range: stage its element data with coalesced loads
@@ -284,7 +298,7 @@ template <bool FIRST, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const int4* __restrict__ cverts,
                                                           const int* __restrict__ cstart,
                                                           const uint32_t* __restrict__ tloc,
-                                                          float4* __restrict__ pos, int cbeg, int cend)
+                                                          float4* __restrict__ pos, int cbeg, int cend, int xcd)
 {
     constexpr int kQuadBlock = BLOCK;
     constexpr int kQuadClusters = BLOCK / 4;
@@ -297,7 +311,7 @@ __global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const i
     __shared__ uint32_t stl[kQuadStage];     // local vertex slots
     const int tid = threadIdx.x;
     const int cl = tid >> 2, j = tid & 3;
-    const int cb = cbeg + blockIdx.x * kQuadClusters;
+    const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * kQuadClusters;
     const int cl_end = min(cend, cb + kQuadClusters);
     const int c = cb + cl;
     const bool active = c < cend;
@@ -528,14 +542,14 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
                                                         const int* __restrict__ pslot, const int* __restrict__ inc_off,
                                                         const uint16_t* __restrict__ inc, float4* __restrict__ pos,
                                                         float4* __restrict__ partial, float4* __restrict__ cacc,
-                                                        int nt, int maxl)
+                                                        int nt, int maxl, int xcd)
 {
     extern __shared__ float4 smem[];
     float4* lb = smem;                                   // base, inverse mass
     float* ld = reinterpret_cast<float*>(smem + maxl);   // in-step motion, 3 per local vertex
     float* red = ld + 3 * maxl;                          // DET: 12 planes of kPatch; else 3 per local vertex
     const int tid = threadIdx.x;
-    const int p = blockIdx.x;
+    const int p = xcd_item(blockIdx.x, gridDim.x, xcd != 0);
     const int v0 = pvoff[p], nl = pvoff[p + 1] - v0;
     // element data first, so its latency overlaps the vertex staging
     const int e = p * kPatch + tid;
@@ -1364,39 +1378,42 @@ hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int 
 }
 
 hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                             const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first)
+                             const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, bool xcd)
 {
     if (cend <= cbeg) return hipSuccess;
     const dim3 g(nblk(cend - cbeg, kClusterBlock));
     if (first)
-        hipLaunchKernelGGL(k_gs_cluster<true>, g, dim3(kClusterBlock), 0, s, ta, cverts, cstart, tloc, pos, cbeg, cend);
+        hipLaunchKernelGGL(k_gs_cluster<true>, g, dim3(kClusterBlock), 0, s, ta, cverts, cstart, tloc, pos, cbeg, cend,
+                           (int)xcd);
     else
-        hipLaunchKernelGGL(k_gs_cluster<false>, g, dim3(kClusterBlock), 0, s, ta, cverts, cstart, tloc, pos, cbeg, cend);
+        hipLaunchKernelGGL(k_gs_cluster<false>, g, dim3(kClusterBlock), 0, s, ta, cverts, cstart, tloc, pos, cbeg, cend,
+                           (int)xcd);
     return hipGetLastError();
 }
 
 template <int BLOCK>
 static void launch_quad_shape(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                              const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first)
+                              const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, bool xcd)
 {
     const dim3 g(nblk(cend - cbeg, BLOCK / 4));
     if (first)
         hipLaunchKernelGGL((k_gs_cluster_quad<true, BLOCK>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc, pos,
-                           cbeg, cend);
+                           cbeg, cend, (int)xcd);
     else
         hipLaunchKernelGGL((k_gs_cluster_quad<false, BLOCK>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc, pos,
-                           cbeg, cend);
+                           cbeg, cend, (int)xcd);
 }
 
 bool quad_block_supported(int block) { return block == 64 || block == 128 || block == 256; }
 
 hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                                  const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block)
+                                  const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block,
+                                  bool xcd)
 {
     if (cend <= cbeg) return hipSuccess;
-    if (block == 64) launch_quad_shape<64>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first);
-    else if (block == 128) launch_quad_shape<128>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first);
-    else if (block == 256) launch_quad_shape<256>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first);
+    if (block == 64) launch_quad_shape<64>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
+    else if (block == 128) launch_quad_shape<128>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
+    else if (block == 256) launch_quad_shape<256>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -1419,13 +1436,13 @@ size_t jacobi_patch_smem(int maxl, bool det)
 int jacobi_patch_size() { return kPatch; }
 
 hipError_t launch_jacobi_patch(hipStream_t s, const TetArrays& ta, const PatchArrays& pa, float4* pos, float4* cacc,
-                               int nt, bool first, bool det, bool con)
+                               int nt, bool first, bool det, bool con, bool xcd)
 {
     if (pa.npatch <= 0) return hipSuccess;
     const size_t sm = jacobi_patch_smem(pa.maxl, det);
 #define XPBD_PATCH_LAUNCH(F, D, Cn)                                                                        \
     hipLaunchKernelGGL((k_jacobi_patch<F, D, Cn>), dim3(pa.npatch), dim3(kPatch), sm, s, ta, pa.tloc, pa.pvoff, \
-                       pa.pverts, pa.pslot, pa.inc_off, pa.inc, pos, pa.partial, cacc, nt, pa.maxl)
+                       pa.pverts, pa.pslot, pa.inc_off, pa.inc, pos, pa.partial, cacc, nt, pa.maxl, (int)xcd)
     const int sel = (first ? 4 : 0) | (det ? 2 : 0) | (con ? 1 : 0);
     switch (sel) {
     case 0: XPBD_PATCH_LAUNCH(false, false, false); break;

@@ -94,7 +94,7 @@ _sig("xpbd_batch_num_instances", [_vp, _ip])
 _sig("xpbd_batch_set_sdf", [_vp, C.c_int32, C.c_int32, _dp, _dp])
 _sig("xpbd_batch_get_positions", [_vp, C.c_int32, _dp])
 _sig("xpbd_autotune", [_vp, C.c_char_p])
-_sig("xpbd_get_variant", [_vp, _ip, _ip])
+_sig("xpbd_get_variant", [_vp, _ip, _ip, _ip])
 _sig("xpbd_set_attachments", [_vp, C.c_int32, _ip, _dp])
 _sig("xpbd_mesh_grid", [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, _dp, _dp, _ip])
 _sig("xpbd_mesh_surface", [C.c_int32, C.c_int32, _ip, _ip, _ip])
@@ -404,10 +404,14 @@ class Solver:
 
     def variant(self):
         """(variant name, workgroup size or 0) of the kernels in use."""
-        v, b = C.c_int32(), C.c_int32()
-        self._c(lib.xpbd_get_variant(self.h, C.byref(v), C.byref(b)))
+        return self.launch_config()[:2]
+
+    def launch_config(self):
+        """(variant name, workgroup size or 0, XCD-contiguous mapping on) of the kernels in use."""
+        v, b, x = C.c_int32(), C.c_int32(), C.c_int32()
+        self._c(lib.xpbd_get_variant(self.h, C.byref(v), C.byref(b), C.byref(x)))
         names = {val: k for k, val in VARIANTS.items()}
-        return names.get(v.value, str(v.value)), b.value
+        return names.get(v.value, str(v.value)), b.value, bool(x.value)
 
     # learned warm start
     def attach(self, net: Optional["Network"]):
