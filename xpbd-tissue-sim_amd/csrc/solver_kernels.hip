@@ -548,6 +548,7 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
     float4* lb = smem;                                   // base, inverse mass
     float* ld = reinterpret_cast<float*>(smem + maxl);   // in-step motion, 3 per local vertex
     float* red = ld + 3 * maxl;                          // DET: 12 planes of kPatch; else 3 per local vertex
+    uint32_t* linc = reinterpret_cast<uint32_t*>(red + 12 * kPatch);  // DET: the patch's incidence list
     const int tid = threadIdx.x;
     const int p = xcd_item(blockIdx.x, gridDim.x, xcd != 0);
     const int v0 = pvoff[p], nl = pvoff[p + 1] - v0;
@@ -567,6 +568,14 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
             l[0] = lv.x;
             l[1] = lv.y;
         }
+    }
+    // DET: the incidence list (4 entries per tet, 16 bits each) goes to LDS with
+    // the other loads, so the gather below walks LDS only
+    if (DET) {
+        // 4-byte LDS stores: the list's LDS offset is a multiple of 4, not always of 8
+        const uint2 e = reinterpret_cast<const uint2*>(inc + (size_t)p * 4 * kPatch)[tid];
+        linc[2 * tid] = e.x;
+        linc[2 * tid + 1] = e.y;
     }
     for (int i = tid; i < nl; i += kPatch) {
         const int g = pverts[v0 + i];
@@ -611,7 +620,7 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
         if (DET) {
             a0 = a1 = a2 = 0.f;
             const int s0 = inc_off[ioff + i], s1 = inc_off[ioff + i + 1];
-            const uint16_t* il = inc + (size_t)p * 4 * kPatch;
+            const uint16_t* il = reinterpret_cast<const uint16_t*>(linc);
             for (int s = s0; s < s1; s++) {
                 const int ent = il[s];
                 const int t = ent >> 2, j = ent & 3;
@@ -1430,7 +1439,7 @@ hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float
 
 size_t jacobi_patch_smem(int maxl, bool det)
 {
-    return (size_t)maxl * 16 + (size_t)maxl * 12 + (det ? (size_t)12 * kPatch * 4 : (size_t)maxl * 12);
+    return (size_t)maxl * 16 + (size_t)maxl * 12 + (det ? (size_t)12 * kPatch * 4 + (size_t)4 * kPatch * 2 : (size_t)maxl * 12);
 }
 
 int jacobi_patch_size() { return kPatch; }

@@ -15,6 +15,8 @@
 // --print-config    parse the config, print a summary, touch no GPU
 // --load-meshes     with --print-config: also load and place each mesh (host only)
 // --solver-type S   override every object's solver-type (a config solver-type name)
+// --autotune        time the kernel variants of every object after set-up and keep
+//                   the fastest (xpbd_autotune)
 // --eager           launch every step without the HIP graph (kernel timing mode);
 //                   needed by profilers that cannot follow graph launches
 // --copy-calibration MB
@@ -54,11 +56,12 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "usage: %s CONFIG.yaml [--beam] [--steps N] [--dump-positions FILE] "
                              "[--dump-order FILE] [--dump-face-rank FILE] [--dump-rest FILE] [--eager] [--copy-calibration MB] "
                              "[--print-config] [--grasp R] [--cursor X,Y,Z] [--cursor-velocity VX,VY,VZ] "
-                             "[--dump-attachments FILE]\n", argv[0]);
+                             "[--dump-attachments FILE] [--autotune]\n", argv[0]);
         return 2;
     }
     std::string cfg_path = argv[1], dump_pos, dump_order, dump_rank, dump_rest, dump_att;
     double grasp_radius = -1;
+    bool autotune = false;
     bool have_cursor = false;
     Sim::Vec3r cursor{ 0, 0, 0 }, cursor_vel{ 0, 0, 0 };
     auto triple = [](const char* t, Sim::Vec3r& out) {
@@ -74,6 +77,7 @@ int main(int argc, char** argv)
         else if (a == "--print-config") print_only = true;
         else if (a == "--load-meshes") load_meshes = true;
         else if (a == "--eager") eager = true;
+        else if (a == "--autotune") autotune = true;
         else if (a == "--solver-type" && i + 1 < argc) solver_override = argv[++i];
         else if (a == "--copy-calibration" && i + 1 < argc) copy_mb = std::atof(argv[++i]);
         else if (a == "--steps" && i + 1 < argc) steps = std::atol(argv[++i]);
@@ -145,6 +149,14 @@ int main(int argc, char** argv)
             std::fprintf(stderr, "calibration copy: %zu bytes device to device\n", bytes);
         }
         sim->setup();
+        if (autotune)
+            for (auto& o : sim->objects()) {
+                const int rc = xpbd_autotune(o->solver().handle(), nullptr);
+                if (rc != XPBD_OK) throw Sim::SimError(rc, "xpbd_autotune");
+                int32_t v = 0, b = 0, x = 0;
+                xpbd_get_variant(o->solver().handle(), &v, &b, &x);
+                std::fprintf(stderr, "autotuned %s: variant %d block %d xcd %d\n", o->name().c_str(), v, b, x);
+            }
         if (grasp) {
             if (!have_cursor && !sim->objects().empty()) {
                 const Sim::AABB b = sim->objects()[0]->boundingBox();
