@@ -293,25 +293,37 @@ __device__ __forceinline__ float quad_lane3(float x)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xFF, 0xF, 0xF, false));
 }
 
-// BLOCK threads per workgroup = BLOCK / 4 clusters; the launch shape is a tuning choice
+// BLOCK threads per workgroup = BLOCK / 4 clusters; the launch shape is a tuning choice.
+template <int BLOCK>
+struct QuadLds {
+    static constexpr int kClusters = BLOCK / 4;
+    static constexpr int kStage = kClusters * 8;  // at most 8 tets per cluster
+    static constexpr int kCol = kClusters + 1;    // padded column stride (LDS banks)
+    float sb[8 * 4 * kCol];    // [slot][xyz, w][cluster]
+    float sd[8 * 3 * kCol];    // [slot][xyz][cluster]
+    float sq[12][kStage];      // Q (9), alpha tilde (2), gamma
+    float sl[2][kStage];       // lambda
+    uint32_t stl[kStage];      // local vertex slots
+};
+
+// One chunk of BLOCK / 4 consecutive clusters starting at cluster cb (clusters
+// below cend), projected by one workgroup and written back.
 template <bool FIRST, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const int4* __restrict__ cverts,
-                                                          const int* __restrict__ cstart,
-                                                          const uint32_t* __restrict__ tloc,
-                                                          float4* __restrict__ pos, int cbeg, int cend, int xcd)
+__device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& ta, const int4* __restrict__ cverts,
+                                           const int* __restrict__ cstart, const uint32_t* __restrict__ tloc,
+                                           float4* __restrict__ pos, int cb, int cend)
 {
     constexpr int kQuadBlock = BLOCK;
-    constexpr int kQuadClusters = BLOCK / 4;
-    constexpr int kQuadStage = kQuadClusters * 8;  // at most 8 tets per cluster
-    constexpr int kQuadCol = kQuadClusters + 1;    // padded column stride (LDS banks)
-    __shared__ float sb[8 * 4 * kQuadCol];   // [slot][xyz, w][cluster]
-    __shared__ float sd[8 * 3 * kQuadCol];   // [slot][xyz][cluster]
-    __shared__ float sq[12][kQuadStage];     // Q (9), alpha tilde (2), gamma
-    __shared__ float sl[2][kQuadStage];      // lambda
-    __shared__ uint32_t stl[kQuadStage];     // local vertex slots
+    constexpr int kQuadClusters = QuadLds<BLOCK>::kClusters;
+    constexpr int kQuadStage = QuadLds<BLOCK>::kStage;
+    constexpr int kQuadCol = QuadLds<BLOCK>::kCol;
+    float* sb = L.sb;
+    float* sd = L.sd;
+    auto& sq = L.sq;
+    auto& sl = L.sl;
+    uint32_t* stl = L.stl;
     const int tid = threadIdx.x;
     const int cl = tid >> 2, j = tid & 3;
-    const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * kQuadClusters;
     const int cl_end = min(cend, cb + kQuadClusters);
     const int c = cb + cl;
     const bool active = c < cend;
@@ -395,9 +407,9 @@ __global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const i
         sd[(q * 3 + 2) * kQuadCol + cl] = dhi.z;
     }
     __syncthreads();
-    // a quad is active or inactive as a whole, so DPP reads stay inside active quads
-    if (!active) return;
-    const int t0 = cs0 - tb, t1 = cs1 - tb;
+    // a quad is active or inactive as a whole, so DPP reads stay inside active quads;
+    // an inactive quad has an empty element range
+    const int t0 = active ? cs0 - tb : 0, t1 = active ? cs1 - tb : 0;
     for (int e = t0; e < t1; e++) {
         const int slot = (int)((stl[e] >> (4 * j)) & 15u);
         float bb[3], dd[3];
@@ -474,6 +486,17 @@ __global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const i
         pos[2 * (size_t)v_hi + 1] = make_float4(sd[(qs * 3 + 0) * kQuadCol + cl], sd[(qs * 3 + 1) * kQuadCol + cl],
                                                 sd[(qs * 3 + 2) * kQuadCol + cl], 0.f);
     }
+}
+
+template <bool FIRST, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const int4* __restrict__ cverts,
+                                                          const int* __restrict__ cstart,
+                                                          const uint32_t* __restrict__ tloc,
+                                                          float4* __restrict__ pos, int cbeg, int cend, int xcd)
+{
+    __shared__ QuadLds<BLOCK> L;
+    const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * QuadLds<BLOCK>::kClusters;
+    quad_chunk<FIRST, BLOCK>(L, ta, cverts, cstart, tloc, pos, cb, cend);
 }
 
 // ---------------------------------------------------------------------------
@@ -1060,18 +1083,19 @@ __device__ __forceinline__ bool contact_dx(const double* const x[3], const float
     return true;
 }
 
-template <bool FIRST>
-__global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__ faces,
-                                                       const int* __restrict__ fcol_off, int ncol,
-                                                       const int* __restrict__ cpos, const int* __restrict__ clist,
-                                                       const double* __restrict__ rest, float4* __restrict__ pos,
-                                                       const float4* __restrict__ lo,
-                                                       const double* __restrict__ cp, const double* __restrict__ cn,
-                                                       const double* __restrict__ cb, double* __restrict__ clam)
+// All contacts, colour by colour (contacts of one colour share no vertex), by
+// one workgroup of nthreads threads.
+__device__ __forceinline__ void contacts_gs_pass(bool first, int tid, int nthreads, const int4* __restrict__ faces,
+                                                 const int* __restrict__ fcol_off, int ncol,
+                                                 const int* __restrict__ cpos, const int* __restrict__ clist,
+                                                 const double* __restrict__ rest, float4* __restrict__ pos,
+                                                 const float4* __restrict__ lo, const double* __restrict__ cp,
+                                                 const double* __restrict__ cn, const double* __restrict__ cb,
+                                                 double* __restrict__ clam)
 {
     for (int c = 0; c < ncol; c++) {
         const int s = cpos[fcol_off[c]], e = cpos[fcol_off[c + 1]];
-        for (int k = s + (int)threadIdx.x; k < e; k += kBlock) {
+        for (int k = s + tid; k < e; k += nthreads) {
             const int j = clist[k];
             const int4 f = faces[j];
             double x0[3], x1[3], x2[3];
@@ -1080,7 +1104,7 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
             world_pos(rest, pos, lo, f.z, true, x2);
             const double* xs[3] = { x0, x1, x2 };
             const float w[3] = { pos[2 * (size_t)f.x].w, pos[2 * (size_t)f.y].w, pos[2 * (size_t)f.z].w };
-            double lam = FIRST ? 0.0 : clam[j];
+            double lam = first ? 0.0 : clam[j];
             double dx[9];
             if (contact_dx(xs, w, cp + 3 * (size_t)j, cn + 3 * (size_t)j, cb + 3 * (size_t)j, &lam, dx)) {
                 const int vid[3] = { f.x, f.y, f.z };
@@ -1097,6 +1121,18 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
         }
         __syncthreads();
     }
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__ faces,
+                                                       const int* __restrict__ fcol_off, int ncol,
+                                                       const int* __restrict__ cpos, const int* __restrict__ clist,
+                                                       const double* __restrict__ rest, float4* __restrict__ pos,
+                                                       const float4* __restrict__ lo,
+                                                       const double* __restrict__ cp, const double* __restrict__ cn,
+                                                       const double* __restrict__ cb, double* __restrict__ clam)
+{
+    contacts_gs_pass(FIRST, threadIdx.x, kBlock, faces, fcol_off, ncol, cpos, clist, rest, pos, lo, cp, cn, cb, clam);
 }
 
 template <bool FIRST>
@@ -1155,6 +1191,27 @@ __device__ __forceinline__ void attach_dx(const double* x, const double* p, floa
     for (int k = 0; k < 3; k++) dx[k] = (double)w * g[k] * dlam;
 }
 
+// Attachments a in [a0, n) by stride, each applied in place.
+__device__ __forceinline__ void attach_gs_pass(bool first, int a0, int stride, const int* __restrict__ av,
+                                               const double* __restrict__ ap, double* __restrict__ alam,
+                                               const double* __restrict__ rest, float4* __restrict__ pos,
+                                               const float4* __restrict__ lo, int n)
+{
+    for (int a = a0; a < n; a += stride) {
+        const int v = av[a];
+        double x[3], dx[3];
+        world_pos(rest, pos, lo, v, true, x);
+        double lam = first ? 0.0 : alam[a];
+        attach_dx(x, ap + 3 * (size_t)a, pos[2 * (size_t)v].w, &lam, dx);
+        alam[a] = lam;
+        float4 d = pos[2 * (size_t)v + 1];
+        d.x += (float)dx[0];
+        d.y += (float)dx[1];
+        d.z += (float)dx[2];
+        pos[2 * (size_t)v + 1] = d;
+    }
+}
+
 template <bool FIRST>
 __global__ __launch_bounds__(kBlock) void k_attach_gs(const int* __restrict__ av, const double* __restrict__ ap,
                                                      double* __restrict__ alam, const double* __restrict__ rest,
@@ -1162,17 +1219,7 @@ __global__ __launch_bounds__(kBlock) void k_attach_gs(const int* __restrict__ av
 {
     const int a = blockIdx.x * kBlock + threadIdx.x;
     if (a >= n) return;
-    const int v = av[a];
-    double x[3], dx[3];
-    world_pos(rest, pos, lo, v, true, x);
-    double lam = FIRST ? 0.0 : alam[a];
-    attach_dx(x, ap + 3 * (size_t)a, pos[2 * (size_t)v].w, &lam, dx);
-    alam[a] = lam;
-    float4 d = pos[2 * (size_t)v + 1];
-    d.x += (float)dx[0];
-    d.y += (float)dx[1];
-    d.z += (float)dx[2];
-    pos[2 * (size_t)v + 1] = d;
+    attach_gs_pass(FIRST, a, n, av, ap, alam, rest, pos, lo, n);
 }
 
 template <bool FIRST>
