@@ -41,6 +41,9 @@ constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH
 static_assert(kNParams == 20099, "parameter count");
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// inference: hardware reciprocal (1 ulp) in place of the IEEE division; the
+// result is rounded to BF16 right after
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z)
 {
     const float s = 1.0f / (1.0f + __expf(-z));
@@ -270,28 +273,54 @@ __global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, 
     __shared__ alignas(16) __bf16 sh[4][16][kH + 8];
     __shared__ float sb[2 * kH + kOut];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int i = tid; i < kH * kInP; i += 256) sw1[i / kInP][i % kInP] = w1[i];
-    for (int i = tid; i < kH * kH; i += 256) sw2[i / kH][i % kH] = w2[i];
-    for (int i = tid; i < kOutP * kH; i += 256) sw3[i / kH][i % kH] = w3[i];
+    // weights in 16-byte pieces (8 BF16 values; rows are multiples of 8 long and
+    // the padded LDS rows start on 16-byte boundaries)
+    for (int i = tid; i < kH * kInP / 8; i += 256)
+        *reinterpret_cast<bf16x8*>(&sw1[i / (kInP / 8)][8 * (i % (kInP / 8))]) = reinterpret_cast<const bf16x8*>(w1)[i];
+    for (int i = tid; i < kH * kH / 8; i += 256)
+        *reinterpret_cast<bf16x8*>(&sw2[i / (kH / 8)][8 * (i % (kH / 8))]) = reinterpret_cast<const bf16x8*>(w2)[i];
+    for (int i = tid; i < kOutP * kH / 8; i += 256)
+        *reinterpret_cast<bf16x8*>(&sw3[i / (kH / 8)][8 * (i % (kH / 8))]) = reinterpret_cast<const bf16x8*>(w3)[i];
     for (int i = tid; i < kH; i += 256) {
         sb[i] = params[oB1 + i];
         sb[kH + i] = params[oB2 + i];
     }
     if (tid < kOut) sb[2 * kH + tid] = params[oB3 + tid];
     __syncthreads();
+    // from here on each wave works on its own rows and its own LDS tiles (sx[wv],
+    // sh[wv]): LDS operations of one wave complete in program order, so a wave
+    // barrier (which also keeps the compiler from moving LDS accesses across it)
+    // orders the lanes' writes before other lanes' reads, and the four waves run
+    // independently, overlapping one wave's MFMA with another's activations
     const int ntile = (n + kFwdRows - 1) / kFwdRows;
     const int col = lane & 15, kq = 8 * (lane >> 4), rq = 4 * (lane >> 4);
+    // input staging: lane holds feature k = lane % 32 of rows lane / 32 + 2 * it;
+    // the next tile's rows are loaded while the current tile is computed
+    constexpr int kXPer = 16 * kInP / 64;
+    const int xk = lane % kInP, xr = lane / kInP;
+    const bool kin = xk < kIn;
+    const float xmean = kin ? mean[xk] : 0.f, xistd = kin ? istd[xk] : 0.f;
+    float xn[kXPer];
+    auto load_rows = [&](int t) {
+        const int r0 = t * kFwdRows + wv * 16;
+#pragma unroll
+        for (int it = 0; it < kXPer; it++) {
+            const int gr = r0 + xr + 2 * it;
+            xn[it] = (gr < n && kin) ? X[(size_t)gr * kIn + xk] : 0.f;
+        }
+    };
+    if (blockIdx.x < ntile) load_rows(blockIdx.x);
     for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
         const int r0 = t * kFwdRows + wv * 16;
         // standardised input rows -> sx (bf16)
-        for (int i = lane; i < 16 * kInP; i += 64) {
-            const int r = i / kInP, k = i % kInP;
-            const int gr = r0 + r;
-            float v = 0.f;
-            if (gr < n && k < kIn) v = (X[(size_t)gr * kIn + k] - mean[k]) * istd[k];
-            sx[wv][r][k] = (__bf16)v;
+#pragma unroll
+        for (int it = 0; it < kXPer; it++) {
+            const int gr = r0 + xr + 2 * it;
+            const float v = (gr < n && kin) ? (xn[it] - xmean) * xistd : 0.f;
+            sx[wv][xr + 2 * it][xk] = (__bf16)v;
         }
-        __syncthreads();
+        if (t + (int)gridDim.x < ntile) load_rows(t + gridDim.x);
+        __builtin_amdgcn_wave_barrier();
         // layer 1: 8 column tiles, one K step
         const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&sx[wv][col][kq]);
 #pragma unroll
@@ -300,9 +329,9 @@ __global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, 
             f32x4 z = { 0.f, 0.f, 0.f, 0.f };
             z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, z, 0, 0, 0);
 #pragma unroll
-            for (int j = 0; j < 4; j++) sh[wv][rq + j][nt * 16 + col] = (__bf16)silu(z[j] + sb[nt * 16 + col]);
+            for (int j = 0; j < 4; j++) sh[wv][rq + j][nt * 16 + col] = (__bf16)silu_fast(z[j] + sb[nt * 16 + col]);
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
         // layer 2: 8 column tiles x 4 K steps
         f32x4 z2[kH / 16];
 #pragma unroll
@@ -316,12 +345,12 @@ __global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, 
                 z2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, z2[nt], 0, 0, 0);
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int nt = 0; nt < kH / 16; nt++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) sh[wv][rq + j][nt * 16 + col] = (__bf16)silu(z2[nt][j] + sb[kH + nt * 16 + col]);
-        __syncthreads();
+            for (int j = 0; j < 4; j++) sh[wv][rq + j][nt * 16 + col] = (__bf16)silu_fast(z2[nt][j] + sb[kH + nt * 16 + col]);
+        __builtin_amdgcn_wave_barrier();
         // layer 3: one padded column tile x 4 K steps
         f32x4 y = { 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
@@ -348,7 +377,7 @@ __global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, 
                 if (gr < n) reinterpret_cast<float*>(out + gr)[3] = 0.f;
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
