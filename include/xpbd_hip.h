@@ -252,15 +252,17 @@ int xpbd_batch_get_positions(xpbd_batch* b, int32_t instance, double* out);
 /* Time the kernel variants and launch shapes that share this handle's device
  * layout on its own problem (Gauss-Seidel with clusters: the one-thread
  * cluster kernel and the quad kernel at 64/128/256 threads per workgroup;
+ * the patch Jacobi kernel with and without the fused one-launch iteration;
  * cluster and patch kernels each with and without the XCD-contiguous
  * workgroup mapping), restore the state, and keep the fastest.  The choice is
  * appended to cache_path (may be NULL) as
  * "shape-key variant block xcd ms-per-iteration" and reused from there for the
  * same shape.  No reference counterpart: the reference has no kernel variants. */
 int xpbd_autotune(xpbd_batch* b, const char* cache_path);
-/* The kernel variant in use (XPBD_VARIANT_*), for CLUSTER_QUAD the workgroup
- * size (0 otherwise), and whether the XCD-contiguous workgroup mapping is on
- * (block and xcd may be NULL). */
+/* The kernel variant in use (XPBD_VARIANT_*); block: for CLUSTER_QUAD the
+ * workgroup size, for PATCH 1 when the fused one-launch iteration is in use
+ * (deterministic mode without attachments), else 0; xcd: whether the
+ * XCD-contiguous workgroup mapping is on (block and xcd may be NULL). */
 int xpbd_get_variant(xpbd_handle* h, int32_t* variant, int32_t* block, int32_t* xcd);
 /* Batch handles are also plain handles: xpbd_step, xpbd_destroy, ... apply. */
 

@@ -5,9 +5,11 @@
 // Gauss-Seidel handles built with clusters can switch between the one-thread
 // cluster kernel and the quad kernel at 64, 128 or 256 threads per workgroup
 // without rebuilding anything (same clusters, colours and order, so the same
-// results up to FP32 summation order within an element).  Cluster and patch
-// kernels are also timed with and without the XCD-contiguous workgroup mapping
-// (which does not change results).  Other handles have a single candidate; the
+// results up to FP32 summation order within an element).  Patch Jacobi handles
+// time the fused one-launch iteration against the patch + shared-vertex +
+// contact launches (the same sums; with contacts the contact corrections are
+// added in another order).  Cluster and patch kernels are also timed with and
+// without the XCD-contiguous workgroup mapping (which does not change results).  Other handles have a single candidate; the
 // call then only records it.
 #include <hip/hip_runtime.h>
 
@@ -67,21 +69,23 @@ bool valid(const xpbd_handle* h, const Candidate& c)
 {
     if (c.variant == XPBD_VARIANT_CLUSTER) return h->nclusters > 0;
     if (c.variant == XPBD_VARIANT_CLUSTER_QUAD) return h->nclusters > 0 && quad_block_supported(c.block);
+    if (c.variant == XPBD_VARIANT_PATCH) return h->variant == XPBD_VARIANT_PATCH && (c.block == 0 || h->fused_ok);
     return c.variant == h->variant;
 }
 
 void apply(xpbd_handle* h, const Candidate& c)
 {
     if (h->variant != c.variant || (c.variant == XPBD_VARIANT_CLUSTER_QUAD && h->quad_block != c.block) ||
-        h->xcd_map != (c.xcd != 0)) {
+        (c.variant == XPBD_VARIANT_PATCH && h->patch_fused != (c.block == 1)) || h->xcd_map != (c.xcd != 0)) {
         h->variant = c.variant;
         if (c.variant == XPBD_VARIANT_CLUSTER_QUAD) h->quad_block = c.block;
+        if (c.variant == XPBD_VARIANT_PATCH) h->patch_fused = c.block == 1;
         h->xcd_map = c.xcd != 0;
         invalidate_graphs(h);
     }
     char buf[64];
     std::snprintf(buf, sizeof buf, "variant %d block %d xcd %d", c.variant,
-                  c.variant == XPBD_VARIANT_CLUSTER_QUAD ? c.block : 0, c.xcd);
+                  c.variant == XPBD_VARIANT_CLUSTER_QUAD || c.variant == XPBD_VARIANT_PATCH ? c.block : 0, c.xcd);
     h->tuned = buf;
 }
 
@@ -104,7 +108,9 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
                                         Candidate{ XPBD_VARIANT_CLUSTER_QUAD, 256, x } })
                 cand.push_back(b);
     } else if (h->variant == XPBD_VARIANT_PATCH) {
-        cand = { { h->variant, 0, 1 }, { h->variant, 0, 0 } };
+        // block 1: the fused one-launch iteration (when the layout allows it)
+        for (int f = h->fused_ok ? 1 : 0; f >= 0; f--)
+            for (int x = 1; x >= 0; x--) cand.push_back({ h->variant, f, x });
     } else {
         cand = { { h->variant, h->quad_block, h->xcd_map ? 1 : 0 } };
     }
@@ -126,7 +132,7 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
     if (e == hipSuccess) e = hipMemcpyAsync(save_lam, h->d_lam, lam_bytes, hipMemcpyDeviceToDevice, s);
     int rc = e == hipSuccess ? XPBD_OK : hip_err(h, e, "autotune set-up");
     const int saved_variant = h->variant, saved_block = h->quad_block;
-    const bool saved_xcd = h->xcd_map;
+    const bool saved_xcd = h->xcd_map, saved_fused = h->patch_fused;
     double best_ms = 1e300;
     Candidate best = cand[0];
     const int iters = h->params.iters > 0 ? h->params.iters : 1;
@@ -134,6 +140,7 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
         if (rc) break;
         h->variant = cd.variant;
         if (cd.variant == XPBD_VARIANT_CLUSTER_QUAD) h->quad_block = cd.block;
+        if (cd.variant == XPBD_VARIANT_PATCH) h->patch_fused = cd.block == 1;
         h->xcd_map = cd.xcd != 0;
         double ms_best = 1e300;
         for (int rep = 0; rep < 4 && !rc; rep++) {  // first repetition warms up
@@ -165,6 +172,7 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
     h->variant = saved_variant;
     h->quad_block = saved_block;
     h->xcd_map = saved_xcd;
+    h->patch_fused = saved_fused;
     if (rc) return rc;
     apply(h, best);
     write_cache(cache_path, key, best, best_ms / iters);
@@ -175,7 +183,10 @@ extern "C" int xpbd_get_variant(xpbd_handle* h, int32_t* variant, int32_t* block
 {
     if (!h || !variant) return XPBD_E_INVALID_ARG;
     *variant = h->variant;
-    if (block) *block = h->variant == XPBD_VARIANT_CLUSTER_QUAD ? h->quad_block : 0;
+    if (block)
+        *block = h->variant == XPBD_VARIANT_CLUSTER_QUAD ? h->quad_block
+               : h->variant == XPBD_VARIANT_PATCH    ? (h->patch_fused && h->fused_ok ? 1 : 0)
+                                                     : 0;
     if (xcd) *xcd = h->xcd_map ? 1 : 0;
     return XPBD_OK;
 }

@@ -419,9 +419,11 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
         rc |= dalloc(h, &h->d_pslot, Pt.pslot.size());
         rc |= dalloc(h, &h->d_inc_off, Pt.inc_off.size());
         rc |= dalloc(h, &h->d_inc, Pt.inc.size());
-        rc |= dalloc(h, &h->d_partial, (size_t)Pt.nslots);
+        rc |= dalloc(h, &h->d_partial, 2 * (size_t)Pt.nslots);
         rc |= dalloc(h, &h->d_shared, Pt.shared.size());
         rc |= dalloc(h, &h->d_soff, Pt.soff.size());
+        rc |= dalloc(h, &h->d_pshare, Pt.pverts.size());
+        rc |= dalloc(h, &h->d_sbuf, 2 * Pt.shared.size());
     }
     if (h->nclusters > 0) {
         rc |= dalloc(h, &h->d_cverts, 2 * (size_t)h->nclusters);
@@ -490,6 +492,60 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
         const int f = h->fperm[j];
         hf[j] = make_int4(faces[4 * (size_t)f], faces[4 * (size_t)f + 1], faces[4 * (size_t)f + 2], faces[4 * (size_t)f + 3]);
     }
+    // fused patch Jacobi: every face goes to the patch of an element that has it,
+    // with the local slots of its three vertices there
+    std::vector<int32_t> pf_off;
+    std::vector<int4> pfv;
+    if (h->variant == XPBD_VARIANT_PATCH) {
+        const Patches& Pt = h->patches;
+        bool ok = Pt.fusable && jacobi_fused_smem(Pt.maxl, true) <= 64 * 1024;
+        const int psz = jacobi_patch_size();
+        pf_off.assign((size_t)Pt.npatch + 1, 0);
+        std::vector<int32_t> fpatch(nf, -1);
+        std::vector<int4> floc(nf);
+        if (ok && nf > 0) {
+            std::vector<int32_t> at(nt);  // element -> solver position
+            for (int i = 0; i < nt; i++) at[h->order[i]] = i;
+            std::vector<int32_t> voff((size_t)nv + 1, 0), vel(4 * (size_t)nt);
+            for (int e = 0; e < nt; e++)
+                for (int k = 0; k < 4; k++) voff[tets[4 * (size_t)e + k] + 1]++;
+            for (int v = 0; v < nv; v++) voff[v + 1] += voff[v];
+            std::vector<int32_t> fill(voff.begin(), voff.end() - 1);
+            for (int e = 0; e < nt; e++)
+                for (int k = 0; k < 4; k++) vel[fill[tets[4 * (size_t)e + k]]++] = e;
+            for (int j = 0; j < nf && ok; j++) {
+                const int a = hf[j].x, b = hf[j].y, c = hf[j].z;
+                int found = -1;
+                for (int k = voff[a]; k < voff[a + 1] && found < 0; k++) {
+                    const int32_t* t = &tets[4 * (size_t)vel[k]];
+                    int hits = 0;
+                    for (int m = 0; m < 4; m++) hits += (t[m] == b) + (t[m] == c);
+                    if (hits == 2) found = vel[k];
+                }
+                if (found < 0) {
+                    ok = false;  // a face that is no element's face: contacts stay in their own pass
+                    break;
+                }
+                const int p = at[found] / psz;
+                auto lb_ = Pt.pverts.begin() + Pt.pvoff[p], le_ = Pt.pverts.begin() + Pt.pvoff[p + 1];
+                auto slot = [&](int v) { return (int)(std::lower_bound(lb_, le_, v) - lb_); };
+                fpatch[j] = p;
+                floc[j] = make_int4(j, slot(a), slot(b), slot(c));
+                pf_off[p + 1]++;
+            }
+        }
+        if (ok) {
+            for (int p = 0; p < Pt.npatch; p++) pf_off[p + 1] += pf_off[p];
+            pfv.resize(nf);
+            std::vector<int32_t> fill(pf_off.begin(), pf_off.end() - 1);
+            for (int j = 0; j < nf; j++) pfv[fill[fpatch[j]]++] = floc[j];  // face order within a patch
+            int rc2 = dalloc(h, &h->d_pf_off, pf_off.size());
+            if (!pfv.empty()) rc2 |= dalloc(h, &h->d_pf, pfv.size());
+            if (rc2) ok = false;
+        }
+        h->fused_ok = ok;
+        h->patch_fused = ok;
+    }
     h->sdf_host.assign(n, SdfDev{});
     for (auto& s : h->sdf_host) s.quat[3] = 1.0;
 #define UP(dst, src, bytes)                                                      \
@@ -522,6 +578,11 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
             UP(h->d_shared, Pt.shared.data(), sizeof(int32_t) * Pt.shared.size());
         }
         UP(h->d_soff, Pt.soff.data(), sizeof(int32_t) * Pt.soff.size());
+        UP(h->d_pshare, Pt.pshare.data(), sizeof(int32_t) * Pt.pshare.size());
+        if (h->fused_ok) {
+            UP(h->d_pf_off, pf_off.data(), sizeof(int32_t) * pf_off.size());
+            if (!pfv.empty()) UP(h->d_pf, pfv.data(), sizeof(int4) * pfv.size());
+        }
     }
     if (h->nclusters > 0) {
         UP(h->d_cverts, cverts_solver.data(), sizeof(int32_t) * cverts_solver.size());
@@ -597,6 +658,57 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
     TetArrays ta = tet_arrays(h);
     const bool con = contacts_enabled(h);
     hipError_t e = hipSuccess;
+    if (h->variant == XPBD_VARIANT_PATCH && h->patch_fused && h->fused_ok && h->params.deterministic != 0 &&
+        h->natt == 0) {
+        // fused patch Jacobi: one launch per iteration, the shared vertices' last
+        // sums after the loop
+        const Patches& Pt = h->patches;
+        const size_t nsl = (size_t)Pt.nslots, nsh = Pt.shared.size();
+        FusedPatchArgs A{};
+        A.tloc = reinterpret_cast<const uint2*>(h->d_ptloc);
+        A.pvoff = h->d_pvoff;
+        A.pverts = h->d_pverts;
+        A.pslot = h->d_pslot;
+        A.pshare = h->d_pshare;
+        A.inc_off = h->d_inc_off;
+        A.inc = h->d_inc;
+        A.pos = h->d_pos;
+        A.npatch = Pt.npatch;
+        A.maxl = Pt.maxl;
+        A.nt = h->nt;
+        A.xcd = h->xcd_map ? 1 : 0;
+        A.pf_off = h->d_pf_off;
+        A.pf = h->d_pf;
+        A.flag = h->d_flag;
+        A.rest = h->d_rest;
+        A.lo = h->d_lo;
+        A.cp = h->d_cp;
+        A.cn = h->d_cn;
+        A.cb = h->d_cb;
+        A.clam = h->d_clam;
+        for (int it = 0; it < iters; it++) {
+            const int par = it & 1;
+            A.prd = h->d_partial + (1 - par) * nsl;
+            A.pwr = h->d_partial + par * nsl;
+            A.brd = h->d_sbuf + par * nsh;
+            A.bwr = h->d_sbuf + (1 - par) * nsh;
+            A.first_iter = it == 0;
+            if (timed) timed_begin(h);
+            e = launch_jacobi_fused(s, ta, A, it == 0, con);
+            if (e != hipSuccess) return hip_err(h, e, "jacobi_fused launch");
+            if (timed) {
+                timed_end(h);
+                h->timed_iterations++;
+            }
+        }
+        if (iters > 0) {
+            // after iteration n-1: values before it in half n % 2, its partials in half (n-1) % 2
+            e = launch_jacobi_fused_final(s, h->d_shared, h->d_soff, h->d_partial + ((iters - 1) & 1) * nsl,
+                                          h->d_sbuf + (iters & 1) * nsh, h->d_pos, (int)nsh);
+            if (e != hipSuccess) return hip_err(h, e, "jacobi_fused_final launch");
+        }
+        return XPBD_OK;
+    }
     for (int it = 0; it < iters; it++) {
         const bool first = it == 0;
         if (h->params.solver == XPBD_SOLVER_GS_COLORED) {

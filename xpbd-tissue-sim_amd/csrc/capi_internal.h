@@ -67,9 +67,16 @@ struct xpbd_handle {
     int32_t* d_pslot = nullptr;
     int32_t* d_inc_off = nullptr;
     uint16_t* d_inc = nullptr;
-    float4* d_partial = nullptr;
+    float4* d_partial = nullptr;     // 2 x nslots: partial sums, by iteration parity in the fused variant
     int32_t* d_shared = nullptr;
     int32_t* d_soff = nullptr;
+    // fused patch Jacobi (one launch per iteration, see FusedPatchArgs)
+    bool fused_ok = false;           // the patch layout and the faces allow it
+    bool patch_fused = false;        // in use (autotuned)
+    int2* d_pshare = nullptr;
+    int32_t* d_pf_off = nullptr;
+    int4* d_pf = nullptr;
+    float4* d_sbuf = nullptr;        // 2 x nshared shared-vertex values, by iteration parity
     double* d_part = nullptr;
     int red_blocks = 1;
     double* d_scalar = nullptr;

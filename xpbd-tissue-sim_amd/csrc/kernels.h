@@ -71,6 +71,41 @@ size_t jacobi_patch_smem(int maxl, bool det);
 // corrections (float4 per vertex) when con is set; it is consumed and zeroed.
 hipError_t launch_jacobi_patch(hipStream_t s, const TetArrays& ta, const PatchArrays& pa, float4* pos, float4* cacc,
                                int nt, bool first, bool det, bool con, bool xcd);
+// Fused patch Jacobi (fixed-order accumulation, no attachments): one launch per
+// iteration; the shared vertices' sums are formed while they are staged in the
+// next iteration, and contacts are projected by the patch that holds their
+// face.  Per iteration i: prd/pwr are the partial buffers of iterations i-1 and i,
+// brd/bwr the shared vertices' values before iterations i-1 and i (the two halves
+// of two double buffers, by parity); first_iter stages every vertex from pos.
+struct FusedPatchArgs {
+    const uint2* tloc;
+    const int* pvoff;
+    const int* pverts;
+    const int* pslot;
+    const int2* pshare;      // per pverts entry: (shared index, first slot | count << 28) or (-1, 0)
+    const int* inc_off;
+    const uint16_t* inc;
+    float4* pos;
+    const float4* prd;
+    float4* pwr;
+    const float4* brd;
+    float4* bwr;
+    int first_iter;
+    int npatch, maxl, nt, xcd;
+    // contacts: per patch, its faces as (solver face index, three local slots)
+    const int* pf_off;
+    const int4* pf;
+    const int* flag;
+    const double* rest;
+    const float4* lo;
+    const double *cp, *cn, *cb;
+    double* clam;
+};
+size_t jacobi_fused_smem(int maxl, bool con);
+hipError_t launch_jacobi_fused(hipStream_t s, const TetArrays& ta, const FusedPatchArgs& A, bool first, bool con);
+// shared vertex j = brd[j] + sum of prd over its slots, written to pos
+hipError_t launch_jacobi_fused_final(hipStream_t s, const int* shared, const int* soff, const float4* prd,
+                                     const float4* brd, float4* pos, int nshared);
 hipError_t launch_fill_zero(hipStream_t s, void* p, size_t bytes);
 
 // FP64 residual r = C + alpha_tilde * lambda per element, block partial sums of

@@ -344,6 +344,16 @@ int build_patches(int nv, int nt, const int32_t* tets, const double* x, const in
     }
     for (int32_t& s : P.pslot)
         if (s >= 0) s = renum[s];
+    P.pshare.assign(2 * P.pverts.size(), 0);
+    for (size_t e = 0; e < P.pverts.size(); e++) {
+        const int32_t j = shared_id[P.pverts[e]];
+        P.pshare[2 * e] = j;
+        if (j >= 0) {
+            const int32_t cnt = P.soff[j + 1] - P.soff[j];
+            if (cnt > 15 || P.soff[j] >= (1 << 28)) P.fusable = false;  // does not fit the packing
+            P.pshare[2 * e + 1] = P.soff[j] | ((cnt & 15) << 28);
+        }
+    }
     // ---- per-patch local incidence lists (local vertex -> tet corners)
     P.inc_off.clear();
     P.inc.assign((size_t)P.npatch * 4 * patch, 0);

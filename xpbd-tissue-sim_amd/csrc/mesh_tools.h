@@ -63,6 +63,9 @@ struct Patches {
     std::vector<uint32_t> tloc;
     std::vector<int32_t> inc_off;  // per patch: nl+1 offsets, stored at pvoff[p] + p
     std::vector<uint16_t> inc;     // per patch: 4*patch entries, stored at p*4*patch
+    // per pverts entry of a shared vertex: (shared index, first slot | slot count << 28); (-1, 0) otherwise
+    std::vector<int32_t> pshare;
+    bool fusable = true;  // pshare can describe every shared vertex (at most 15 patches, 2^28 slots)
     int patch = 256, npatch = 0, maxl = 0, nslots = 0;
 };
 int build_patches(int nv, int nt, const int32_t* tets, const double* x, const int32_t* tet_instance,

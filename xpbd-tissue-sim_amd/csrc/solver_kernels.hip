@@ -1167,6 +1167,214 @@ __global__ __launch_bounds__(kBlock) void k_contacts_jacobi(const int4* __restri
 }
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Fused patch Jacobi (fixed-order accumulation), one launch per iteration.
+// A shared vertex's value after the previous iteration is formed while it is
+// staged: its value before that iteration (brd, kept by the patch that holds
+// the vertex's first slot) plus the previous iteration's partial sums in slot
+// order, the same float operations as the separate shared-vertex pass.  The
+// partial sums and those values alternate between two buffers by iteration
+// parity, so no patch reads what another writes in the same launch.  Contacts
+// are projected by the patch that holds their face, from its staged positions,
+// face by face, and summed after that patch's element corrections.
+// k_jacobi_fused_final writes the shared vertices once the last iteration is done.
+// ---------------------------------------------------------------------------
+constexpr int kFusedSlots = 4;  // partial slots of a shared vertex loaded together while staging
+
+template <bool FIRST, bool CON>
+__global__ __launch_bounds__(kPatch) void k_jacobi_fused(TetArrays ta, FusedPatchArgs A)
+{
+    extern __shared__ float4 smem[];
+    const int maxl = A.maxl;
+    float4* lb = smem;                                   // base, inverse mass
+    float* ld = reinterpret_cast<float*>(smem + maxl);   // in-step motion, 3 per local vertex
+    float* red = ld + 3 * maxl;                          // 12 planes of kPatch element corrections
+    uint32_t* linc = reinterpret_cast<uint32_t*>(red + 12 * kPatch);  // the patch's incidence list
+    float* lcon = reinterpret_cast<float*>(linc + 2 * kPatch);        // CON: contact corrections, 3 per local vertex
+    float* fdx = lcon + 3 * maxl;                                      // CON: one batch of faces' corrections, 9 each
+    int* fact = reinterpret_cast<int*>(fdx + 9 * kPatch);              // CON: which faces of the batch are active
+    const int tid = threadIdx.x;
+    const int p = xcd_item(blockIdx.x, gridDim.x, A.xcd != 0);
+    const int v0 = A.pvoff[p], nl = A.pvoff[p + 1] - v0;
+    const int e = p * kPatch + tid;
+    const bool has = e < A.nt;
+    uint2 lc = make_uint2(0u, 0u);
+    float q[9];
+    float2 at = make_float2(0.f, 0.f);
+    float gm = 0.f;
+    float l[2] = { 0.0f, 0.0f };
+    if (has) {
+        lc = A.tloc[e];
+        load_tet(ta, e, q, at, gm);
+        if (!FIRST) {
+            const float2 lv = ta.lam[e];
+            l[0] = lv.x;
+            l[1] = lv.y;
+        }
+    }
+    {
+        const uint2 en = reinterpret_cast<const uint2*>(A.inc + (size_t)p * 4 * kPatch)[tid];
+        linc[2 * tid] = en.x;
+        linc[2 * tid + 1] = en.y;
+    }
+    for (int i = tid; i < nl; i += kPatch) {
+        const int g = A.pverts[v0 + i];
+        lb[i] = A.pos[2 * (size_t)g];
+        const int2 sh = A.pshare[v0 + i];
+        float4 d;
+        if (sh.x < 0 || A.first_iter) {
+            d = A.pos[2 * (size_t)g + 1];
+        } else {
+            // the first kFusedSlots partials are loaded together (predicated), then
+            // summed in slot order; rarer vertices with more slots loop over the rest
+            const int s0 = sh.y & ((1 << 28) - 1), cnt = (int)((unsigned)sh.y >> 28);
+            float4 v[kFusedSlots];
+#pragma unroll
+            for (int k = 0; k < kFusedSlots; k++) v[k] = k < cnt ? A.prd[s0 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+            d = A.brd[sh.x];
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < kFusedSlots; k++)
+                if (k < cnt) {
+                    a0 += v[k].x;
+                    a1 += v[k].y;
+                    a2 += v[k].z;
+                }
+            for (int k = kFusedSlots; k < cnt; k++) {
+                const float4 u = A.prd[s0 + k];
+                a0 += u.x;
+                a1 += u.y;
+                a2 += u.z;
+            }
+            d.x += a0;
+            d.y += a1;
+            d.z += a2;
+        }
+        // the patch holding the vertex's first slot keeps this value for the next iteration
+        if (sh.x >= 0 && A.pslot[v0 + i] == (sh.y & ((1 << 28) - 1))) A.bwr[sh.x] = d;
+        ld[3 * i] = d.x;
+        ld[3 * i + 1] = d.y;
+        ld[3 * i + 2] = d.z;
+        if (CON) lcon[3 * i] = lcon[3 * i + 1] = lcon[3 * i + 2] = 0.f;
+    }
+    __syncthreads();
+    if (has) {
+        const int sl[4] = { (int)(lc.x & 0xffffu), (int)(lc.x >> 16), (int)(lc.y & 0xffffu), (int)(lc.y >> 16) };
+        float bb[12], dd[12], w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float4 b = lb[sl[j]];
+            bb[3 * j] = b.x;
+            bb[3 * j + 1] = b.y;
+            bb[3 * j + 2] = b.z;
+            w[j] = b.w;
+#pragma unroll
+            for (int k = 0; k < 3; k++) dd[3 * j + k] = ld[3 * sl[j] + k];
+        }
+        float dx[12];
+        project_tet_split<float>(bb, dd, w, q, at.x, at.y, gm, l, dx);
+        ta.lam[e] = make_float2(l[0], l[1]);
+#pragma unroll
+        for (int k = 0; k < 12; k++) red[k * kPatch + tid] = dx[k];
+    }
+    if (CON) {
+        // the patch's surface faces, a batch of kPatch at a time: each thread
+        // projects one face in FP64 on the staged state, then thread 0 adds the
+        // batch's corrections in face order (a fixed summation order)
+        const int f0 = A.pf_off[p], f1 = A.pf_off[p + 1];
+        for (int fb = f0; fb < f1; fb += kPatch) {
+            const int f = fb + tid;
+            bool act = false;
+            if (f < f1) {
+                const int4 F = A.pf[f];  // (face, local slots of its three vertices)
+                if (A.flag[F.x]) {
+                    const int ls[3] = { F.y, F.z, F.w };
+                    double x[3][3];
+                    float w[3];
+#pragma unroll
+                    for (int m = 0; m < 3; m++) {
+                        const int g = A.pverts[v0 + ls[m]];
+                        const float4 b = lb[ls[m]];
+                        const float4 lo = A.lo[g];
+                        x[m][0] = A.rest[3 * (size_t)g] + (((double)b.x + (double)lo.x) + (double)ld[3 * ls[m]]);
+                        x[m][1] = A.rest[3 * (size_t)g + 1] + (((double)b.y + (double)lo.y) + (double)ld[3 * ls[m] + 1]);
+                        x[m][2] = A.rest[3 * (size_t)g + 2] + (((double)b.z + (double)lo.z) + (double)ld[3 * ls[m] + 2]);
+                        w[m] = b.w;
+                    }
+                    const double* xs[3] = { x[0], x[1], x[2] };
+                    double lam = FIRST ? 0.0 : A.clam[F.x];
+                    double cdx[9];
+                    act = contact_dx(xs, w, A.cp + 3 * (size_t)F.x, A.cn + 3 * (size_t)F.x, A.cb + 3 * (size_t)F.x,
+                                     &lam, cdx);
+                    if (act)
+                        for (int k = 0; k < 9; k++) fdx[9 * tid + k] = (float)cdx[k];
+                    A.clam[F.x] = lam;
+                }
+            }
+            fact[tid] = act ? 1 : 0;
+            __syncthreads();
+            if (tid == 0)
+                for (int t = 0; t < min(kPatch, f1 - fb); t++) {
+                    if (!fact[t]) continue;
+                    const int4 F = A.pf[fb + t];
+                    const int ls[3] = { F.y, F.z, F.w };
+                    for (int m = 0; m < 3; m++)
+                        for (int k = 0; k < 3; k++) lcon[3 * ls[m] + k] += fdx[9 * t + 3 * m + k];
+                }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    const int ioff = v0 + p;  // this patch's nl+1 incidence offsets
+    for (int i = tid; i < nl; i += kPatch) {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        const int s0 = A.inc_off[ioff + i], s1 = A.inc_off[ioff + i + 1];
+        const uint16_t* il = reinterpret_cast<const uint16_t*>(linc);
+        for (int s = s0; s < s1; s++) {
+            const int ent = il[s];
+            const int t = ent >> 2, j = ent & 3;
+            a0 += red[(3 * j) * kPatch + t];
+            a1 += red[(3 * j + 1) * kPatch + t];
+            a2 += red[(3 * j + 2) * kPatch + t];
+        }
+        if (CON) {
+            a0 += lcon[3 * i];
+            a1 += lcon[3 * i + 1];
+            a2 += lcon[3 * i + 2];
+        }
+        const int g = A.pverts[v0 + i];
+        const int slot = A.pslot[v0 + i];
+        if (slot < 0)
+            A.pos[2 * (size_t)g + 1] = make_float4(ld[3 * i] + a0, ld[3 * i + 1] + a1, ld[3 * i + 2] + a2, 0.f);
+        else
+            A.pwr[slot] = make_float4(a0, a1, a2, 0.f);
+    }
+}
+
+// After the last fused iteration: shared vertex j = its value before that
+// iteration + that iteration's partial sums in slot order.
+__global__ __launch_bounds__(kBlock) void k_jacobi_fused_final(const int* __restrict__ shared,
+                                                              const int* __restrict__ soff,
+                                                              const float4* __restrict__ prd,
+                                                              const float4* __restrict__ brd,
+                                                              float4* __restrict__ pos, int ns)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= ns) return;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int k = soff[j]; k < soff[j + 1]; k++) {
+        const float4 v = prd[k];
+        a0 += v.x;
+        a1 += v.y;
+        a2 += v.z;
+    }
+    float4 d = brd[j];
+    d.x += a0;
+    d.y += a1;
+    d.z += a2;
+    pos[2 * (size_t)shared[j] + 1] = d;
+}
+
 // Attachments (AttachmentConstraint.hpp:31-71 with ConstraintProjector.hpp:111-172,
 // alpha = 0): vertex v pulled to the point p, C = |x - p|, gradient
 // (x - p) / |x - p|; no correction below |x - p| = 1e-12, where the constraint
@@ -1519,6 +1727,31 @@ hipError_t launch_jacobi_patch(hipStream_t s, const TetArrays& ta, const PatchAr
     else
         hipLaunchKernelGGL(k_jacobi_shared<false>, dim3(nblk(pa.nshared, kBlock)), dim3(kBlock), 0, s, pa.shared,
                            pa.soff, pa.partial, cacc, pos, pa.nshared);
+    return hipGetLastError();
+}
+
+size_t jacobi_fused_smem(int maxl, bool con)
+{
+    return jacobi_patch_smem(maxl, true) + (con ? (size_t)maxl * 12 + (size_t)kPatch * 40 : 0);
+}
+
+hipError_t launch_jacobi_fused(hipStream_t s, const TetArrays& ta, const FusedPatchArgs& A, bool first, bool con)
+{
+    if (A.npatch <= 0) return hipSuccess;
+    const size_t sm = jacobi_fused_smem(A.maxl, con);
+    if (first && con) hipLaunchKernelGGL((k_jacobi_fused<true, true>), dim3(A.npatch), dim3(kPatch), sm, s, ta, A);
+    else if (first) hipLaunchKernelGGL((k_jacobi_fused<true, false>), dim3(A.npatch), dim3(kPatch), sm, s, ta, A);
+    else if (con) hipLaunchKernelGGL((k_jacobi_fused<false, true>), dim3(A.npatch), dim3(kPatch), sm, s, ta, A);
+    else hipLaunchKernelGGL((k_jacobi_fused<false, false>), dim3(A.npatch), dim3(kPatch), sm, s, ta, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_fused_final(hipStream_t s, const int* shared, const int* soff, const float4* prd,
+                                     const float4* brd, float4* pos, int nshared)
+{
+    if (nshared <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jacobi_fused_final, dim3(nblk(nshared, kBlock)), dim3(kBlock), 0, s, shared, soff, prd, brd,
+                       pos, nshared);
     return hipGetLastError();
 }
 
