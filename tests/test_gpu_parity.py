@@ -332,3 +332,158 @@ def test_fused_patch_jacobi_matches_the_three_launch_iteration(tmp_path):
     xb, lb = run(False)
     np.testing.assert_array_equal(xa, xb)
     np.testing.assert_array_equal(la, lb)
+
+
+def _uneven_batch(n, E_range, seed, kind=X.SDF_SPHERE, pen=0.1):
+    """n independent cubes of 3..8 cells per axis, each with its own collider
+    reaching into its top: a sphere off the grid's symmetry lines, or a randomly
+    posed box."""
+    rng = np.random.default_rng(seed)
+    meshes, sdfs = [], []
+    for _ in range(n):
+        nx, ny, nz = (int(v) for v in rng.integers(3, 9, size=3))
+        E = float(np.exp(rng.uniform(np.log(E_range[0]), np.log(E_range[1]))))
+        m = W.grid_object(nx, ny, nz, 0.01, E=E, nu=float(rng.uniform(0.45, 0.49)), pin_axis=2, noise=0.0,
+                          faces=True, rng=rng)
+        ext = np.array([nx, ny, nz]) * 0.01
+        meshes.append(m)
+        if kind == X.SDF_SPHERE:
+            r = 0.6 * ext.min()
+            u, v = rng.uniform(0.3, 0.7, size=2)
+            sdfs.append((X.SDF_SPHERE, (ext[0] * u, ext[1] * v, ext[2] + r - pen * ext[2]), (0, 0, 0, 1), (r, 0, 0)))
+        else:
+            size = np.full(3, 0.5 * ext.min())
+            sdfs.append((X.SDF_BOX, (ext[0] / 2, ext[1] / 2, ext[2] * 0.9 + 0.5 * size[0]),
+                         tuple(W.random_quat(rng)), tuple(size)))
+    return meshes, sdfs
+
+
+@pytest.mark.parametrize("solver,variant,E_range,pen,g", [("gs", "quad", (1e3, 5e4), 0.1, 9.81),
+                                                          ("gs", "cluster", (1e3, 5e4), 0.1, 9.81),
+                                                          ("jacobi", "patch", (1e3, 3e3), 0.05, 3.0)])
+def test_batch_of_uneven_instances_matches_each_instance_alone(solver, variant, E_range, pen, g):
+    """One batched handle (every launch covers all instances, each instance its own
+    sphere collider) against an oracle per instance, over 60 steps with contacts
+    re-detected every step; Gauss-Seidel oracles follow each instance's part of
+    the batch's element and face order.  Spheres off the grid's symmetry lines
+    keep the Frank-Wolfe detection away from ties: with randomly posed boxes the
+    detection of a few faces flips under the 1e-8 state difference between the
+    FP32 and FP64 trajectories (for one of these instances the FP64 oracle moves
+    by 7e-2 under a 1e-10 perturbation of its own state), while on identical
+    positions the device's detection equals the oracle's
+    (test_batch_box_detection_equals_oracle_on_the_same_positions).  Jacobi runs
+    in a regime where the reference's Jacobi is well conditioned under contact
+    (the FP64 oracle moves by 1.4e-8 under a 1e-11 perturbation; softer objects
+    pressed deeper make it diverge)."""
+    meshes, sdfs = _uneven_batch(6, E_range, seed=11, pen=pen)
+    gpu = X.Solver(meshes, dt=1e-3, g=g, iters=10, solver=solver, variant=variant, collision_every=1)
+    for i, (kind, pos, quat, prm) in enumerate(sdfs):
+        gpu.set_sdf(kind, pos, quat, prm, instance=i)
+    order, rank = gpu.tet_order(), gpu.face_rank()
+    orcs, t0, f0 = [], 0, 0
+    for m, (kind, pos, quat, prm) in zip(meshes, sdfs):
+        o = orc.Sim(m, dt=1e-3, g=g, iters=10, mode="gs" if solver == "gs" else "jacobi")
+        o.set_sdf(kind, pos, quat, prm)
+        if solver == "gs":
+            o.set_order(order[(order >= t0) & (order < t0 + m.nt)] - t0)
+            o.set_contact_order(rank[f0:f0 + len(m.faces)])
+        orcs.append(o)
+        t0 += m.nt
+        f0 += len(m.faces)
+    errs, contacts = [], 0
+    for _ in range(60):
+        gpu.step(1)
+        for o in orcs:
+            o.step(1, detect=True)
+            contacts += o.num_contacts() if hasattr(o, "num_contacts") else len(o.contacts()[0])
+        errs.append(max(rel_err(gpu.instance_positions(i), o.x()) for i, o in enumerate(orcs)))
+    print(f"batch {solver}/{variant}: {len(meshes)} instances, {gpu.nt} tets, max rel err {max(errs):.3e}, "
+          f"oracle contacts over the run {contacts}")
+    assert contacts > 0
+    assert max(errs) <= TOL
+
+
+def test_box_collider_matches_oracle():
+    """A randomly posed box pressed into a cube (the batch workload's collider),
+    single instance, quad GS, against the oracle with the same face order."""
+    meshes, sdfs = _uneven_batch(3, (1e3, 5e4), seed=11, kind=X.SDF_BOX)
+    for m, (kind, pos, quat, prm) in zip(meshes, sdfs):
+        gpu = X.Solver(m, dt=1e-3, iters=10, solver="gs", variant="quad", collision_every=1)
+        gpu.set_sdf(kind, pos, quat, prm)
+        o = orc.Sim(m, dt=1e-3, iters=10, mode="gs")
+        o.set_sdf(kind, pos, quat, prm)
+        o.set_order(gpu.tet_order())
+        o.set_contact_order(gpu.face_rank())
+        errs, contacts = [], 0
+        for _ in range(20):
+            gpu.step(1)
+            o.step(1, detect=True)
+            contacts += len(o.contacts()[0])
+            errs.append(rel_err(gpu.positions(), o.x()))
+        print(f"box on {m.nt} tets: max rel err {max(errs):.3e}, oracle contacts {contacts}")
+        assert contacts > 0
+        assert max(errs) <= TOL
+
+
+def test_batch_box_detection_equals_oracle_on_the_same_positions():
+    """Contact detection of a batch with randomly posed box colliders: after each
+    step, the oracle's face-vs-SDF detection run on the device's own positions
+    (which the FP64 read-back reproduces exactly) finds the same faces and the
+    same contact points, normals and barycentric coordinates to rounding."""
+    meshes, sdfs = _uneven_batch(6, (1e3, 5e4), seed=11, kind=X.SDF_BOX)
+    gpu = X.Solver(meshes, dt=1e-3, iters=10, solver="gs", variant="quad", collision_every=1)
+    for i, (kind, pos, quat, prm) in enumerate(sdfs):
+        gpu.set_sdf(kind, pos, quat, prm, instance=i)
+    foffs = np.cumsum([0] + [len(m.faces) for m in meshes])
+    total = 0
+    for _ in range(8):
+        xs = [gpu.instance_positions(i) for i in range(len(meshes))]
+        gpu.step(1)  # detects from xs, then solves
+        gf, gp, gn, gb = gpu.contacts()
+        for i, (m, (kind, pos, quat, prm)) in enumerate(zip(meshes, sdfs)):
+            o = orc.Sim(m, dt=1e-3, iters=10, mode="gs")
+            o.set_sdf(kind, pos, quat, prm)
+            o.set_state(xs[i])
+            o.detect()
+            of, op, on, ob = o.contacts()
+            sel = (gf >= foffs[i]) & (gf < foffs[i + 1])
+            assert sorted((of + foffs[i]).tolist()) == sorted(gf[sel].tolist())
+            for c, f in enumerate(of):
+                g = int(np.where(gf == f + foffs[i])[0][0])
+                np.testing.assert_allclose(gp[g], op[c], rtol=0, atol=1e-14)
+                np.testing.assert_allclose(gn[g], on[c], rtol=0, atol=1e-13)
+                np.testing.assert_allclose(gb[g], ob[c], rtol=0, atol=1e-12)
+            total += len(of)
+    assert total > 0
+
+
+def _single_tet():
+    x0 = np.array([[0.0, 0.0, 0.0], [0.01, 0.0, 0.0], [0.0, 0.01, 0.0], [0.0, 0.0, 0.01]])
+    v0 = np.array([[0.0, 0.0, 0.0], [0.05, 0.0, 0.0], [0.0, -0.03, 0.0], [0.0, 0.0, 0.02]])
+    return X.Mesh(x0=x0, tets=np.array([[0, 1, 2, 3]], np.int32), E=np.array([3e3]), nu=np.array([0.45]),
+                  rho=np.array([1000.0]), v0=v0)
+
+
+@pytest.mark.parametrize("solver,variant,det", VARIANTS)
+@pytest.mark.parametrize("case", ["single tet", "ragged 3x5x7", "no iterations", "all pinned"])
+def test_edge_cases_match_oracle(case, solver, variant, det):
+    """The smallest mesh (one element, moving vertices), sizes that fill no patch,
+    cluster chunk or colour evenly, a solve with zero iterations (free flight),
+    and a fully pinned object (nothing moves), for every kernel variant."""
+    iters = 10
+    if case == "single tet":
+        mesh = _single_tet()
+    elif case == "ragged 3x5x7":
+        mesh = W.grid_object(3, 5, 7, 0.01, E=3e3, nu=0.45, pin_axis=1, noise=0.02, seed=5)
+    elif case == "no iterations":
+        mesh, iters = W.cube(4, seed=6), 0
+    else:
+        mesh = W.grid_object(3, 3, 3, 0.01, E=3e3, nu=0.45, pin_axis=None, noise=0.02, seed=7)
+        mesh.pinned = np.ones(mesh.nv, np.uint8)
+    gpu, o = _pair(mesh, solver, iters, variant=variant, det=det)
+    errs = run_pair(gpu, o, 50)
+    print(f"{case} {solver}/{variant}/det{det}: max rel err {errs.max():.3e}")
+    assert np.all(np.isfinite(errs))
+    assert errs.max() <= TOL
+    if case == "all pinned":
+        np.testing.assert_array_equal(gpu.positions(), mesh.x0)
