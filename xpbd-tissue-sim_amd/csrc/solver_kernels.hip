@@ -277,11 +277,11 @@ __global__ __launch_bounds__(kClusterBlock) void k_gs_cluster(TetArrays ta, cons
 
 __device__ __forceinline__ float dpp_xor1(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float dpp_xor2(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float quad_sum(float x)
 {
@@ -290,18 +290,21 @@ __device__ __forceinline__ float quad_sum(float x)
 }
 __device__ __forceinline__ float quad_lane3(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xFF, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xFF, 0xF, 0xF, true));
 }
 
 // BLOCK threads per workgroup = BLOCK / 4 clusters; the launch shape is a tuning choice.
 template <int BLOCK>
 struct QuadLds {
     static constexpr int kClusters = BLOCK / 4;
-    static constexpr int kStage = kClusters * 8;  // at most 8 tets per cluster
+    static constexpr int kMaxTets = 8;            // per cluster
+    static constexpr int kStage = kClusters * kMaxTets;
     static constexpr int kCol = kClusters + 1;    // padded column stride (LDS banks)
     float sb[8 * 4 * kCol];    // [slot][xyz, w][cluster]
     float sd[8 * 3 * kCol];    // [slot][xyz][cluster]
-    float sq[12][kStage];      // Q (9), alpha tilde (2), gamma
+    // row 3 j + c: Q(j, c), the gradient weights of element vertex j (vertex 3:
+    // -(Q(0, c) + Q(1, c) + Q(2, c))); rows 12, 13: alpha tilde; row 14: gamma
+    float sq[15][kStage];
     float sl[2][kStage];       // lambda
     uint32_t stl[kStage];      // local vertex slots
 };
@@ -317,6 +320,7 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
     constexpr int kQuadClusters = QuadLds<BLOCK>::kClusters;
     constexpr int kQuadStage = QuadLds<BLOCK>::kStage;
     constexpr int kQuadCol = QuadLds<BLOCK>::kCol;
+    constexpr int kMaxTets = QuadLds<BLOCK>::kMaxTets;
     float* sb = L.sb;
     float* sd = L.sd;
     auto& sq = L.sq;
@@ -378,8 +382,18 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
     for (int r = 0; r < kPer; r++) {
         const int i = tid + r * kQuadBlock;
         if (i < n) {
+            // Q is column-major (eq[3 c + r] = Q(r, c)); row 3 j + c of sq holds Q(j, c)
 #pragma unroll
-            for (int k = 0; k < 12; k++) sq[k][i] = eq[r][k];
+            for (int cc = 0; cc < 3; cc++) {
+                const float q0 = eq[r][3 * cc], q1 = eq[r][3 * cc + 1], q2 = eq[r][3 * cc + 2];
+                sq[cc][i] = q0;
+                sq[3 + cc][i] = q1;
+                sq[6 + cc][i] = q2;
+                sq[9 + cc][i] = -(q0 + q1 + q2);
+            }
+            sq[12][i] = eq[r][9];
+            sq[13][i] = eq[r][10];
+            sq[14][i] = eq[r][11];
             if (!FIRST) {
                 sl[0][i] = el[r][0];
                 sl[1][i] = el[r][1];
@@ -407,76 +421,104 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
         sd[(q * 3 + 2) * kQuadCol + cl] = dhi.z;
     }
     __syncthreads();
-    // a quad is active or inactive as a whole, so DPP reads stay inside active quads;
-    // an inactive quad has an empty element range
-    const int t0 = active ? cs0 - tb : 0, t1 = active ? cs1 - tb : 0;
-    for (int e = t0; e < t1; e++) {
-        const int slot = (int)((stl[e] >> (4 * j)) & 15u);
-        float bb[3], dd[3];
+    // A quad is active or inactive as a whole (an inactive quad has no
+    // elements), so DPP reads stay inside active quads.
+    const int t0 = active ? cs0 - tb : 0;
+    const int nte = active ? cs1 - cs0 : 0;   // at most kMaxTets
+    // What stays fixed during the solve is read once per element, before the
+    // dependent chain: this lane's slot (as its LDS offset in sd), the base part
+    // of its row of D = (b_j - b_3) + (d_j - d_3), and its inverse mass.
+    int soff[kMaxTets];
+    float Db[kMaxTets][3], wv[kMaxTets];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            bb[k] = sb[(slot * 4 + k) * kQuadCol + cl];
-            dd[k] = sd[(slot * 3 + k) * kQuadCol + cl];
+    for (int k = 0; k < kMaxTets; k++) {
+        soff[k] = cl;
+        Db[k][0] = Db[k][1] = Db[k][2] = 0.f;
+        wv[k] = 0.f;
+        if (k < nte) {
+            const int slot = (int)((stl[t0 + k] >> (4 * j)) & 15u);
+            soff[k] = slot * 3 * kQuadCol + cl;
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) {
+                const float bb = sb[(slot * 4 + kk) * kQuadCol + cl];
+                Db[k][kk] = bb - quad_lane3(bb);
+            }
+            wv[k] = sb[(slot * 4 + 3) * kQuadCol + cl];
         }
-        const float w = sb[(slot * 4 + 3) * kQuadCol + cl];
-        float q[9];
+    }
 #pragma unroll
-        for (int k = 0; k < 9; k++) q[k] = sq[k][e];
-        // column j of [Q ; -(Q_0 + Q_1 + Q_2)] (row i of Q multiplies vertex i)
-        float qj[3];
+    for (int k = 0; k < kMaxTets; k++) {
+        if (k < nte) {
+            const int e = t0 + k;
+            const int so = soff[k];
+            const float w = wv[k];
+            // every LDS read of the step is independent of the others: one round trip
+            float dd[3], qj[3];
 #pragma unroll
-        for (int cc = 0; cc < 3; cc++) {
-            const float q0 = q[3 * cc], q1 = q[3 * cc + 1], q2 = q[3 * cc + 2];
-            qj[cc] = j == 0 ? q0 : j == 1 ? q1 : j == 2 ? q2 : -(q0 + q1 + q2);
+            for (int kk = 0; kk < 3; kk++) dd[kk] = sd[so + kk * kQuadCol];
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) qj[cc] = sq[3 * j + cc][e];
+            const float gamma = sq[14][e];
+            const float atD = sq[12][e], atH = sq[13][e];
+            const float l0 = FIRST ? 0.f : sl[0][e], l1 = FIRST ? 0.f : sl[1][e];
+            // this lane's row of D (zero for vertex 3)
+            float D[3];
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) D[kk] = Db[k][kk] + (dd[kk] - quad_lane3(dd[kk]));
+            float F[9];
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    const float sum = quad_sum(D[r] * qj[cc]);
+                    F[3 * cc + r] = r == cc ? sum + 1.f : sum;
+                }
+            // C_D, C_H as element_constraints, with both branches of C_H evaluated
+            float ss = 0.f;
+#pragma unroll
+            for (int i = 0; i < 9; i++) ss += F[i] * F[i];
+            const float CD = xsqrt<float>(ss);
+            const float J = det3cm<float>(F);
+            const float a = J - 1.f;
+            const float ch_log = -gamma + xlog<float>(J);
+            const float ch_series = -gamma + a - a * a * 0.5f + a * a * a * (1.f / 3.f);
+            const float CH = J >= 1.f ? ch_log : ch_series;
+            const float inv_C = xrcp<float>(CD);
+            const float fac = (J >= 1.f) ? xrcp<float>(J) : 1.f - a + a * a;
+            float Fc[9];
+            Fc[0] = F[4] * F[8] - F[5] * F[7];
+            Fc[1] = F[5] * F[6] - F[3] * F[8];
+            Fc[2] = F[3] * F[7] - F[4] * F[6];
+            Fc[3] = F[7] * F[2] - F[8] * F[1];
+            Fc[4] = F[8] * F[0] - F[6] * F[2];
+            Fc[5] = F[6] * F[1] - F[7] * F[0];
+            Fc[6] = F[1] * F[5] - F[2] * F[4];
+            Fc[7] = F[2] * F[3] - F[0] * F[5];
+            Fc[8] = F[0] * F[4] - F[1] * F[3];
+            float gD[3], gH[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                gD[r] = inv_C * (F[r] * qj[0] + F[3 + r] * qj[1] + F[6 + r] * qj[2]);
+                gH[r] = fac * (Fc[r] * qj[0] + Fc[3 + r] * qj[1] + Fc[6 + r] * qj[2]);
+            }
+            const float s00 = quad_sum(w * (gD[0] * gD[0] + gD[1] * gD[1] + gD[2] * gD[2]));
+            const float s01 = quad_sum(w * (gD[0] * gH[0] + gD[1] * gH[1] + gD[2] * gH[2]));
+            const float s11 = quad_sum(w * (gH[0] * gH[0] + gH[1] * gH[1] + gH[2] * gH[2]));
+            const float A00 = atD + s00, A11 = atH + s11, A01 = s01;
+            const float b0 = -CD - atD * l0;
+            const float b1 = -CH - atH * l1;
+            const float inv_det = xrcp<float>(A00 * A11 - A01 * A01);
+            const float d0 = (b0 * A11 - b1 * A01) * inv_det;
+            const float d1 = (b1 * A00 - b0 * A01) * inv_det;
+            // the four lanes hold the same multipliers and store the same value
+            sl[0][e] = l0 + d0;
+            sl[1][e] = l1 + d1;
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) sd[so + kk * kQuadCol] = dd[kk] + w * (gD[kk] * d0 + gH[kk] * d1);
+            // the next element's reads of other lanes' slots follow these writes in
+            // program order (LDS operations of one wave complete in order)
+            __builtin_amdgcn_wave_barrier();
         }
-        // this lane's row of D (zero for vertex 3)
-        float D[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) D[k] = (bb[k] - quad_lane3(bb[k])) + (dd[k] - quad_lane3(dd[k]));
-        const float dsel = j == 3 ? 0.f : 1.f;
-        float F[9];
-#pragma unroll
-        for (int cc = 0; cc < 3; cc++)
-#pragma unroll
-            for (int r = 0; r < 3; r++) F[3 * cc + r] = quad_sum(dsel * D[r] * qj[cc]) + (r == cc ? 1.f : 0.f);
-        float CD, CH, J;
-        element_constraints<float>(F, sq[11][e], &CD, &CH, &J);
-        const float inv_C = xrcp<float>(CD);
-        const float a = J - 1.f;
-        const float fac = (J >= 1.f) ? xrcp<float>(J) : 1.f - a + a * a;
-        float Fc[9];
-        Fc[0] = F[4] * F[8] - F[5] * F[7];
-        Fc[1] = F[5] * F[6] - F[3] * F[8];
-        Fc[2] = F[3] * F[7] - F[4] * F[6];
-        Fc[3] = F[7] * F[2] - F[8] * F[1];
-        Fc[4] = F[8] * F[0] - F[6] * F[2];
-        Fc[5] = F[6] * F[1] - F[7] * F[0];
-        Fc[6] = F[1] * F[5] - F[2] * F[4];
-        Fc[7] = F[2] * F[3] - F[0] * F[5];
-        Fc[8] = F[0] * F[4] - F[1] * F[3];
-        float gD[3], gH[3];
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            gD[r] = inv_C * (F[r] * qj[0] + F[3 + r] * qj[1] + F[6 + r] * qj[2]);
-            gH[r] = fac * (Fc[r] * qj[0] + Fc[3 + r] * qj[1] + Fc[6 + r] * qj[2]);
-        }
-        const float s00 = quad_sum(w * (gD[0] * gD[0] + gD[1] * gD[1] + gD[2] * gD[2]));
-        const float s01 = quad_sum(w * (gD[0] * gH[0] + gD[1] * gH[1] + gD[2] * gH[2]));
-        const float s11 = quad_sum(w * (gH[0] * gH[0] + gH[1] * gH[1] + gH[2] * gH[2]));
-        const float atD = sq[9][e], atH = sq[10][e];
-        const float l0 = FIRST ? 0.f : sl[0][e], l1 = FIRST ? 0.f : sl[1][e];
-        const float A00 = atD + s00, A11 = atH + s11, A01 = s01;
-        const float b0 = -CD - atD * l0;
-        const float b1 = -CH - atH * l1;
-        const float inv_det = xrcp<float>(A00 * A11 - A01 * A01);
-        const float d0 = (b0 * A11 - b1 * A01) * inv_det;
-        const float d1 = (b1 * A00 - b0 * A01) * inv_det;
-        if (j == 0) ta.lam[tb + e] = make_float2(l0 + d0, l1 + d1);
-#pragma unroll
-        for (int k = 0; k < 3; k++) sd[(slot * 3 + k) * kQuadCol + cl] = dd[k] + w * (gD[k] * d0 + gH[k] * d1);
-        // the next element's reads of other lanes' slots follow these writes in
-        // program order (LDS operations of one wave complete in order)
-        __builtin_amdgcn_wave_barrier();
     }
     if (v_lo >= 0)
         pos[2 * (size_t)v_lo + 1] = make_float4(sd[(j * 3 + 0) * kQuadCol + cl], sd[(j * 3 + 1) * kQuadCol + cl],
@@ -485,6 +527,13 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
         const int qs = j + 4;
         pos[2 * (size_t)v_hi + 1] = make_float4(sd[(qs * 3 + 0) * kQuadCol + cl], sd[(qs * 3 + 1) * kQuadCol + cl],
                                                 sd[(qs * 3 + 2) * kQuadCol + cl], 0.f);
+    }
+    // multipliers of the workgroup's whole tet range, coalesced
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int i = tid + r * kQuadBlock;
+        if (i < n) ta.lam[tb + i] = make_float2(sl[0][i], sl[1][i]);
     }
 }
 
