@@ -608,6 +608,42 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_apply(float4* __restrict__ po
 // ---------------------------------------------------------------------------
 constexpr int kPatch = 256;
 
+// Sum the corrections of one local vertex over its incidence entries s0..s1-1
+// (entry = 4 * local tet + corner; plane k of red holds component k of every
+// tet's corrections) in entry order.  Four entries at a time: their indices,
+// then their values, are read together, and added in the same order.
+__device__ __forceinline__ void gather_incidence(const uint16_t* il, const float* red, int s0, int s1, float& a0,
+                                                 float& a1, float& a2)
+{
+    int s = s0;
+    for (; s + 4 <= s1; s += 4) {
+        int off[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int ent = il[s + u];
+            off[u] = (3 * (ent & 3)) * kPatch + (ent >> 2);
+        }
+        float r[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) r[u][k] = red[off[u] + k * kPatch];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a0 += r[u][0];
+            a1 += r[u][1];
+            a2 += r[u][2];
+        }
+    }
+    for (; s < s1; s++) {
+        const int ent = il[s];
+        const int off = (3 * (ent & 3)) * kPatch + (ent >> 2);
+        a0 += red[off];
+        a1 += red[off + kPatch];
+        a2 += red[off + 2 * kPatch];
+    }
+}
+
 template <bool FIRST, bool DET, bool CON>
 __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uint2* __restrict__ tloc,
                                                         const int* __restrict__ pvoff, const int* __restrict__ pverts,
@@ -649,8 +685,21 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
         linc[2 * tid] = e.x;
         linc[2 * tid + 1] = e.y;
     }
+    // this thread's first local vertex: its id, slot and incidence range are
+    // loaded with the staging, so the write-back phase starts without a global
+    // round trip
+    const int ioff = v0 + p;  // this patch's nl+1 incidence offsets
+    int g_first = 0, slot_first = 0, s0_first = 0, s1_first = 0;
+    if (tid < nl) {
+        slot_first = pslot[v0 + tid];
+        if (DET) {
+            s0_first = inc_off[ioff + tid];
+            s1_first = inc_off[ioff + tid + 1];
+        }
+    }
     for (int i = tid; i < nl; i += kPatch) {
         const int g = pverts[v0 + i];
+        if (i == tid) g_first = g;
         lb[i] = pos[2 * (size_t)g];
         const float4 d = pos[2 * (size_t)g + 1];
         ld[3 * i] = d.x;
@@ -686,27 +735,25 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_patch(TetArrays ta, const uin
         }
     }
     __syncthreads();
-    const int ioff = v0 + p;  // this patch's nl+1 incidence offsets
+    int g = g_first, slot = slot_first, s0 = s0_first, s1 = s1_first;
     for (int i = tid; i < nl; i += kPatch) {
+        if (i != tid) {
+            g = pverts[v0 + i];
+            slot = pslot[v0 + i];
+            if (DET) {
+                s0 = inc_off[ioff + i];
+                s1 = inc_off[ioff + i + 1];
+            }
+        }
         float a0, a1, a2;
         if (DET) {
             a0 = a1 = a2 = 0.f;
-            const int s0 = inc_off[ioff + i], s1 = inc_off[ioff + i + 1];
-            const uint16_t* il = reinterpret_cast<const uint16_t*>(linc);
-            for (int s = s0; s < s1; s++) {
-                const int ent = il[s];
-                const int t = ent >> 2, j = ent & 3;
-                a0 += red[(3 * j) * kPatch + t];
-                a1 += red[(3 * j + 1) * kPatch + t];
-                a2 += red[(3 * j + 2) * kPatch + t];
-            }
+            gather_incidence(reinterpret_cast<const uint16_t*>(linc), red, s0, s1, a0, a1, a2);
         } else {
             a0 = red[3 * i];
             a1 = red[3 * i + 1];
             a2 = red[3 * i + 2];
         }
-        const int g = pverts[v0 + i];
-        const int slot = pslot[v0 + i];
         if (slot < 0) {
             if (CON) {
                 const float4 c = cacc[g];
@@ -731,8 +778,22 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_shared(const int* __restrict_
     if (j >= ns) return;
     const int g = shared[j];
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-    // the vertex's slots are contiguous, in patch order (fixed summation order)
-    for (int k = soff[j]; k < soff[j + 1]; k++) {
+    // the vertex's slots are contiguous, in patch order (fixed summation order);
+    // four loads are issued together
+    const int k0 = soff[j], k1 = soff[j + 1];
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = partial[k + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a0 += v[u].x;
+            a1 += v[u].y;
+            a2 += v[u].z;
+        }
+    }
+    for (; k < k1; k++) {
         const float4 v = partial[k];
         a0 += v.x;
         a1 += v.y;
@@ -1378,14 +1439,7 @@ __global__ __launch_bounds__(kPatch) void k_jacobi_fused(TetArrays ta, FusedPatc
     for (int i = tid; i < nl; i += kPatch) {
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
         const int s0 = A.inc_off[ioff + i], s1 = A.inc_off[ioff + i + 1];
-        const uint16_t* il = reinterpret_cast<const uint16_t*>(linc);
-        for (int s = s0; s < s1; s++) {
-            const int ent = il[s];
-            const int t = ent >> 2, j = ent & 3;
-            a0 += red[(3 * j) * kPatch + t];
-            a1 += red[(3 * j + 1) * kPatch + t];
-            a2 += red[(3 * j + 2) * kPatch + t];
-        }
+        gather_incidence(reinterpret_cast<const uint16_t*>(linc), red, s0, s1, a0, a1, a2);
         if (CON) {
             a0 += lcon[3 * i];
             a1 += lcon[3 * i + 1];
