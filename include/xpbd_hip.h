@@ -254,7 +254,8 @@ int xpbd_batch_get_positions(xpbd_batch* b, int32_t instance, double* out);
  * cluster kernel and the quad kernel at 64/128/256 threads per workgroup;
  * the patch Jacobi kernel with and without the fused one-launch iteration;
  * cluster and patch kernels each with and without the XCD-contiguous
- * workgroup mapping), restore the state, and keep the fastest.  The choice is
+ * workgroup mapping) as replays of a captured graph of the handle's
+ * iterations, restore the state, and keep the fastest.  The choice is
  * appended to cache_path (may be NULL) as
  * "shape-key variant block xcd ms-per-iteration" and reused from there for the
  * same shape.  No reference counterpart: the reference has no kernel variants. */

@@ -9,8 +9,9 @@
 // time the fused one-launch iteration against the patch + shared-vertex +
 // contact launches (the same sums; with contacts the contact corrections are
 // added in another order).  Cluster and patch kernels are also timed with and
-// without the XCD-contiguous workgroup mapping (which does not change results).  Other handles have a single candidate; the
-// call then only records it.
+// without the XCD-contiguous workgroup mapping (which does not change results).
+// Each candidate's iterations are captured into a HIP graph and its replays are
+// timed.  Other handles have a single candidate; the call then only records it.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -143,9 +144,29 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
         if (cd.variant == XPBD_VARIANT_PATCH) h->patch_fused = cd.block == 1;
         h->xcd_map = cd.xcd != 0;
         double ms_best = 1e300;
+        // the candidate's iterations are captured into a graph and the replays are
+        // timed, as the solver replays its steps: launches issued one by one from
+        // the host would time the host where its per-launch cost exceeds a kernel's
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        e = hipStreamSynchronize(s);
+        if (e == hipSuccess) e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            rc = hip_err(h, e, "autotune capture");
+            break;
+        }
+        rc = enqueue_iterations(h, iters, false);
+        e = hipStreamEndCapture(s, &g);
+        if (!rc && e != hipSuccess) rc = hip_err(h, e, "autotune capture");
+        if (!rc) {
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = hip_err(h, e, "autotune graph");
+        }
+        if (g) (void)hipGraphDestroy(g);
         for (int rep = 0; rep < 4 && !rc; rep++) {  // first repetition warms up
             (void)hipEventRecord(e0, s);
-            rc = enqueue_iterations(h, iters, false);
+            e = hipGraphLaunch(ge, s);
+            if (e != hipSuccess) rc = hip_err(h, e, "autotune replay");
             (void)hipEventRecord(e1, s);
             if (!rc) {
                 e = hipEventSynchronize(e1);
@@ -158,6 +179,10 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
             if (!rc) e = hipMemcpyAsync(h->d_pos, save_pos, pos_bytes, hipMemcpyDeviceToDevice, s);
             if (!rc && e == hipSuccess) e = hipMemcpyAsync(h->d_lam, save_lam, lam_bytes, hipMemcpyDeviceToDevice, s);
             if (!rc && e != hipSuccess) rc = hip_err(h, e, "autotune restore");
+        }
+        if (ge) {
+            (void)hipStreamSynchronize(s);
+            (void)hipGraphExecDestroy(ge);
         }
         if (!rc && ms_best < best_ms) {
             best_ms = ms_best;

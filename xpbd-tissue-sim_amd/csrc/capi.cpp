@@ -21,6 +21,9 @@ using namespace xpbd;
 
 namespace {
 thread_local std::string g_create_error;
+// largest patch count for which a patch Jacobi handle starts on the fused
+// one-launch iteration before any autotuning
+constexpr int kFusedDefaultMaxPatches = 256;
 }
 
 namespace xpbd {
@@ -544,7 +547,10 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
             if (rc2) ok = false;
         }
         h->fused_ok = ok;
-        h->patch_fused = ok;
+        // untuned default: the one-launch iteration wins where launches dominate
+        // (measured: 24 patches), the three-launch one on large meshes (1,875 and
+        // 4,116 patches); xpbd_autotune times both
+        h->patch_fused = ok && Pt.npatch <= kFusedDefaultMaxPatches;
     }
     h->sdf_host.assign(n, SdfDev{});
     for (auto& s : h->sdf_host) s.quat[3] = 1.0;
