@@ -1193,46 +1193,41 @@ __device__ __forceinline__ bool contact_dx(const double* const x[3], const float
     return true;
 }
 
-// All contacts, colour by colour (contacts of one colour share no vertex), by
-// one workgroup of nthreads threads.
-__device__ __forceinline__ void contacts_gs_pass(bool first, int tid, int nthreads, const int4* __restrict__ faces,
-                                                 const int* __restrict__ fcol_off, int ncol,
-                                                 const int* __restrict__ cpos, const int* __restrict__ clist,
-                                                 const double* __restrict__ rest, float4* __restrict__ pos,
-                                                 const float4* __restrict__ lo, const double* __restrict__ cp,
-                                                 const double* __restrict__ cn, const double* __restrict__ cb,
-                                                 double* __restrict__ clam)
+// One contact of the Gauss-Seidel pass: project it from the current positions
+// and apply its correction to the face's three vertices.
+__device__ __forceinline__ void contact_gs_one(bool first, int j, const int4* __restrict__ faces,
+                                               const double* __restrict__ rest, float4* __restrict__ pos,
+                                               const float4* __restrict__ lo, const double* __restrict__ cp,
+                                               const double* __restrict__ cn, const double* __restrict__ cb,
+                                               double* __restrict__ clam)
 {
-    for (int c = 0; c < ncol; c++) {
-        const int s = cpos[fcol_off[c]], e = cpos[fcol_off[c + 1]];
-        for (int k = s + tid; k < e; k += nthreads) {
-            const int j = clist[k];
-            const int4 f = faces[j];
-            double x0[3], x1[3], x2[3];
-            world_pos(rest, pos, lo, f.x, true, x0);
-            world_pos(rest, pos, lo, f.y, true, x1);
-            world_pos(rest, pos, lo, f.z, true, x2);
-            const double* xs[3] = { x0, x1, x2 };
-            const float w[3] = { pos[2 * (size_t)f.x].w, pos[2 * (size_t)f.y].w, pos[2 * (size_t)f.z].w };
-            double lam = first ? 0.0 : clam[j];
-            double dx[9];
-            if (contact_dx(xs, w, cp + 3 * (size_t)j, cn + 3 * (size_t)j, cb + 3 * (size_t)j, &lam, dx)) {
-                const int vid[3] = { f.x, f.y, f.z };
+    const int4 f = faces[j];
+    double x0[3], x1[3], x2[3];
+    world_pos(rest, pos, lo, f.x, true, x0);
+    world_pos(rest, pos, lo, f.y, true, x1);
+    world_pos(rest, pos, lo, f.z, true, x2);
+    const double* xs[3] = { x0, x1, x2 };
+    const float w[3] = { pos[2 * (size_t)f.x].w, pos[2 * (size_t)f.y].w, pos[2 * (size_t)f.z].w };
+    double lam = first ? 0.0 : clam[j];
+    double dx[9];
+    if (contact_dx(xs, w, cp + 3 * (size_t)j, cn + 3 * (size_t)j, cb + 3 * (size_t)j, &lam, dx)) {
+        const int vid[3] = { f.x, f.y, f.z };
 #pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    float4 d = pos[2 * (size_t)vid[i] + 1];
-                    d.x = (float)((double)d.x + dx[3 * i]);
-                    d.y = (float)((double)d.y + dx[3 * i + 1]);
-                    d.z = (float)((double)d.z + dx[3 * i + 2]);
-                    pos[2 * (size_t)vid[i] + 1] = d;
-                }
-            }
-            clam[j] = lam;
+        for (int i = 0; i < 3; i++) {
+            float4 d = pos[2 * (size_t)vid[i] + 1];
+            d.x = (float)((double)d.x + dx[3 * i]);
+            d.y = (float)((double)d.y + dx[3 * i + 1]);
+            d.z = (float)((double)d.z + dx[3 * i + 2]);
+            pos[2 * (size_t)vid[i] + 1] = d;
         }
-        __syncthreads();
     }
+    clam[j] = lam;
 }
 
+// One workgroup projects the active contacts colour by colour (faces of one
+// colour share no vertex; a barrier separates the colours).  The contact range
+// of every face colour is loaded up front, kBlock colours at a time, so colours
+// without contacts cost no memory access.
 template <bool FIRST>
 __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__ faces,
                                                        const int* __restrict__ fcol_off, int ncol,
@@ -1242,7 +1237,21 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
                                                        const double* __restrict__ cp, const double* __restrict__ cn,
                                                        const double* __restrict__ cb, double* __restrict__ clam)
 {
-    contacts_gs_pass(FIRST, threadIdx.x, kBlock, faces, fcol_off, ncol, cpos, clist, rest, pos, lo, cp, cn, cb, clam);
+    __shared__ int rng[kBlock + 1];
+    const int tid = threadIdx.x;
+    for (int c0 = 0; c0 < ncol; c0 += kBlock) {
+        const int nc = min(kBlock, ncol - c0);
+        for (int t = tid; t <= nc; t += kBlock) rng[t] = cpos[fcol_off[c0 + t]];
+        __syncthreads();
+        for (int c = 0; c < nc; c++) {
+            const int cs = rng[c], ce = rng[c + 1];
+            if (cs == ce) continue;  // the same for every thread
+            for (int k = cs + tid; k < ce; k += kBlock)
+                contact_gs_one(FIRST, clist[k], faces, rest, pos, lo, cp, cn, cb, clam);
+            __syncthreads();
+        }
+        __syncthreads();  // rng is rewritten by the next chunk
+    }
 }
 
 template <bool FIRST>
