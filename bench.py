@@ -12,7 +12,9 @@ Prints one JSON line.  `roofline` covers the element-projection kernel
 (algorithmic bytes per solver iteration, 80 B/tet + 28 B/vertex, divided by
 the summed projection-kernel time per iteration, HIP events on the solver's
 stream); `cpu_baseline` times the FP64 oracle (single thread, same loop order
-as the reference CPU solver) on a bounded sample of the same workload.
+as the reference CPU solver) on a bounded sample of the same workload, and
+`cpu_baseline_parallel` the same oracle on all host cores (OpenMP colour-ordered
+Gauss-Seidel over element clusters, or parallel Jacobi projection).
 """
 import argparse
 import json
@@ -135,6 +137,37 @@ def cpu_baseline(meshes, cfg, sdf, nsteps):
     return {"value": nsteps / el, "unit": "steps/s", "cores": 1, "kind": "port",
             "sample": f"{nsteps} steps of the same workload ({mesh.nt} tets, {cfg['iterations']} "
                       f"{cfg['solver']} iterations) on one host core, FP64 oracle",
+            "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el,
+            "host": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
+
+
+def cpu_baseline_parallel(meshes, cfg, sdf, nsteps):
+    """FP64 oracle on all host cores (OpenMP): colour-ordered Gauss-Seidel over
+    clusters of consecutive elements (one hex cell each, 8 colours on the block;
+    an order of this build, not the reference's element order), or Jacobi with
+    the projections in parallel.  Bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    mesh = meshes[0]
+    o = orc.Sim(mesh, dt=cfg["dt"], iters=cfg["iterations"], mode=cfg["solver"])
+    ncol = o.color_elements(cluster=True) if cfg["solver"] == "gs" else 0
+    o.set_threads(threads)
+    if sdf is not None:
+        kind, pos, quat, prm, vel = sdf
+        o.set_sdf(kind, pos, quat, prm)
+    t0 = time.perf_counter()
+    for n in range(nsteps):
+        if sdf is not None:
+            p = [pos[k] + vel[k] * n * cfg["dt"] for k in range(3)]
+            o.set_sdf(kind, p, quat, prm)
+        o.step(1, detect=sdf is not None)
+    el = time.perf_counter() - t0
+    how = (f"colour-ordered GS, {ncol} colours of element clusters" if cfg["solver"] == "gs"
+           else "Jacobi, projections in parallel")
+    return {"value": nsteps / el, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} steps of the same workload ({mesh.nt} tets, {cfg['iterations']} "
+                      f"{cfg['solver']} iterations), FP64 oracle, {how}, {threads} OpenMP threads",
             "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el,
             "host": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
 
@@ -283,6 +316,9 @@ def main():
     if not a.no_cpu:
         n = a.cpu_steps if a.cpu_steps is not None else {"block": 10, "batch": 40}.get(a.workload, 20)
         out["cpu_baseline"] = cpu_baseline(meshes, cfg, sdf, n)
+        if a.workload != "batch":
+            # second reported baseline: the same oracle on all host cores
+            out["cpu_baseline_parallel"] = cpu_baseline_parallel(meshes, cfg, sdf, 4 * n)
     print(json.dumps(out))
 
 

@@ -74,6 +74,9 @@ _sig("orc_set_local_collision_iters", [C.c_void_p, C.c_int], None)
 _sig("orc_last_local_list_size", [C.c_void_p], C.c_int)
 _sig("orc_local_list_size", [C.c_void_p], C.c_int)
 _sig("orc_get_contact_lambdas", [C.c_void_p, _dp], None)
+_sig("orc_color_elements", [C.c_void_p, C.c_int], C.c_int)
+_sig("orc_set_threads", [C.c_void_p, C.c_int], None)
+_sig("orc_get_colors", [C.c_void_p, _ip, _ip, _ip], C.c_int)
 
 
 def project_contact(x9, w3, cp, cn, bary, lam=0.0):
@@ -179,6 +182,28 @@ class Sim:
     def set_order(self, order):
         o = np.ascontiguousarray(order, np.int32)
         lib.orc_set_order(self.s, p(o, C.c_int32))
+
+    def color_elements(self, cluster=True):
+        """Greedy colouring of units (runs of consecutive elements over at most 8
+        vertices when cluster is set, single elements otherwise); the GS order
+        becomes colour-major.  Returns the number of colours (-1 if more than 64
+        would be needed)."""
+        self._ncolor = lib.orc_color_elements(self.s, int(bool(cluster)))
+        return self._ncolor
+
+    def set_threads(self, n):
+        """OpenMP threads of the colour-parallel GS / parallel Jacobi projection."""
+        lib.orc_set_threads(self.s, int(n))
+
+    def colors(self):
+        """(colour-major order, unit offsets into it, colour offsets into the
+        units) after color_elements."""
+        nu = lib.orc_get_colors(self.s, None, None, None)
+        order = np.empty(self.nt, np.int32)
+        uoff = np.empty(nu + 1, np.int32)
+        coff = np.empty(self._ncolor + 1, np.int32)
+        lib.orc_get_colors(self.s, p(order, C.c_int32), p(uoff, C.c_int32), p(coff, C.c_int32))
+        return order, uoff, coff
 
     def set_contact_order(self, face_rank):
         r = np.ascontiguousarray(face_rank, np.int32)

@@ -2,8 +2,9 @@
  * xpbd_oracle.c -- CPU FP64 oracle for the XPBD Neo-Hookean solver path.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
- * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline legs may
  * load liboracle.so.  The shipped solver (the HIP library) never links it.
+ * Its colour-parallel mode (OpenMP) serves only the all-core CPU baseline.
  *
  * It is a clean-room restatement, in plain C and double precision, of the
  * reference CPU solver (MAVYWAVY902/xpbd-tissue-sim), which cannot be built
@@ -456,6 +457,11 @@ typedef struct orc_sim {
     int order_set;         /* s->order was set by orc_set_order (device order) */
     int *v2e_off, *v2e;    /* elements attached to each vertex, ascending (TetMesh.cpp:88-96) */
     int last_local_n;      /* element projections per local pass in the last step */
+    int ncolor;            /* > 0: s->order is colour-major over units of consecutive positions: */
+    int nunit;             /*   unit u at positions [unit_off[u], unit_off[u+1]), colour c holds */
+    int *unit_off;         /*   units [color_off[c], color_off[c+1]), which share no vertex */
+    int* color_off;
+    int nthreads;          /* OpenMP threads of the colour-parallel CPU baseline (<= 1: sequential) */
 } orc_sim;
 
 /* XPBDMeshObject.cpp:102-160 (setup) + 277-331 (masses) + 352-417 (constraints) */
@@ -578,7 +584,7 @@ ORC_API void orc_destroy(orc_sim* s)
     free(s->tets); free(s->Q); free(s->vol); free(s->aD); free(s->aH); free(s->gam);
     free(s->lam); free(s->order); free(s->faces); free(s->jbuf); free(s->jidx);
     free(s->c_face); free(s->c_p); free(s->c_n); free(s->c_bary); free(s->c_lam);
-    free(s->face_rank); free(s->corder); free(s->dz1); free(s->E); free(s->nu);
+    free(s->face_rank); free(s->corder); free(s->dz1); free(s->E); free(s->nu); free(s->color_off); free(s->unit_off);
     free(s->att_v); free(s->att_p); free(s->att_lam);
     free(s->v2e_off); free(s->v2e);
     free(s);
@@ -588,6 +594,100 @@ ORC_API void orc_set_order(orc_sim* s, const int* order)
 {
     memcpy(s->order, order, sizeof(int) * s->nt);
     s->order_set = 1;
+    s->ncolor = 0;
+}
+
+/* Colour-parallel CPU baseline (an order of this build, not the reference's).
+ * Units: with cluster = 0 every element is a unit; otherwise a unit is a run
+ * of consecutive elements (element order) spanning at most 8 vertices and 8
+ * elements -- one hex cell of a mesh generated cell by cell.  Units are given
+ * colours greedily (first fit, unit order) so that units sharing a vertex
+ * differ; the Gauss-Seidel order becomes colour-major (unit order within a
+ * colour, element order within a unit).  With orc_set_threads > 1 the units of
+ * one colour are then projected concurrently, each unit by one thread in
+ * order.  Returns the number of colours, or -1 (order unchanged) if more than
+ * 64 would be needed. */
+ORC_API int orc_color_elements(orc_sim* s, int cluster)
+{
+    const int nt = s->nt;
+    int* uoff = (int*)malloc(sizeof(int) * ((size_t)nt + 1));
+    int nu = 0;
+    for (int e = 0; e < nt;) {
+        uoff[nu++] = e;
+        int vs[8], nvs = 0, k = 0;
+        for (; e < nt && k < 8; e++, k++) {
+            const int* t = s->tets + 4 * e;
+            int add[4], na = 0;
+            for (int i = 0; i < 4; i++) {
+                int seen = 0;
+                for (int j = 0; j < nvs; j++) seen |= vs[j] == t[i];
+                for (int j = 0; j < na; j++) seen |= add[j] == t[i];
+                if (!seen) add[na++] = t[i];
+            }
+            if (k > 0 && (!cluster || nvs + na > 8)) break;
+            for (int j = 0; j < na && nvs < 8; j++) vs[nvs++] = add[j];
+        }
+    }
+    uoff[nu] = nt;
+    unsigned long long* used = (unsigned long long*)calloc((size_t)s->nv, sizeof(unsigned long long));
+    int* col = (int*)malloc(sizeof(int) * (size_t)(nu > 0 ? nu : 1));
+    int ncol = 0;
+    for (int u = 0; u < nu; u++) {
+        unsigned long long m = 0ull;
+        for (int e = uoff[u]; e < uoff[u + 1]; e++)
+            for (int i = 0; i < 4; i++) m |= used[s->tets[4 * e + i]];
+        if (m == ~0ull) {
+            free(uoff);
+            free(used);
+            free(col);
+            return -1;
+        }
+        const int c = __builtin_ctzll(~m);
+        col[u] = c;
+        for (int e = uoff[u]; e < uoff[u + 1]; e++)
+            for (int i = 0; i < 4; i++) used[s->tets[4 * e + i]] |= 1ull << c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    /* colour-major unit order */
+    free(s->color_off);
+    free(s->unit_off);
+    s->color_off = (int*)calloc((size_t)ncol + 1, sizeof(int));
+    for (int u = 0; u < nu; u++) s->color_off[col[u] + 1]++;
+    for (int c = 0; c < ncol; c++) s->color_off[c + 1] += s->color_off[c];
+    int* fill = (int*)malloc(sizeof(int) * ((size_t)ncol + 1));
+    memcpy(fill, s->color_off, sizeof(int) * ((size_t)ncol + 1));
+    int* uorder = (int*)malloc(sizeof(int) * (size_t)(nu > 0 ? nu : 1));
+    for (int u = 0; u < nu; u++) uorder[fill[col[u]]++] = u;
+    s->unit_off = (int*)malloc(sizeof(int) * ((size_t)nu + 1));
+    int pos = 0;
+    for (int i = 0; i < nu; i++) {
+        const int u = uorder[i];
+        s->unit_off[i] = pos;
+        for (int e = uoff[u]; e < uoff[u + 1]; e++) s->order[pos++] = e;
+    }
+    s->unit_off[nu] = pos;
+    s->nunit = nu;
+    s->ncolor = ncol;
+    s->order_set = 1;
+    free(uorder);
+    free(fill);
+    free(uoff);
+    free(used);
+    free(col);
+    return ncol;
+}
+
+ORC_API void orc_set_threads(orc_sim* s, int n) { s->nthreads = n; }
+
+/* After orc_color_elements: the order (nt), unit offsets (nunit + 1, positions)
+ * and colour offsets (ncolor + 1, units).  Returns the number of units. */
+ORC_API int orc_get_colors(const orc_sim* s, int* order, int* unit_off, int* color_off)
+{
+    if (s->ncolor <= 0) return 0;
+    if (order) memcpy(order, s->order, sizeof(int) * (size_t)s->nt);
+    if (unit_off) memcpy(unit_off, s->unit_off, sizeof(int) * ((size_t)s->nunit + 1));
+    if (color_off) memcpy(color_off, s->color_off, sizeof(int) * ((size_t)s->ncolor + 1));
+    return s->nunit;
 }
 
 ORC_API void orc_set_mode(orc_sim* s, int mode, int iters)
@@ -772,19 +872,34 @@ ORC_API void orc_set_attachments(orc_sim* s, int n, const int* verts, const doub
     }
 }
 
+/* Gauss-Seidel projection of element e, its 12 updates applied at once */
+static void gs_element(orc_sim* s, int e)
+{
+    const int* t = s->tets + 4 * e;
+    const double* p[4] = { s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3] };
+    const double w[4] = { s->w[t[0]], s->w[t[1]], s->w[t[2]], s->w[t[3]] };
+    double dx[12];
+    project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->ts, s->lam + 2 * e, dx);
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 3; k++) s->x[3 * t[i] + k] += dx[3 * i + k];
+}
+
 static void iterate(orc_sim* s)
 {
     const int nt = s->nt;
     if (s->mode == ORC_GS) {
-        for (int oi = 0; oi < nt; oi++) {
-            const int e = s->order[oi];
-            const int* t = s->tets + 4 * e;
-            const double* p[4] = { s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3] };
-            const double w[4] = { s->w[t[0]], s->w[t[1]], s->w[t[2]], s->w[t[3]] };
-            double dx[12];
-            project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->ts, s->lam + 2 * e, dx);
-            for (int i = 0; i < 4; i++)
-                for (int k = 0; k < 3; k++) s->x[3 * t[i] + k] += dx[3 * i + k];
+        if (s->ncolor > 0 && s->nthreads > 1) {
+            /* units of one colour share no vertex: projecting them concurrently,
+             * each unit in order, is the sequential pass over the same order,
+             * bit for bit */
+            for (int c = 0; c < s->ncolor; c++) {
+                const int b = s->color_off[c], en = s->color_off[c + 1];
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+                for (int u = b; u < en; u++)
+                    for (int oi = s->unit_off[u]; oi < s->unit_off[u + 1]; oi++) gs_element(s, s->order[oi]);
+            }
+        } else {
+            for (int oi = 0; oi < nt; oi++) gs_element(s, s->order[oi]);
         }
         for (int ci = 0; ci < s->nc; ci++) {
             const int c = s->corder[ci];
@@ -804,18 +919,20 @@ static void iterate(orc_sim* s)
         }
         return;
     }
-    /* Jacobi */
-    size_t nu = 0;
+    /* Jacobi: every element writes its own 12 buffer slots (in parallel when
+     * threads are set); the sum below stays in buffer order */
+    const size_t nu = 12 * (size_t)nt;
     double* cbuf = NULL;
     int* cidx = NULL;
+#pragma omp parallel for if (s->nthreads > 1) num_threads(s->nthreads > 1 ? s->nthreads : 1) schedule(static)
     for (int e = 0; e < nt; e++) {
         const int* t = s->tets + 4 * e;
         const double* p[4] = { s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3] };
         const double w[4] = { s->w[t[0]], s->w[t[1]], s->w[t[2]], s->w[t[3]] };
-        project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->ts, s->lam + 2 * e, s->jbuf + nu);
+        const size_t u0 = 12 * (size_t)e;
+        project_combined(p, w, s->Q + 9 * e, s->aD[e], s->aH[e], s->gam[e], s->ts, s->lam + 2 * e, s->jbuf + u0);
         for (int i = 0; i < 4; i++)
-            for (int k = 0; k < 3; k++) s->jidx[nu + 3 * i + k] = 3 * t[i] + k;
-        nu += 12;
+            for (int k = 0; k < 3; k++) s->jidx[u0 + 3 * i + k] = 3 * t[i] + k;
     }
     int ncu = 0;
     if (s->nc) {

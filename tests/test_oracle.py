@@ -596,3 +596,63 @@ def test_local_collision_passes_do_nothing_without_contacts():
     b.step(10)
     np.testing.assert_array_equal(a.x(), b.x())
     assert b.last_local_list_size() == 0
+
+
+# ---------------------------------------------------------------- colour-parallel CPU baseline
+@pytest.mark.parametrize("cluster", [False, True])
+def test_colouring_is_a_valid_partition(cluster):
+    m = _mesh(5)
+    s = orc.Sim(m, iters=1, mode="gs")
+    nc = s.color_elements(cluster)
+    order, uoff, coff = s.colors()
+    assert nc > 0 and len(coff) == nc + 1 and coff[-1] == len(uoff) - 1 and uoff[-1] == m.nt
+    assert np.array_equal(np.sort(order), np.arange(m.nt))
+    sizes = np.diff(uoff)
+    assert sizes.min() >= 1 and sizes.max() <= (8 if cluster else 1)
+    if cluster:   # a hex cell generated cell by cell: 6 consecutive elements over 8 vertices
+        assert np.all(sizes == 6)
+    for u in range(len(sizes)):
+        assert np.all(np.diff(order[uoff[u]:uoff[u + 1]]) == 1)            # consecutive elements
+        assert len(np.unique(m.tets[order[uoff[u]:uoff[u + 1]]])) <= 8
+    for c in range(nc):
+        verts = [np.unique(m.tets[order[uoff[u]:uoff[u + 1]]]) for u in range(coff[c], coff[c + 1])]
+        allv = np.concatenate(verts)
+        assert len(np.unique(allv)) == len(allv)          # units of one colour share no vertex
+    if cluster:
+        assert nc == 8                                    # eight cell colours on a structured grid
+
+
+@pytest.mark.parametrize("mode", ["gs", "jacobi"])
+def test_threaded_oracle_equals_the_single_threaded_pass(mode):
+    """Colour-parallel GS (elements of a colour share no vertex) and the parallel
+    Jacobi projection (disjoint buffer slots, sum in buffer order) give the
+    single-threaded results bit for bit, with contacts detected every step."""
+    import workloads as W
+    m = W.grid_object(6, 6, 6, 0.01, pin_axis=2, noise=0.01, seed=4, faces=True)
+    runs = []
+    for threads in (1, 4):
+        s = orc.Sim(m, iters=4, mode=mode)
+        if mode == "gs":
+            assert s.color_elements(cluster=True) > 0
+        s.set_threads(threads)
+        s.set_sdf(orc.SDF_SPHERE, [0.03, 0.03, 0.06 + 0.02 - 0.004], [0, 0, 0, 1], [0.02, 0, 0])
+        most = 0
+        for _ in range(15):
+            s.step(1, detect=True)
+            most = max(most, len(s.contacts()[0]))
+        runs.append((s.x(), s.lambdas(), most))
+    assert runs[0][2] > 0
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_colour_ordered_run_equals_the_same_order_set_explicitly():
+    m = _mesh(4)
+    a = orc.Sim(m, iters=3, mode="gs")
+    a.color_elements(cluster=True)
+    a.set_threads(4)
+    order, _, _ = a.colors()
+    b = orc.Sim(m, iters=3, mode="gs")
+    b.set_order(order)
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.x(), b.x())
