@@ -1,7 +1,7 @@
 """ctypes wrapper of liboracle.so, the FP64 CPU oracle.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py.  It is the checker, never the product.
+cpu_baseline legs of bench.py.  It is the checker, never the product.
 """
 from __future__ import annotations
 
