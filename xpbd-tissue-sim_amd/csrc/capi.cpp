@@ -393,6 +393,14 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
         for (int f = 0; f < nf; f++) h->fperm[pos[col[f]]++] = f;
     }
     h->nfcolors = (int)h->fcol_off.size() - 1;
+    h->fci_off.assign((size_t)h->nfcolors * (h->ninst + 1), 0);
+    for (int c = 0; c < h->nfcolors; c++) {
+        int j = h->fcol_off[c];
+        for (int i = 0; i <= h->ninst; i++) {
+            while (j < h->fcol_off[c + 1] && faces[4 * (size_t)h->fperm[j] + 3] < i) j++;
+            h->fci_off[(size_t)c * (h->ninst + 1) + i] = j;
+        }
+    }
 
     // ---- device buffers
     hipError_t he = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -439,7 +447,7 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     rc |= dalloc(h, &h->d_step, 1);
     rc |= dalloc(h, &h->d_sdf, n);
     rc |= dalloc(h, &h->d_faces, nf);
-    rc |= dalloc(h, &h->d_fcol_off, h->fcol_off.size());
+    rc |= dalloc(h, &h->d_fci_off, h->fci_off.size());
     rc |= dalloc(h, &h->d_flag, nf);
     rc |= dalloc(h, &h->d_cpos, (size_t)nf + 1);
     rc |= dalloc(h, &h->d_clist, nf);
@@ -596,7 +604,7 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
         UP(h->d_tloc, tloc_solver.data(), sizeof(uint32_t) * tloc_solver.size());
     }
     if (nf > 0) UP(h->d_faces, hf.data(), sizeof(int4) * nf);
-    UP(h->d_fcol_off, h->fcol_off.data(), sizeof(int32_t) * h->fcol_off.size());
+    UP(h->d_fci_off, h->fci_off.data(), sizeof(int32_t) * h->fci_off.size());
     if (h->d_vt_off) {
         UP(h->d_vt_off, vt_off.data(), sizeof(int32_t) * vt_off.size());
         UP(h->d_vt_pos, vt_pos.data(), sizeof(int32_t) * vt_pos.size());
@@ -734,7 +742,7 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
             }
             if (timed) timed_end(h);
             if (con) {
-                e = launch_contacts_gs(s, h->d_faces, h->d_fcol_off, h->nfcolors, h->d_cpos, h->d_clist, h->d_rest,
+                e = launch_contacts_gs(s, h->d_faces, h->d_fci_off, h->nfcolors, h->ninst, h->d_cpos, h->d_clist, h->d_rest,
                                        h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
                 if (e != hipSuccess) return hip_err(h, e, "contacts_gs launch");
             }
@@ -811,7 +819,7 @@ int enqueue_local_collision(xpbd_handle* h)
                             : launch_gs_local_color(s, ta, h->d_mult, h->d_pos, h->color_off[c], h->color_off[c + 1]);
                 if (e != hipSuccess) return hip_err(h, e, "local gs launch");
             }
-            e = launch_contacts_gs(s, h->d_faces, h->d_fcol_off, h->nfcolors, h->d_cpos, h->d_clist, h->d_rest,
+            e = launch_contacts_gs(s, h->d_faces, h->d_fci_off, h->nfcolors, h->ninst, h->d_cpos, h->d_clist, h->d_rest,
                                    h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, false);
             if (e != hipSuccess) return hip_err(h, e, "local contacts_gs launch");
         } else {

@@ -43,6 +43,10 @@ struct xpbd_handle {
     std::vector<int32_t> fperm;                // solver face position -> caller's face index
     std::vector<int32_t> fcol_off;
     int nfcolors = 1;
+    // within a face colour the faces follow face order, so each instance's faces
+    // are one run: fci_off[c * (ninst + 1) + i] is where instance i's faces of
+    // colour c start (i = ninst: the colour's end)
+    std::vector<int32_t> fci_off;
     std::vector<xpbd::SdfDev> sdf_host;
     bool sdf_active = false;
 
@@ -84,7 +88,7 @@ struct xpbd_handle {
     unsigned long long* d_step = nullptr;
     xpbd::SdfDev* d_sdf = nullptr;
     int4* d_faces = nullptr;
-    int32_t* d_fcol_off = nullptr;
+    int32_t* d_fci_off = nullptr;    // see fci_off
     int32_t* d_flag = nullptr;
     int32_t* d_cpos = nullptr;
     int32_t* d_clist = nullptr;

@@ -125,9 +125,12 @@ hipError_t launch_detect(hipStream_t s, const int4* faces, int nf, const double*
 // Ordered compaction of hit flags; bsum is scratch of compact_blocks(nf) ints.
 int compact_blocks(int nf);
 hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int* clist, int* bsum);
-hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fcol_off, int ncol, const int* cpos,
-                              const int* clist, const double* rest, float4* pos, const float4* lo, const double* cp,
-                              const double* cn, const double* cb, double* clam, bool first);
+// Gauss-Seidel contacts: one workgroup per instance (instances share no
+// vertex), colour by colour; fci_off[c * (ninst + 1) + i] is the face position
+// where instance i's faces of colour c start.
+hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fci_off, int ncol, int ninst,
+                              const int* cpos, const int* clist, const double* rest, float4* pos, const float4* lo,
+                              const double* cp, const double* cn, const double* cb, double* clam, bool first);
 hipError_t launch_contacts_jacobi(hipStream_t s, const int4* faces, int nf, const int* cpos, const int* clist,
                                   const double* rest, const float4* pos, const float4* lo, float* acc,
                                   const double* cp, const double* cn, const double* cb, double* clam, bool first);

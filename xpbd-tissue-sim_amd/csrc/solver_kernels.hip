@@ -1224,27 +1224,33 @@ __device__ __forceinline__ void contact_gs_one(bool first, int j, const int4* __
     clam[j] = lam;
 }
 
-// One workgroup projects the active contacts colour by colour (faces of one
-// colour share no vertex; a barrier separates the colours).  The contact range
-// of every face colour is loaded up front, kBlock colours at a time, so colours
-// without contacts cost no memory access.
+// Workgroup i projects the active contacts of instance i colour by colour
+// (faces of one colour share no vertex; a barrier separates the colours), in
+// face order within a colour; instances share no vertex, so their workgroups
+// run side by side.  The contact range of every face colour is loaded up
+// front, kBlock colours at a time, so colours without contacts cost no memory
+// access.
 template <bool FIRST>
 __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__ faces,
-                                                       const int* __restrict__ fcol_off, int ncol,
+                                                       const int* __restrict__ fci_off, int ncol, int ninst,
                                                        const int* __restrict__ cpos, const int* __restrict__ clist,
                                                        const double* __restrict__ rest, float4* __restrict__ pos,
                                                        const float4* __restrict__ lo,
                                                        const double* __restrict__ cp, const double* __restrict__ cn,
                                                        const double* __restrict__ cb, double* __restrict__ clam)
 {
-    __shared__ int rng[kBlock + 1];
-    const int tid = threadIdx.x;
+    __shared__ int rng[2 * kBlock];
+    const int tid = threadIdx.x, inst = blockIdx.x;
     for (int c0 = 0; c0 < ncol; c0 += kBlock) {
         const int nc = min(kBlock, ncol - c0);
-        for (int t = tid; t <= nc; t += kBlock) rng[t] = cpos[fcol_off[c0 + t]];
+        if (tid < nc) {
+            const int* o = fci_off + (size_t)(c0 + tid) * (ninst + 1) + inst;
+            rng[2 * tid] = cpos[o[0]];
+            rng[2 * tid + 1] = cpos[o[1]];
+        }
         __syncthreads();
         for (int c = 0; c < nc; c++) {
-            const int cs = rng[c], ce = rng[c + 1];
+            const int cs = rng[2 * c], ce = rng[2 * c + 1];
             if (cs == ce) continue;  // the same for every thread
             for (int k = cs + tid; k < ce; k += kBlock)
                 contact_gs_one(FIRST, clist[k], faces, rest, pos, lo, cp, cn, cb, clam);
@@ -1926,16 +1932,17 @@ hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int
     return hipGetLastError();
 }
 
-hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fcol_off, int ncol, const int* cpos,
-                              const int* clist, const double* rest, float4* pos, const float4* lo, const double* cp,
-                              const double* cn, const double* cb, double* clam, bool first)
+hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fci_off, int ncol, int ninst,
+                              const int* cpos, const int* clist, const double* rest, float4* pos, const float4* lo,
+                              const double* cp, const double* cn, const double* cb, double* clam, bool first)
 {
+    if (ninst <= 0 || ncol <= 0) return hipSuccess;
     if (first)
-        hipLaunchKernelGGL(k_contacts_gs<true>, dim3(1), dim3(kBlock), 0, s, faces, fcol_off, ncol, cpos, clist,
-                           rest, pos, lo, cp, cn, cb, clam);
+        hipLaunchKernelGGL(k_contacts_gs<true>, dim3(ninst), dim3(kBlock), 0, s, faces, fci_off, ncol, ninst, cpos,
+                           clist, rest, pos, lo, cp, cn, cb, clam);
     else
-        hipLaunchKernelGGL(k_contacts_gs<false>, dim3(1), dim3(kBlock), 0, s, faces, fcol_off, ncol, cpos, clist,
-                           rest, pos, lo, cp, cn, cb, clam);
+        hipLaunchKernelGGL(k_contacts_gs<false>, dim3(ninst), dim3(kBlock), 0, s, faces, fci_off, ncol, ninst, cpos,
+                           clist, rest, pos, lo, cp, cn, cb, clam);
     return hipGetLastError();
 }
 
