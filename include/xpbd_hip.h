@@ -40,8 +40,8 @@ extern "C" {
 #define XPBD_SOLVER_JACOBI 0
 #define XPBD_SOLVER_GS_COLORED 1
 
-/* Kernel variants (xpbd_params.variant).  AUTO picks CLUSTER for Gauss-Seidel
- * and PATCH for Jacobi.  SIMPLE: one thread per tet (Gauss-Seidel: one launch
+/* Kernel variants (xpbd_params.variant).  AUTO picks CLUSTER_QUAD for
+ * Gauss-Seidel and PATCH for Jacobi (xpbd_autotune then times the shapes).  SIMPLE: one thread per tet (Gauss-Seidel: one launch
  * per tet colour; Jacobi: float atomics).  CLUSTER (Gauss-Seidel): runs of
  * consecutive tets sharing at most 8 vertices form clusters, clusters are
  * coloured, one thread projects its cluster's tets in order.  PATCH (Jacobi):

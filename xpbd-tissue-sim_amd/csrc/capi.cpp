@@ -297,8 +297,10 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     // ---- solver order of elements
     h->order.resize(nt);
     int variant = P->variant;
+    // untuned default: the quad cluster kernel (faster than one thread per
+    // cluster on every measured workload) or patch Jacobi
     if (variant == XPBD_VARIANT_AUTO)
-        variant = P->solver == XPBD_SOLVER_GS_COLORED ? XPBD_VARIANT_CLUSTER : XPBD_VARIANT_PATCH;
+        variant = P->solver == XPBD_SOLVER_GS_COLORED ? XPBD_VARIANT_CLUSTER_QUAD : XPBD_VARIANT_PATCH;
     const bool cluster = variant == XPBD_VARIANT_CLUSTER || variant == XPBD_VARIANT_CLUSTER_QUAD;
     if ((P->solver == XPBD_SOLVER_GS_COLORED && variant == XPBD_VARIANT_PATCH) ||
         (P->solver == XPBD_SOLVER_JACOBI && cluster) || variant < 0 || variant > XPBD_VARIANT_CLUSTER_QUAD) {
