@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "mesh_tools.h"
 #include "xpbd_math.h"
 
 namespace xpbd {
@@ -308,6 +309,9 @@ struct QuadLds {
     float sl[2][kStage];       // lambda
     uint32_t stl[kStage];      // local vertex slots
 };
+
+// The unrolled step loop covers every element of a cluster.
+static_assert(QuadLds<64>::kMaxTets == kClusterTets && kClusterVerts == 8, "cluster shape of build_clusters");
 
 // One chunk of BLOCK / 4 consecutive clusters starting at cluster cb (clusters
 // below cend), projected by one workgroup and written back.
