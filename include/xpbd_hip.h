@@ -249,6 +249,11 @@ int xpbd_batch_num_instances(xpbd_batch* b, int32_t* out);
 int xpbd_batch_set_sdf(xpbd_batch* b, int32_t instance, int32_t kind, const double* pose7,
                        const double* params3);
 int xpbd_batch_get_positions(xpbd_batch* b, int32_t instance, double* out);
+/* Advance every instance by nsteps, one graph replay per step covering all of
+ * them (the same as xpbd_step on the batch handle, which is also read and
+ * destroyed with the single-object calls).  Replaces the reference's loop over
+ * objects in Simulation::_timeStep (Simulation.cpp:246-255). */
+int xpbd_batch_step(xpbd_batch* b, int32_t nsteps);
 /* Time the kernel variants and launch shapes that share this handle's device
  * layout on its own problem (Gauss-Seidel with clusters: the one-thread
  * cluster kernel and the quad kernel at 64/128/256 threads per workgroup;
@@ -305,6 +310,12 @@ int xpbd_mlp_set_params(xpbd_mlp* m, const float* params);
 int xpbd_mlp_get_params(xpbd_mlp* m, float* params);
 /* Feature / target scaling: y_model = scale_out * MLP((x - mean_in) / std_in). */
 int xpbd_mlp_set_scaling(xpbd_mlp* m, const float* mean_in24, const float* std_in24, float scale_out);
+/* Network file: "XPBDMLP1", int32 parameter and feature counts (20,099, 24),
+ * the float32 parameters in the layout above, mean_in[24], std_in[24],
+ * scale_out (little-endian).  Save needs the scaling set; load checks the tag
+ * and the counts and sets parameters and scaling. */
+int xpbd_mlp_save(xpbd_mlp* m, const char* path);
+int xpbd_mlp_load(xpbd_mlp* m, const char* path);
 /* Forward on n host rows (n x 24) -> out (n x 3). */
 int xpbd_mlp_forward(xpbd_mlp* m, int32_t n, const float* x, float* out);
 /* Loss and parameter gradient on n rows (host arrays): the mean over n*3 of

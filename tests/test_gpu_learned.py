@@ -237,3 +237,27 @@ def test_trained_network_predicts_the_solver_correction():
           f"none 1.000 ({len(x)} training rows, {len(xe)} held-out rows)")
     assert err_net < 0.5
     assert err_net < err_prev
+
+
+def test_network_file_round_trip(tmp_path):
+    """xpbd_mlp_save / xpbd_mlp_load keep parameters and scaling exactly; a
+    truncated file or a network without scaling is refused."""
+    rng = np.random.default_rng(21)
+    a = X.Network(glorot(5))
+    mean, std = rng.normal(size=X.MLP_IN).astype(np.float32), rng.uniform(0.5, 2.0, X.MLP_IN).astype(np.float32)
+    a.set_scaling(mean, std, 3e-4)
+    path = str(tmp_path / "net.bin")
+    a.save(path)
+    b = X.Network()
+    b.load(path)
+    assert np.array_equal(a.params(), b.params())
+    x = rng.normal(size=(1000, X.MLP_IN)).astype(np.float32) * std + mean
+    assert np.array_equal(a.forward(x), b.forward(x))
+    raw = open(path, "rb").read()
+    assert len(raw) == 8 + 8 + 4 * (X.MLP_PARAMS + 2 * X.MLP_IN + 1)
+    bad = tmp_path / "short.bin"
+    bad.write_bytes(raw[:-3])
+    with pytest.raises(X.XpbdError):
+        X.Network().load(str(bad))
+    with pytest.raises(X.XpbdError):
+        X.Network(glorot(6)).save(str(tmp_path / "unscaled.bin"))

@@ -487,3 +487,19 @@ def test_edge_cases_match_oracle(case, solver, variant, det):
     assert errs.max() <= TOL
     if case == "all pinned":
         np.testing.assert_array_equal(gpu.positions(), mesh.x0)
+
+
+def test_batch_step_is_the_step_of_the_batch_handle():
+    meshes, sdfs = _uneven_batch(4, (1e3, 5e4), seed=5)
+    runs = []
+    for batch_call in (False, True):
+        s = X.Solver(meshes, dt=1e-3, iters=5, solver="gs", collision_every=1)
+        for i, (kind, pos, quat, prm) in enumerate(sdfs):
+            s.set_sdf(kind, pos, quat, prm, instance=i)
+        if batch_call:
+            s._c(X.lib.xpbd_batch_step(s.h, 7))
+        else:
+            s.step(7)
+        runs.append(np.concatenate([s.instance_positions(i) for i in range(len(meshes))]))
+        s.close()
+    assert np.array_equal(runs[0], runs[1])

@@ -1022,6 +1022,11 @@ extern "C" int xpbd_batch_get_positions(xpbd_batch* b, int32_t instance, double*
     return positions_range(b, b->v_off[instance], b->v_off[instance + 1], out);
 }
 
+extern "C" int xpbd_batch_step(xpbd_batch* b, int32_t nsteps)
+{
+    return xpbd_step(b, nsteps);
+}
+
 extern "C" int xpbd_batch_num_instances(xpbd_batch* b, int32_t* out)
 {
     if (!b || !out) return XPBD_E_INVALID_ARG;

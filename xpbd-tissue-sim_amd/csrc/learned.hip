@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <random>
@@ -490,6 +491,8 @@ struct xpbd_mlp {
     float* d_istd = nullptr;
     float* d_split = nullptr; // split-K partial products
     float scale = 1.f;
+    float h_mean[24] = {}, h_std[24] = {};  // host copies of the scaling (for xpbd_mlp_save)
+    bool has_scaling = false;
     int64_t adam_t = 0;
     // training workspace (rows = capacity)
     int cap = 0;
@@ -714,7 +717,59 @@ extern "C" int xpbd_mlp_set_scaling(xpbd_mlp* m, const float* mean_in, const flo
     if (e == hipSuccess) e = hipMemcpyAsync(m->d_istd, istd, sizeof(float) * kIn, hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     m->scale = scale_out;
+    if (e == hipSuccess) {
+        std::memcpy(m->h_mean, mean_in, sizeof(float) * kIn);
+        std::memcpy(m->h_std, std_in, sizeof(float) * kIn);
+        m->has_scaling = true;
+    }
     return e == hipSuccess ? XPBD_OK : merr(m, XPBD_E_HIP, hipGetErrorString(e));
+}
+
+namespace {
+constexpr char kMlpMagic[8] = { 'X', 'P', 'B', 'D', 'M', 'L', 'P', '1' };
+}
+
+extern "C" int xpbd_mlp_save(xpbd_mlp* m, const char* path)
+{
+    if (!m || !path) return XPBD_E_INVALID_ARG;
+    if (!m->has_scaling) return merr(m, XPBD_E_INVALID_ARG, "no feature scaling set");
+    std::vector<float> p(kNParams);
+    int rc = xpbd_mlp_get_params(m, p.data());
+    if (rc) return rc;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return merr(m, XPBD_E_INVALID_ARG, std::string("cannot write ") + path);
+    const int32_t sizes[2] = { kNParams, kIn };
+    bool ok = std::fwrite(kMlpMagic, 1, sizeof(kMlpMagic), f) == sizeof(kMlpMagic) &&
+              std::fwrite(sizes, sizeof(int32_t), 2, f) == 2 &&
+              std::fwrite(p.data(), sizeof(float), kNParams, f) == (size_t)kNParams &&
+              std::fwrite(m->h_mean, sizeof(float), kIn, f) == (size_t)kIn &&
+              std::fwrite(m->h_std, sizeof(float), kIn, f) == (size_t)kIn &&
+              std::fwrite(&m->scale, sizeof(float), 1, f) == 1;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? XPBD_OK : merr(m, XPBD_E_INVALID_ARG, std::string("short write to ") + path);
+}
+
+extern "C" int xpbd_mlp_load(xpbd_mlp* m, const char* path)
+{
+    if (!m || !path) return XPBD_E_INVALID_ARG;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return merr(m, XPBD_E_INVALID_ARG, std::string("cannot read ") + path);
+    char magic[8];
+    int32_t sizes[2];
+    std::vector<float> p(kNParams);
+    float mean[kIn], sd[kIn], scale = 0.f;
+    bool ok = std::fread(magic, 1, sizeof(magic), f) == sizeof(magic) &&
+              std::memcmp(magic, kMlpMagic, sizeof(magic)) == 0 && std::fread(sizes, sizeof(int32_t), 2, f) == 2 &&
+              sizes[0] == kNParams && sizes[1] == kIn &&
+              std::fread(p.data(), sizeof(float), kNParams, f) == (size_t)kNParams &&
+              std::fread(mean, sizeof(float), kIn, f) == (size_t)kIn &&
+              std::fread(sd, sizeof(float), kIn, f) == (size_t)kIn && std::fread(&scale, sizeof(float), 1, f) == 1;
+    char extra;
+    ok = ok && std::fread(&extra, 1, 1, f) == 0;  // nothing after the scale
+    std::fclose(f);
+    if (!ok) return merr(m, XPBD_E_INVALID_ARG, std::string("not a network file of this layout: ") + path);
+    int rc = xpbd_mlp_set_params(m, p.data());
+    return rc ? rc : xpbd_mlp_set_scaling(m, mean, sd, scale);
 }
 
 namespace {

@@ -106,6 +106,9 @@ _sig("xpbd_mlp_num_params", [_ip])
 _sig("xpbd_mlp_set_params", [_vp, _fp])
 _sig("xpbd_mlp_get_params", [_vp, _fp])
 _sig("xpbd_mlp_set_scaling", [_vp, _fp, _fp, C.c_float])
+_sig("xpbd_mlp_save", [_vp, C.c_char_p])
+_sig("xpbd_mlp_load", [_vp, C.c_char_p])
+_sig("xpbd_batch_step", [_vp, C.c_int32])
 _sig("xpbd_mlp_forward", [_vp, C.c_int32, _fp, _fp])
 _sig("xpbd_mlp_loss_grad", [_vp, C.c_int32, _fp, _fp, _dp, _fp])
 _sig("xpbd_mlp_train", [_vp, C.c_int32, _fp, _fp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _dp])
@@ -489,6 +492,14 @@ class Network:
         mu = np.ascontiguousarray(mean, np.float32).reshape(MLP_IN)
         sd = np.ascontiguousarray(std, np.float32).reshape(MLP_IN)
         self._c(lib.xpbd_mlp_set_scaling(self.m, ptr(mu, C.c_float), ptr(sd, C.c_float), float(scale_out)))
+
+    def save(self, path: str):
+        """Write parameters and scaling as a network file (xpbd_mlp_save)."""
+        self._c(lib.xpbd_mlp_save(self.m, os.fsencode(path)))
+
+    def load(self, path: str):
+        """Read parameters and scaling from a network file (xpbd_mlp_load)."""
+        self._c(lib.xpbd_mlp_load(self.m, os.fsencode(path)))
 
     def forward(self, x) -> np.ndarray:
         xa = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_IN)
