@@ -75,6 +75,8 @@ def save(path, net: X.Network, mean, std, scale, meta):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     np.savez(path, params=net.params(), mean=np.asarray(mean, np.float32), std=np.asarray(std, np.float32),
              scale=np.float32(scale), meta=np.array(json.dumps(meta)))
+    # the same network as a file that C and C++ callers read with xpbd_mlp_load
+    net.save(os.path.splitext(path)[0] + ".bin")
 
 
 def load(path) -> X.Network:
