@@ -284,10 +284,16 @@ __device__ __forceinline__ float dpp_xor2(float x)
 {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, true));
 }
-__device__ __forceinline__ float quad_sum(float x)
+// Sum of a * b over the four lanes of a quad, (x + swap1) + swap2, the same
+// bits in all four lanes.  The product is rounded in every lane first:
+// contracted into an FMA, each lane would add its own unrounded product to its
+// partners' rounded ones, and the lanes would no longer hold the same sum.
+__device__ __forceinline__ float quad_sum_of_product(float a, float b)
 {
-    x = x + dpp_xor1(x);
-    return x + dpp_xor2(x);
+#pragma clang fp contract(off)
+    const float x = a * b;
+    const float y = x + dpp_xor1(x);
+    return y + dpp_xor2(y);
 }
 __device__ __forceinline__ float quad_lane3(float x)
 {
@@ -474,7 +480,7 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
             for (int cc = 0; cc < 3; cc++)
 #pragma unroll
                 for (int r = 0; r < 3; r++) {
-                    const float sum = quad_sum(D[r] * qj[cc]);
+                    const float sum = quad_sum_of_product(D[r], qj[cc]);
                     F[3 * cc + r] = r == cc ? sum + 1.f : sum;
                 }
             // C_D, C_H as element_constraints, with both branches of C_H evaluated
@@ -505,9 +511,9 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
                 gD[r] = inv_C * (F[r] * qj[0] + F[3 + r] * qj[1] + F[6 + r] * qj[2]);
                 gH[r] = fac * (Fc[r] * qj[0] + Fc[3 + r] * qj[1] + Fc[6 + r] * qj[2]);
             }
-            const float s00 = quad_sum(w * (gD[0] * gD[0] + gD[1] * gD[1] + gD[2] * gD[2]));
-            const float s01 = quad_sum(w * (gD[0] * gH[0] + gD[1] * gH[1] + gD[2] * gH[2]));
-            const float s11 = quad_sum(w * (gH[0] * gH[0] + gH[1] * gH[1] + gH[2] * gH[2]));
+            const float s00 = quad_sum_of_product(w, gD[0] * gD[0] + gD[1] * gD[1] + gD[2] * gD[2]);
+            const float s01 = quad_sum_of_product(w, gD[0] * gH[0] + gD[1] * gH[1] + gD[2] * gH[2]);
+            const float s11 = quad_sum_of_product(w, gH[0] * gH[0] + gH[1] * gH[1] + gH[2] * gH[2]);
             const float A00 = atD + s00, A11 = atH + s11, A01 = s01;
             const float b0 = -CD - atD * l0;
             const float b1 = -CH - atH * l1;
