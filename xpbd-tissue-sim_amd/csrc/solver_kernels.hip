@@ -1238,8 +1238,12 @@ __device__ __forceinline__ void contact_gs_one(bool first, int j, const int4* __
 // (faces of one colour share no vertex; a barrier separates the colours), in
 // face order within a colour; instances share no vertex, so their workgroups
 // run side by side.  The contact range of every face colour is loaded up
-// front, kBlock colours at a time, so colours without contacts cost no memory
-// access.
+// front, kBlock colours at a time.  The instance's contacts are numbered in
+// colour order, and thread t owns contact t: everything that stays fixed
+// during the solve (its face, rest positions, base + low displacement, inverse
+// masses, contact point, normal, barycentrics, multiplier) is loaded once, so a
+// colour costs one round trip for the in-step motion of its vertices.  Contacts
+// past the first kBlock of an instance are loaded as they are projected.
 template <bool FIRST>
 __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__ faces,
                                                        const int* __restrict__ fci_off, int ncol, int ninst,
@@ -1249,7 +1253,8 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
                                                        const double* __restrict__ cp, const double* __restrict__ cn,
                                                        const double* __restrict__ cb, double* __restrict__ clam)
 {
-    __shared__ int rng[2 * kBlock];
+    __shared__ int rng[2 * kBlock];  // per colour of the chunk: the instance's range in clist
+    __shared__ int loc[kBlock + 1];  // per colour: number of the instance's first contact of that colour
     const int tid = threadIdx.x, inst = blockIdx.x;
     for (int c0 = 0; c0 < ncol; c0 += kBlock) {
         const int nc = min(kBlock, ncol - c0);
@@ -1259,14 +1264,86 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
             rng[2 * tid + 1] = cpos[o[1]];
         }
         __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int c = 0; c < nc; c++) {
+                loc[c] = acc;
+                acc += rng[2 * c + 1] - rng[2 * c];
+            }
+            loc[nc] = acc;
+        }
+        __syncthreads();
+        // this thread's contact: its colour (loc is non-decreasing) and fixed data
+        int mc = -1, j = 0;
+        int vid[3] = { 0, 0, 0 };
+        double xr[3][3], sbl[3][3], ccp[3], ccn[3], ccb[3], lam = 0.0;
+        float w[3] = { 0.f, 0.f, 0.f };
+        if (tid < loc[nc]) {
+            int lo_c = 0, hi_c = nc - 1;
+            while (lo_c < hi_c) {
+                const int mid = (lo_c + hi_c + 1) >> 1;
+                if (loc[mid] <= tid) lo_c = mid;
+                else hi_c = mid - 1;
+            }
+            mc = lo_c;
+            j = clist[rng[2 * mc] + (tid - loc[mc])];
+            const int4 f = faces[j];
+            vid[0] = f.x;
+            vid[1] = f.y;
+            vid[2] = f.z;
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                const float4 bv = pos[2 * (size_t)vid[m]];
+                const float4 l = lo[vid[m]];
+                sbl[m][0] = (double)bv.x + (double)l.x;
+                sbl[m][1] = (double)bv.y + (double)l.y;
+                sbl[m][2] = (double)bv.z + (double)l.z;
+                w[m] = bv.w;
+#pragma unroll
+                for (int k = 0; k < 3; k++) xr[m][k] = rest[3 * (size_t)vid[m] + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                ccp[k] = cp[3 * (size_t)j + k];
+                ccn[k] = cn[3 * (size_t)j + k];
+                ccb[k] = cb[3 * (size_t)j + k];
+            }
+            if (!FIRST) lam = clam[j];
+        }
         for (int c = 0; c < nc; c++) {
             const int cs = rng[2 * c], ce = rng[2 * c + 1];
             if (cs == ce) continue;  // the same for every thread
-            for (int k = cs + tid; k < ce; k += kBlock)
-                contact_gs_one(FIRST, clist[k], faces, rest, pos, lo, cp, cn, cb, clam);
+            if (mc == c) {
+                // world position = rest + ((base + low) + in-step motion), as world_pos
+                float4 d[3];
+                double x[3][3];
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    d[m] = pos[2 * (size_t)vid[m] + 1];
+                    const double u[3] = { sbl[m][0] + (double)d[m].x, sbl[m][1] + (double)d[m].y,
+                                          sbl[m][2] + (double)d[m].z };
+#pragma unroll
+                    for (int k = 0; k < 3; k++) x[m][k] = xr[m][k] + u[k];
+                }
+                const double* xs[3] = { x[0], x[1], x[2] };
+                double dx[9];
+                if (contact_dx(xs, w, ccp, ccn, ccb, &lam, dx)) {
+#pragma unroll
+                    for (int m = 0; m < 3; m++) {
+                        float4 dn = d[m];
+                        dn.x = (float)((double)dn.x + dx[3 * m]);
+                        dn.y = (float)((double)dn.y + dx[3 * m + 1]);
+                        dn.z = (float)((double)dn.z + dx[3 * m + 2]);
+                        pos[2 * (size_t)vid[m] + 1] = dn;
+                    }
+                }
+            }
+            for (int q = loc[c] + tid; q < loc[c + 1]; q += kBlock)
+                if (q >= kBlock) contact_gs_one(FIRST, clist[cs + (q - loc[c])], faces, rest, pos, lo, cp, cn, cb, clam);
             __syncthreads();
         }
-        __syncthreads();  // rng is rewritten by the next chunk
+        if (mc >= 0) clam[j] = lam;
+        __syncthreads();  // rng and loc are rewritten by the next chunk
     }
 }
 
