@@ -320,14 +320,16 @@ struct QuadLds {
 static_assert(QuadLds<64>::kMaxTets == kClusterTets && kClusterVerts == 8, "cluster shape of build_clusters");
 
 // One chunk of BLOCK / 4 consecutive clusters starting at cluster cb (clusters
-// below cend), projected by one workgroup and written back.
+// below cend), projected by one workgroup and written back.  Each wave owns 16
+// consecutive clusters and everything it stages in LDS (their vertex columns
+// and, in its own slice of the element planes, their elements), so the waves
+// need no workgroup barrier: a wave's LDS operations complete in order.
 template <bool FIRST, int BLOCK>
 __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& ta, const int4* __restrict__ cverts,
                                            const int* __restrict__ cstart, const uint32_t* __restrict__ tloc,
                                            float4* __restrict__ pos, int cb, int cend)
 {
     constexpr int kQuadBlock = BLOCK;
-    constexpr int kQuadClusters = QuadLds<BLOCK>::kClusters;
     constexpr int kQuadStage = QuadLds<BLOCK>::kStage;
     constexpr int kQuadCol = QuadLds<BLOCK>::kCol;
     constexpr int kMaxTets = QuadLds<BLOCK>::kMaxTets;
@@ -336,15 +338,21 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
     auto& sq = L.sq;
     auto& sl = L.sl;
     uint32_t* stl = L.stl;
+    constexpr int kWaveClusters = 16;                      // 64 lanes, 4 per cluster
+    constexpr int kWaveStage = kWaveClusters * kMaxTets;    // element slots of one wave
+    constexpr int kPer = kWaveStage / 64;                  // elements staged per lane
+    static_assert(kQuadStage == (kQuadBlock / 64) * kWaveStage, "element planes split into wave slices");
     const int tid = threadIdx.x;
-    const int cl = tid >> 2, j = tid & 3;
-    const int cl_end = min(cend, cb + kQuadClusters);
+    const int cl = tid >> 2, j = tid & 3, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int c = cb + cl;
     const bool active = c < cend;
-    // Every global load of the prologue is issued before the first wait: the
-    // cluster table and ranges, the element data of the workgroup's contiguous
-    // tet range (kPer tets per thread, into registers), then the vertex records
-    // (which need the cluster table); the LDS stores follow.
+    // clusters of this wave below cend (the same in every lane)
+    const int nwc = min(kWaveClusters, cend - (cb + wave * kWaveClusters));
+    if (nwc <= 0) return;  // no workgroup barrier follows
+    // The cluster table and ranges first; the wave's element range then comes
+    // from them (its clusters are consecutive), and its element data and the
+    // vertex records are loaded together.
     int4 va = make_int4(-1, -1, -1, -1), vb = va;
     int cs0 = 0, cs1 = 0;
     if (active) {
@@ -353,14 +361,14 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
         cs0 = cstart[c];
         cs1 = cstart[c + 1];
     }
-    const int tb = cstart[cb];
-    const int n = cstart[cl_end] - tb;
-    constexpr int kPer = kQuadStage / kQuadBlock;
+    const int tb = __builtin_amdgcn_readfirstlane(cs0);              // lane 0's cluster is the wave's first
+    const int n = __builtin_amdgcn_readlane(cs1, 4 * nwc - 1) - tb;  // end of its last active cluster
+    const int sbase = wave * kWaveStage;                             // the wave's slice of the element planes
     float eq[kPer][12], el[kPer][2];
     uint32_t etl[kPer];
 #pragma unroll
     for (int r = 0; r < kPer; r++) {
-        const int i = tid + r * kQuadBlock;
+        const int i = lane + r * 64;
         if (i < n) {
 #pragma unroll
             for (int k = 0; k < 9; k++) eq[r][k] = ta.Q[(size_t)k * ta.qstride + tb + i];
@@ -390,8 +398,8 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
     }
 #pragma unroll
     for (int r = 0; r < kPer; r++) {
-        const int i = tid + r * kQuadBlock;
-        if (i < n) {
+        const int i = sbase + lane + r * 64;
+        if (lane + r * 64 < n) {
             // Q is column-major (eq[3 c + r] = Q(r, c)); row 3 j + c of sq holds Q(j, c)
 #pragma unroll
             for (int cc = 0; cc < 3; cc++) {
@@ -430,10 +438,11 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
         sd[(q * 3 + 1) * kQuadCol + cl] = dhi.y;
         sd[(q * 3 + 2) * kQuadCol + cl] = dhi.z;
     }
-    __syncthreads();
+    // the wave reads only what it staged (LDS operations of one wave complete in order)
+    __builtin_amdgcn_wave_barrier();
     // A quad is active or inactive as a whole (an inactive quad has no
     // elements), so DPP reads stay inside active quads.
-    const int t0 = active ? cs0 - tb : 0;
+    const int t0 = active ? sbase + cs0 - tb : 0;
     const int nte = active ? cs1 - cs0 : 0;   // at most kMaxTets
     // What stays fixed during the solve is read once per element, before the
     // dependent chain: this lane's slot (as its LDS offset in sd), the base part
@@ -538,12 +547,12 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
         pos[2 * (size_t)v_hi + 1] = make_float4(sd[(qs * 3 + 0) * kQuadCol + cl], sd[(qs * 3 + 1) * kQuadCol + cl],
                                                 sd[(qs * 3 + 2) * kQuadCol + cl], 0.f);
     }
-    // multipliers of the workgroup's whole tet range, coalesced
-    __syncthreads();
+    // multipliers of the wave's tet range, coalesced
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int r = 0; r < kPer; r++) {
-        const int i = tid + r * kQuadBlock;
-        if (i < n) ta.lam[tb + i] = make_float2(sl[0][i], sl[1][i]);
+        const int i = lane + r * 64;
+        if (i < n) ta.lam[tb + i] = make_float2(sl[0][sbase + i], sl[1][sbase + i]);
     }
 }
 
