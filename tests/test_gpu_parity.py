@@ -425,6 +425,35 @@ def test_box_collider_matches_oracle():
         assert max(errs) <= TOL
 
 
+@pytest.mark.parametrize("variant", ["quad", "cluster"])
+def test_hundreds_of_contacts_in_one_instance_match_oracle(variant):
+    """A wide, slightly tilted box pressed 1.5 mm into the whole top of a
+    20 x 20 x 3-cell slab: ~870 contacts per step in one instance, so the
+    Gauss-Seidel contact kernel projects most of them past the 256 it loads up
+    front.  Well conditioned in the oracle (a 1e-11 perturbation stays 1e-11
+    over the run)."""
+    m = W.grid_object(20, 20, 3, 0.01, E=1e4, nu=0.45, pin_axis=2, noise=0.0, faces=True,
+                      rng=np.random.default_rng(3))
+    ax = np.array([0.3, 1.0, 0.0]) / np.linalg.norm([0.3, 1.0, 0.0])
+    quat = (*(ax * np.sin(0.01)), np.cos(0.01))
+    pos, size = (0.1, 0.1, 0.03 + 0.05 - 0.0015), (0.3, 0.3, 0.1)
+    gpu = X.Solver(m, dt=1e-3, iters=10, solver="gs", variant=variant, collision_every=1)
+    gpu.set_sdf(X.SDF_BOX, pos, quat, size)
+    o = orc.Sim(m, dt=1e-3, iters=10, mode="gs")
+    o.set_sdf(orc.SDF_BOX, pos, quat, size)
+    o.set_order(gpu.tet_order())
+    o.set_contact_order(gpu.face_rank())
+    errs, most = [], 0
+    for _ in range(10):
+        gpu.step(1)
+        o.step(1, detect=True)
+        most = max(most, len(o.contacts()[0]))
+        errs.append(rel_err(gpu.positions(), o.x()))
+    print(f"{variant}: up to {most} contacts per step, max rel err {max(errs):.3e}")
+    assert most > 600
+    assert max(errs) <= TOL
+
+
 def test_batch_box_detection_equals_oracle_on_the_same_positions():
     """Contact detection of a batch with randomly posed box colliders: after each
     step, the oracle's face-vs-SDF detection run on the device's own positions
