@@ -497,13 +497,6 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
 #pragma unroll
             for (int i = 0; i < 9; i++) ss += F[i] * F[i];
             const float CD = xsqrt<float>(ss);
-            const float J = det3cm<float>(F);
-            const float a = J - 1.f;
-            const float ch_log = -gamma + xlog<float>(J);
-            const float ch_series = -gamma + a - a * a * 0.5f + a * a * a * (1.f / 3.f);
-            const float CH = J >= 1.f ? ch_log : ch_series;
-            const float inv_C = xrcp<float>(CD);
-            const float fac = (J >= 1.f) ? xrcp<float>(J) : 1.f - a + a * a;
             float Fc[9];
             Fc[0] = F[4] * F[8] - F[5] * F[7];
             Fc[1] = F[5] * F[6] - F[3] * F[8];
@@ -514,6 +507,14 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
             Fc[6] = F[1] * F[5] - F[2] * F[4];
             Fc[7] = F[2] * F[3] - F[0] * F[5];
             Fc[8] = F[0] * F[4] - F[1] * F[3];
+            // J = det F by cofactors along column 0 (Fc[0..2] are its cofactors)
+            const float J = F[0] * Fc[0] + F[1] * Fc[1] + F[2] * Fc[2];
+            const float a = J - 1.f;
+            const float ch_log = -gamma + xlog<float>(J);
+            const float ch_series = -gamma + a - a * a * 0.5f + a * a * a * (1.f / 3.f);
+            const float CH = J >= 1.f ? ch_log : ch_series;
+            const float inv_C = xrcp<float>(CD);
+            const float fac = (J >= 1.f) ? xrcp<float>(J) : 1.f - a + a * a;
             float gD[3], gH[3];
 #pragma unroll
             for (int r = 0; r < 3; r++) {
