@@ -261,3 +261,30 @@ def test_network_file_round_trip(tmp_path):
         X.Network().load(str(bad))
     with pytest.raises(X.XpbdError):
         X.Network(glorot(6)).save(str(tmp_path / "unscaled.bin"))
+
+
+def test_learned_step_on_a_batch_falls_back_to_the_plain_batch_step():
+    """The learned step drives a batched handle as one object (features over the
+    concatenated vertices, one network pass, one residual over all elements);
+    its fallback reproduces the plain batched step bit for bit."""
+    meshes = [W.cube(4, seed=40), W.cube(5, seed=41), W.cube(3, seed=42)]
+    mk = lambda: X.Solver(meshes, dt=1e-3, iters=8, solver="gs", variant="quad")  # noqa: E731
+    plain, learned = mk(), mk()
+    for s in (plain, learned):
+        s.set_residual_policy(True)
+    net = X.Network(glorot(12) * 1e3)
+    net.set_scaling(np.zeros(24), np.full(24, 1e-6), 1.0)
+    learned.attach(net)
+    for k in range(4):
+        plain.step(1)
+        fb, r = learned.step_learned(2, ref_residual=plain.last_step_residual(), ratio=1.1)
+        assert fb, k
+        for i in range(len(meshes)):
+            assert np.array_equal(learned.instance_positions(i), plain.instance_positions(i)), (k, i)
+    # a sane network on the same batch: finite positions, residual reported
+    good = X.Network(glorot(13))
+    good.set_scaling(np.zeros(24), np.ones(24), 1e-6)
+    learned.attach(good)
+    fb, r = learned.step_learned(8, ref_residual=1e30, ratio=1.1)
+    assert not fb and np.isfinite(r)
+    assert all(np.all(np.isfinite(learned.instance_positions(i))) for i in range(len(meshes)))
