@@ -573,6 +573,44 @@ def test_edge_cases_match_oracle(case, solver, variant, det):
         np.testing.assert_array_equal(gpu.positions(), mesh.x0)
 
 
+def _eight_element_cells(nx, ny, nz, h=0.01, seed=0):
+    """A grid whose cells hold 8 elements on their 8 corners: the 6-element
+    split plus the two corner elements (000, 100, 010, 001) and (111, 011, 101,
+    110), which overlap it.  Consecutive in element order, a cell's 8 elements
+    fill one cluster."""
+    x, t6 = X.grid_mesh(nx, ny, nz, h)
+    cells = t6.reshape(-1, 6, 4)
+    tets = []
+    for c in cells:
+        vs = np.unique(c)
+        off = np.rint((x[vs] - x[vs].min(axis=0)) / h).astype(int)
+        corner = {tuple(o): v for o, v in zip(off, vs)}
+        extra = [[corner[(0, 0, 0)], corner[(1, 0, 0)], corner[(0, 1, 0)], corner[(0, 0, 1)]],
+                 [corner[(1, 1, 1)], corner[(0, 1, 1)], corner[(1, 0, 1)], corner[(1, 1, 0)]]]
+        tets.append(np.concatenate([c, np.array(extra, np.int32)]))
+    tets = np.concatenate(tets).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    nt = len(tets)
+    pinned = (np.abs(x[:, 2] - x[:, 2].min()) < 1e-9).astype(np.uint8)
+    return X.Mesh(x0=x, tets=tets, E=np.exp(rng.uniform(np.log(1e3), np.log(5e4), nt)),
+                  nu=rng.uniform(0.45, 0.49, nt), rho=np.full(nt, 1000.0), pinned=pinned,
+                  v0=rng.normal(0.0, 0.01, size=x.shape))
+
+
+@pytest.mark.parametrize("variant", ["cluster", "quad"])
+def test_clusters_of_eight_elements_match_oracle(variant):
+    """Clusters that hold the most elements a cluster can (8), which the
+    cluster kernels unroll for; hexahedral grids fill only 6 of them."""
+    mesh = _eight_element_cells(3, 4, 5, seed=23)
+    gpu, o = _pair(mesh, "gs", 10, variant=variant)
+    order = np.asarray(gpu.tet_order())
+    assert np.all(order[:8] // 8 == order[0] // 8)  # the first cluster is one whole cell
+    errs = run_pair(gpu, o, 50)
+    print(f"8-element clusters {variant}: max rel err {errs.max():.3e}")
+    assert np.all(np.isfinite(errs))
+    assert errs.max() <= TOL
+
+
 def test_batch_step_is_the_step_of_the_batch_handle():
     meshes, sdfs = _uneven_batch(4, (1e3, 5e4), seed=5)
     runs = []

@@ -315,6 +315,8 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
         std::vector<int32_t> cst, cv, ccol;
         std::vector<uint32_t> tl;
         const int ncl = build_clusters(nt, tets.data(), cst, cv, tl);
+        h->cluster_max_tets = 1;
+        for (int c = 0; c < ncl; c++) h->cluster_max_tets = std::max(h->cluster_max_tets, cst[c + 1] - cst[c]);
         const int nc = greedy_color(nv, ncl, kClusterVerts, cv.data(), ccol);
         if (nc <= 0) {
             delete h;
@@ -737,7 +739,7 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
                 else if (h->variant == XPBD_VARIANT_CLUSTER_QUAD)
                     e = launch_gs_cluster_quad(s, ta, h->d_cverts, h->d_cstart, h->d_tloc, h->d_pos,
                                                h->color_off[c], h->color_off[c + 1], first, h->quad_block,
-                                               h->xcd_map);
+                                               h->xcd_map, h->cluster_max_tets);
                 else
                     e = launch_gs_color(s, ta, h->d_pos, h->color_off[c], h->color_off[c + 1], first);
                 if (e != hipSuccess) return hip_err(h, e, "gs_color launch");

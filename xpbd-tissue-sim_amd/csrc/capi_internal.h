@@ -35,6 +35,7 @@ struct xpbd_handle {
     int ncolors = 1;
     int variant = XPBD_VARIANT_SIMPLE;         // resolved kernel variant
     int quad_block = 256;                      // CLUSTER_QUAD workgroup size (autotuned)
+    int cluster_max_tets = xpbd::kClusterTets; // most elements in one cluster
     bool xcd_map = true;                       // XCD-contiguous workgroup mapping (autotuned)
     std::string tuned;                         // last autotuner choice, "" if none
     std::vector<int32_t> cl_start;             // cluster variant: tet range of each cluster (solver order)

@@ -301,10 +301,13 @@ __device__ __forceinline__ float quad_lane3(float x)
 }
 
 // BLOCK threads per workgroup = BLOCK / 4 clusters; the launch shape is a tuning choice.
-template <int BLOCK>
+// MT bounds the elements of a cluster: 8 in general, 6 where every cluster is
+// one hexahedral cell, which leaves a wave's element planes a quarter smaller
+// and fits more waves in a CU's LDS.
+template <int BLOCK, int MT>
 struct QuadLds {
     static constexpr int kClusters = BLOCK / 4;
-    static constexpr int kMaxTets = 8;            // per cluster
+    static constexpr int kMaxTets = MT;           // per cluster
     static constexpr int kStage = kClusters * kMaxTets;
     static constexpr int kCol = kClusters + 1;    // padded column stride (LDS banks)
     float sb[8 * 4 * kCol];    // [slot][xyz, w][cluster]
@@ -317,22 +320,22 @@ struct QuadLds {
 };
 
 // The unrolled step loop covers every element of a cluster.
-static_assert(QuadLds<64>::kMaxTets == kClusterTets && kClusterVerts == 8, "cluster shape of build_clusters");
+static_assert(kClusterTets == 8 && kClusterVerts == 8, "cluster shape of build_clusters");
 
 // One chunk of BLOCK / 4 consecutive clusters starting at cluster cb (clusters
 // below cend), projected by one workgroup and written back.  Each wave owns 16
 // consecutive clusters and everything it stages in LDS (their vertex columns
 // and, in its own slice of the element planes, their elements), so the waves
 // need no workgroup barrier: a wave's LDS operations complete in order.
-template <bool FIRST, int BLOCK>
-__device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& ta, const int4* __restrict__ cverts,
+template <bool FIRST, int BLOCK, int MT>
+__device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT>& L, const TetArrays& ta, const int4* __restrict__ cverts,
                                            const int* __restrict__ cstart, const uint32_t* __restrict__ tloc,
                                            float4* __restrict__ pos, int cb, int cend)
 {
     constexpr int kQuadBlock = BLOCK;
-    constexpr int kQuadStage = QuadLds<BLOCK>::kStage;
-    constexpr int kQuadCol = QuadLds<BLOCK>::kCol;
-    constexpr int kMaxTets = QuadLds<BLOCK>::kMaxTets;
+    constexpr int kQuadStage = QuadLds<BLOCK, MT>::kStage;
+    constexpr int kQuadCol = QuadLds<BLOCK, MT>::kCol;
+    constexpr int kMaxTets = QuadLds<BLOCK, MT>::kMaxTets;
     float* sb = L.sb;
     float* sd = L.sd;
     auto& sq = L.sq;
@@ -340,7 +343,7 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
     uint32_t* stl = L.stl;
     constexpr int kWaveClusters = 16;                      // 64 lanes, 4 per cluster
     constexpr int kWaveStage = kWaveClusters * kMaxTets;    // element slots of one wave
-    constexpr int kPer = kWaveStage / 64;                  // elements staged per lane
+    constexpr int kPer = (kWaveStage + 63) / 64;           // elements staged per lane
     static_assert(kQuadStage == (kQuadBlock / 64) * kWaveStage, "element planes split into wave slices");
     const int tid = threadIdx.x;
     const int cl = tid >> 2, j = tid & 3, lane = tid & 63;
@@ -557,15 +560,15 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK>& L, const TetArrays& t
     }
 }
 
-template <bool FIRST, int BLOCK>
+template <bool FIRST, int BLOCK, int MT>
 __global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const int4* __restrict__ cverts,
                                                           const int* __restrict__ cstart,
                                                           const uint32_t* __restrict__ tloc,
                                                           float4* __restrict__ pos, int cbeg, int cend, int xcd)
 {
-    __shared__ QuadLds<BLOCK> L;
-    const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * QuadLds<BLOCK>::kClusters;
-    quad_chunk<FIRST, BLOCK>(L, ta, cverts, cstart, tloc, pos, cb, cend);
+    __shared__ QuadLds<BLOCK, MT> L;
+    const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * QuadLds<BLOCK, MT>::kClusters;
+    quad_chunk<FIRST, BLOCK, MT>(L, ta, cverts, cstart, tloc, pos, cb, cend);
 }
 
 // ---------------------------------------------------------------------------
@@ -1870,30 +1873,41 @@ hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cve
     return hipGetLastError();
 }
 
-template <int BLOCK>
+template <int BLOCK, int MT>
 static void launch_quad_shape(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                               const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, bool xcd)
 {
     const dim3 g(nblk(cend - cbeg, BLOCK / 4));
     if (first)
-        hipLaunchKernelGGL((k_gs_cluster_quad<true, BLOCK>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc, pos,
-                           cbeg, cend, (int)xcd);
+        hipLaunchKernelGGL((k_gs_cluster_quad<true, BLOCK, MT>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc,
+                           pos, cbeg, cend, (int)xcd);
     else
-        hipLaunchKernelGGL((k_gs_cluster_quad<false, BLOCK>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc, pos,
-                           cbeg, cend, (int)xcd);
+        hipLaunchKernelGGL((k_gs_cluster_quad<false, BLOCK, MT>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc,
+                           pos, cbeg, cend, (int)xcd);
+}
+
+template <int MT>
+static bool launch_quad_mt(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
+                           const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block, bool xcd)
+{
+    if (block == 64) launch_quad_shape<64, MT>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
+    else if (block == 128) launch_quad_shape<128, MT>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
+    else if (block == 256) launch_quad_shape<256, MT>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
+    else return false;
+    return true;
 }
 
 bool quad_block_supported(int block) { return block == 64 || block == 128 || block == 256; }
 
 hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                                   const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block,
-                                  bool xcd)
+                                  bool xcd, int max_tets)
 {
     if (cend <= cbeg) return hipSuccess;
-    if (block == 64) launch_quad_shape<64>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
-    else if (block == 128) launch_quad_shape<128>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
-    else if (block == 256) launch_quad_shape<256>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
-    else return hipErrorInvalidValue;
+    if (max_tets < 1 || max_tets > kClusterTets) return hipErrorInvalidValue;
+    const bool ok = max_tets <= 6 ? launch_quad_mt<6>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd)
+                                  : launch_quad_mt<8>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd);
+    if (!ok) return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
