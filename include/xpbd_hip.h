@@ -256,7 +256,9 @@ int xpbd_batch_get_positions(xpbd_batch* b, int32_t instance, double* out);
 int xpbd_batch_step(xpbd_batch* b, int32_t nsteps);
 /* Time the kernel variants and launch shapes that share this handle's device
  * layout on its own problem (Gauss-Seidel with clusters: the one-thread
- * cluster kernel and the quad kernel at 64/128/256 threads per workgroup;
+ * cluster kernel and the quad kernel at 64/128/256 threads per workgroup, each
+ * in its full and its LDS-lean form (vertex 3's weights summed in the quad and
+ * vertex bases read across lanes: more instructions, more waves per CU);
  * the patch Jacobi kernel with and without the fused one-launch iteration;
  * cluster and patch kernels each with and without the XCD-contiguous
  * workgroup mapping) as replays of a captured graph of the handle's
@@ -266,7 +268,7 @@ int xpbd_batch_step(xpbd_batch* b, int32_t nsteps);
  * same shape.  No reference counterpart: the reference has no kernel variants. */
 int xpbd_autotune(xpbd_batch* b, const char* cache_path);
 /* The kernel variant in use (XPBD_VARIANT_*); block: for CLUSTER_QUAD the
- * workgroup size, for PATCH 1 when the fused one-launch iteration is in use
+ * workgroup size (negative for the LDS-lean form), for PATCH 1 when the fused one-launch iteration is in use
  * (deterministic mode without attachments), else 0; xcd: whether the
  * XCD-contiguous workgroup mapping is on (block and xcd may be NULL). */
 int xpbd_get_variant(xpbd_handle* h, int32_t* variant, int32_t* block, int32_t* xcd);

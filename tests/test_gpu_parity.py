@@ -611,6 +611,31 @@ def test_clusters_of_eight_elements_match_oracle(variant):
     assert errs.max() <= TOL
 
 
+@pytest.mark.parametrize("mesh_kind", ["hexahedral cells", "8-element clusters"])
+def test_lds_lean_quad_form_gives_the_bits_of_the_full_form(tmp_path, mesh_kind):
+    """Both forms of the quad kernel, each forced through the tuning cache: the
+    lean form sums vertex 3's weights in the quad, in the order the full form
+    stages them, and moves vertex bases between lanes unchanged, so the two give
+    the same bits, and both match the oracle."""
+    mesh = W.cube(8, seed=21) if mesh_kind == "hexahedral cells" else _eight_element_cells(3, 4, 5, seed=23)
+    probe = X.Solver(mesh, dt=1e-3, iters=10, solver="gs", variant="cluster")
+    probe.autotune(str(tmp_path / "probe.txt"))
+    key = open(tmp_path / "probe.txt").read().split()[0]
+    out = {}
+    for block in (64, -64):
+        cache = tmp_path / f"tune{block}.txt"
+        cache.write_text(f"{key} {X.VARIANTS['quad']} {block} 1 0\n")
+        gpu = X.Solver(mesh, dt=1e-3, iters=10, solver="gs", variant="cluster")
+        assert gpu.autotune(str(cache)) == ("quad", block)
+        o = orc.Sim(mesh, dt=1e-3, g=9.81, iters=10, mode="gs")
+        o.set_order(gpu.tet_order())
+        errs = run_pair(gpu, o, 30)
+        print(f"{mesh_kind}, quad block {block}: max rel err {errs.max():.3e}")
+        assert errs.max() <= TOL
+        out[block] = gpu.positions()
+    assert np.array_equal(out[64], out[-64])
+
+
 def test_batch_step_is_the_step_of_the_batch_handle():
     meshes, sdfs = _uneven_batch(4, (1e3, 5e4), seed=5)
     runs = []

@@ -106,7 +106,10 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
         for (int x = 1; x >= 0; x--)
             for (const Candidate& b : { Candidate{ XPBD_VARIANT_CLUSTER, 0, x }, Candidate{ XPBD_VARIANT_CLUSTER_QUAD, 64, x },
                                         Candidate{ XPBD_VARIANT_CLUSTER_QUAD, 128, x },
-                                        Candidate{ XPBD_VARIANT_CLUSTER_QUAD, 256, x } })
+                                        Candidate{ XPBD_VARIANT_CLUSTER_QUAD, 256, x },
+                                        Candidate{ XPBD_VARIANT_CLUSTER_QUAD, -64, x },    // LDS-lean forms
+                                        Candidate{ XPBD_VARIANT_CLUSTER_QUAD, -128, x },
+                                        Candidate{ XPBD_VARIANT_CLUSTER_QUAD, -256, x } })
                 cand.push_back(b);
     } else if (h->variant == XPBD_VARIANT_PATCH) {
         // block 1: the fused one-launch iteration (when the layout allows it)

@@ -43,7 +43,8 @@ hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int 
 hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                              const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, bool xcd);
 // Same clusters and order, four lanes per cluster (one per vertex of the element
-// being projected); block = threads per workgroup (64, 128 or 256); max_tets =
+// being projected); block = threads per workgroup (64, 128 or 256; negative:
+// the same shape in the LDS-lean form, which fits more waves per CU); max_tets =
 // the most elements any cluster holds (at most kClusterTets).
 hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                                   const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block,

@@ -304,17 +304,23 @@ __device__ __forceinline__ float quad_lane3(float x)
 // MT bounds the elements of a cluster: 8 in general, 6 where every cluster is
 // one hexahedral cell, which leaves a wave's element planes a quarter smaller
 // and fits more waves in a CU's LDS.
-template <int BLOCK, int MT>
+// LEAN keeps only what the solve's dependent chain reads in LDS: vertex 3's
+// column of Q is then summed in the quad, and the vertex bases are fetched
+// from the staging lanes' registers before the chain (more instructions, less
+// LDS, more waves per CU).
+template <int BLOCK, int MT, bool LEAN>
 struct QuadLds {
     static constexpr int kClusters = BLOCK / 4;
     static constexpr int kMaxTets = MT;           // per cluster
     static constexpr int kStage = kClusters * kMaxTets;
     static constexpr int kCol = kClusters + 1;    // padded column stride (LDS banks)
-    float sb[8 * 4 * kCol];    // [slot][xyz, w][cluster]
-    float sd[8 * 3 * kCol];    // [slot][xyz][cluster]
+    static constexpr int kQRows = LEAN ? 9 : 12;
+    float sb[LEAN ? 1 : 8 * 4 * kCol];    // [slot][xyz, w][cluster]
+    float sd[8 * 3 * kCol];               // [slot][xyz][cluster]
     // row 3 j + c: Q(j, c), the gradient weights of element vertex j (vertex 3:
-    // -(Q(0, c) + Q(1, c) + Q(2, c))); rows 12, 13: alpha tilde; row 14: gamma
-    float sq[15][kStage];
+    // -(Q(0, c) + Q(1, c) + Q(2, c)), rows 9..11, not kept when LEAN); then
+    // alpha tilde (2 rows) and gamma
+    float sq[kQRows + 3][kStage];
     float sl[2][kStage];       // lambda
     uint32_t stl[kStage];      // local vertex slots
 };
@@ -327,15 +333,17 @@ static_assert(kClusterTets == 8 && kClusterVerts == 8, "cluster shape of build_c
 // consecutive clusters and everything it stages in LDS (their vertex columns
 // and, in its own slice of the element planes, their elements), so the waves
 // need no workgroup barrier: a wave's LDS operations complete in order.
-template <bool FIRST, int BLOCK, int MT>
-__device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT>& L, const TetArrays& ta, const int4* __restrict__ cverts,
+template <bool FIRST, int BLOCK, int MT, bool LEAN>
+__device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT, LEAN>& L, const TetArrays& ta, const int4* __restrict__ cverts,
                                            const int* __restrict__ cstart, const uint32_t* __restrict__ tloc,
                                            float4* __restrict__ pos, int cb, int cend)
 {
     constexpr int kQuadBlock = BLOCK;
-    constexpr int kQuadStage = QuadLds<BLOCK, MT>::kStage;
-    constexpr int kQuadCol = QuadLds<BLOCK, MT>::kCol;
-    constexpr int kMaxTets = QuadLds<BLOCK, MT>::kMaxTets;
+    using Lds = QuadLds<BLOCK, MT, LEAN>;
+    constexpr int kQuadStage = Lds::kStage;
+    constexpr int kQuadCol = Lds::kCol;
+    constexpr int kMaxTets = Lds::kMaxTets;
+    constexpr int kQR = Lds::kQRows;   // first row after Q's
     float* sb = L.sb;
     float* sd = L.sd;
     auto& sq = L.sq;
@@ -410,11 +418,11 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT>& L, const TetArray
                 sq[cc][i] = q0;
                 sq[3 + cc][i] = q1;
                 sq[6 + cc][i] = q2;
-                sq[9 + cc][i] = -(q0 + q1 + q2);
+                if constexpr (!LEAN) sq[9 + cc][i] = -(q0 + q1 + q2);
             }
-            sq[12][i] = eq[r][9];
-            sq[13][i] = eq[r][10];
-            sq[14][i] = eq[r][11];
+            sq[kQR][i] = eq[r][9];
+            sq[kQR + 1][i] = eq[r][10];
+            sq[kQR + 2][i] = eq[r][11];
             if (!FIRST) {
                 sl[0][i] = el[r][0];
                 sl[1][i] = el[r][1];
@@ -423,20 +431,24 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT>& L, const TetArray
         }
     }
     if (v_lo >= 0) {
-        sb[(j * 4 + 0) * kQuadCol + cl] = blo.x;
-        sb[(j * 4 + 1) * kQuadCol + cl] = blo.y;
-        sb[(j * 4 + 2) * kQuadCol + cl] = blo.z;
-        sb[(j * 4 + 3) * kQuadCol + cl] = blo.w;
+        if constexpr (!LEAN) {
+            sb[(j * 4 + 0) * kQuadCol + cl] = blo.x;
+            sb[(j * 4 + 1) * kQuadCol + cl] = blo.y;
+            sb[(j * 4 + 2) * kQuadCol + cl] = blo.z;
+            sb[(j * 4 + 3) * kQuadCol + cl] = blo.w;
+        }
         sd[(j * 3 + 0) * kQuadCol + cl] = dlo.x;
         sd[(j * 3 + 1) * kQuadCol + cl] = dlo.y;
         sd[(j * 3 + 2) * kQuadCol + cl] = dlo.z;
     }
     if (v_hi >= 0) {
         const int q = j + 4;
-        sb[(q * 4 + 0) * kQuadCol + cl] = bhi.x;
-        sb[(q * 4 + 1) * kQuadCol + cl] = bhi.y;
-        sb[(q * 4 + 2) * kQuadCol + cl] = bhi.z;
-        sb[(q * 4 + 3) * kQuadCol + cl] = bhi.w;
+        if constexpr (!LEAN) {
+            sb[(q * 4 + 0) * kQuadCol + cl] = bhi.x;
+            sb[(q * 4 + 1) * kQuadCol + cl] = bhi.y;
+            sb[(q * 4 + 2) * kQuadCol + cl] = bhi.z;
+            sb[(q * 4 + 3) * kQuadCol + cl] = bhi.w;
+        }
         sd[(q * 3 + 0) * kQuadCol + cl] = dhi.x;
         sd[(q * 3 + 1) * kQuadCol + cl] = dhi.y;
         sd[(q * 3 + 2) * kQuadCol + cl] = dhi.z;
@@ -449,9 +461,12 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT>& L, const TetArray
     const int nte = active ? cs1 - cs0 : 0;   // at most kMaxTets
     // What stays fixed during the solve is read once per element, before the
     // dependent chain: this lane's slot (as its LDS offset in sd), the base part
-    // of its row of D = (b_j - b_3) + (d_j - d_3), and its inverse mass.
+    // of its row of D = (b_j - b_3) + (d_j - d_3), and its inverse mass.  LEAN:
+    // slot s of a cluster is staged by lane s & 3 of its quad (low or high
+    // record) and is fetched from that lane's registers.
     int soff[kMaxTets];
     float Db[kMaxTets][3], wv[kMaxTets];
+    const float blv[4] = { blo.x, blo.y, blo.z, blo.w }, bhv[4] = { bhi.x, bhi.y, bhi.z, bhi.w };
 #pragma unroll
     for (int k = 0; k < kMaxTets; k++) {
         soff[k] = cl;
@@ -460,12 +475,22 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT>& L, const TetArray
         if (k < nte) {
             const int slot = (int)((stl[t0 + k] >> (4 * j)) & 15u);
             soff[k] = slot * 3 * kQuadCol + cl;
+            float bb[4];
+            if constexpr (LEAN) {
+                const int src = ((lane & ~3) | (slot & 3)) << 2;   // byte address of the owning lane
 #pragma unroll
-            for (int kk = 0; kk < 3; kk++) {
-                const float bb = sb[(slot * 4 + kk) * kQuadCol + cl];
-                Db[k][kk] = bb - quad_lane3(bb);
+                for (int kk = 0; kk < 4; kk++) {
+                    const float lo = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(blv[kk])));
+                    const float hi = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(bhv[kk])));
+                    bb[kk] = slot < 4 ? lo : hi;
+                }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) bb[kk] = sb[(slot * 4 + kk) * kQuadCol + cl];
             }
-            wv[k] = sb[(slot * 4 + 3) * kQuadCol + cl];
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) Db[k][kk] = bb[kk] - quad_lane3(bb[kk]);
+            wv[k] = bb[3];
         }
     }
 #pragma unroll
@@ -479,9 +504,19 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT>& L, const TetArray
 #pragma unroll
             for (int kk = 0; kk < 3; kk++) dd[kk] = sd[so + kk * kQuadCol];
 #pragma unroll
-            for (int cc = 0; cc < 3; cc++) qj[cc] = sq[3 * j + cc][e];
-            const float gamma = sq[14][e];
-            const float atD = sq[12][e], atH = sq[13][e];
+            for (int cc = 0; cc < 3; cc++) {
+                if constexpr (LEAN) {
+                    // vertex 3: -((Q(0, c) + Q(1, c)) + Q(2, c)), the quad sum with lane 3 adding zero
+                    const float x = j < 3 ? sq[3 * j + cc][e] : 0.f;
+                    const float y = x + dpp_xor1(x);
+                    const float sum = y + dpp_xor2(y);
+                    qj[cc] = j < 3 ? x : -sum;
+                } else {
+                    qj[cc] = sq[3 * j + cc][e];
+                }
+            }
+            const float gamma = sq[kQR + 2][e];
+            const float atD = sq[kQR][e], atH = sq[kQR + 1][e];
             const float l0 = FIRST ? 0.f : sl[0][e], l1 = FIRST ? 0.f : sl[1][e];
             // this lane's row of D (zero for vertex 3)
             float D[3];
@@ -560,15 +595,15 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT>& L, const TetArray
     }
 }
 
-template <bool FIRST, int BLOCK, int MT>
+template <bool FIRST, int BLOCK, int MT, bool LEAN>
 __global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const int4* __restrict__ cverts,
                                                           const int* __restrict__ cstart,
                                                           const uint32_t* __restrict__ tloc,
                                                           float4* __restrict__ pos, int cbeg, int cend, int xcd)
 {
-    __shared__ QuadLds<BLOCK, MT> L;
-    const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * QuadLds<BLOCK, MT>::kClusters;
-    quad_chunk<FIRST, BLOCK, MT>(L, ta, cverts, cstart, tloc, pos, cb, cend);
+    __shared__ QuadLds<BLOCK, MT, LEAN> L;
+    const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * QuadLds<BLOCK, MT, LEAN>::kClusters;
+    quad_chunk<FIRST, BLOCK, MT, LEAN>(L, ta, cverts, cstart, tloc, pos, cb, cend);
 }
 
 // ---------------------------------------------------------------------------
@@ -1873,31 +1908,44 @@ hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cve
     return hipGetLastError();
 }
 
-template <int BLOCK, int MT>
+template <int BLOCK, int MT, bool LEAN>
 static void launch_quad_shape(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                               const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, bool xcd)
 {
     const dim3 g(nblk(cend - cbeg, BLOCK / 4));
     if (first)
-        hipLaunchKernelGGL((k_gs_cluster_quad<true, BLOCK, MT>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc,
+        hipLaunchKernelGGL((k_gs_cluster_quad<true, BLOCK, MT, LEAN>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc,
                            pos, cbeg, cend, (int)xcd);
     else
-        hipLaunchKernelGGL((k_gs_cluster_quad<false, BLOCK, MT>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc,
+        hipLaunchKernelGGL((k_gs_cluster_quad<false, BLOCK, MT, LEAN>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc,
                            pos, cbeg, cend, (int)xcd);
 }
 
-template <int MT>
+template <int MT, bool LEAN>
 static bool launch_quad_mt(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                            const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block, bool xcd)
 {
-    if (block == 64) launch_quad_shape<64, MT>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
-    else if (block == 128) launch_quad_shape<128, MT>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
-    else if (block == 256) launch_quad_shape<256, MT>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
+    if (block == 64) launch_quad_shape<64, MT, LEAN>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
+    else if (block == 128) launch_quad_shape<128, MT, LEAN>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
+    else if (block == 256) launch_quad_shape<256, MT, LEAN>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
     else return false;
     return true;
 }
 
-bool quad_block_supported(int block) { return block == 64 || block == 128 || block == 256; }
+template <int MT>
+static bool launch_quad_lean(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
+                             const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block, bool xcd)
+{
+    // a negative shape selects the LDS-lean form
+    return block < 0 ? launch_quad_mt<MT, true>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, -block, xcd)
+                     : launch_quad_mt<MT, false>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd);
+}
+
+bool quad_block_supported(int block)
+{
+    const int b = block < 0 ? -block : block;
+    return b == 64 || b == 128 || b == 256;
+}
 
 hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                                   const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block,
@@ -1905,8 +1953,9 @@ hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4
 {
     if (cend <= cbeg) return hipSuccess;
     if (max_tets < 1 || max_tets > kClusterTets) return hipErrorInvalidValue;
-    const bool ok = max_tets <= 6 ? launch_quad_mt<6>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd)
-                                  : launch_quad_mt<8>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd);
+    const bool ok = max_tets <= 6
+                        ? launch_quad_lean<6>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd)
+                        : launch_quad_lean<8>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd);
     if (!ok) return hipErrorInvalidValue;
     return hipGetLastError();
 }
