@@ -190,6 +190,13 @@ int xpbd_num_colors(xpbd_handle* h, int32_t* out);
 int xpbd_get_tet_order(xpbd_handle* h, int32_t* out);
 /* Rank of every surface face in the contact visiting order (F entries). */
 int xpbd_get_face_rank(xpbd_handle* h, int32_t* out);
+/* Colour boundaries of the visiting order: ncolors + 1 positions into the
+ * xpbd_get_tet_order array (colour c visits positions [out[c], out[c+1])).
+ * No two tets of one colour in different clusters share a vertex, so a
+ * sequential Gauss-Seidel pass may run the clusters of one colour side by side
+ * (XPBDGaussSeidelSolver.hpp:29-80 visits the same order one tet at a time).
+ * Jacobi: {0, T}. */
+int xpbd_get_color_offsets(xpbd_handle* h, int32_t* out);
 
 /* ---- contacts ------------------------------------------------------------- */
 

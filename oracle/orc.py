@@ -77,6 +77,8 @@ _sig("orc_get_contact_lambdas", [C.c_void_p, _dp], None)
 _sig("orc_color_elements", [C.c_void_p, C.c_int], C.c_int)
 _sig("orc_set_threads", [C.c_void_p, C.c_int], None)
 _sig("orc_get_colors", [C.c_void_p, _ip, _ip, _ip], C.c_int)
+_sig("orc_set_order_levels", [C.c_void_p, _ip, C.c_int, _ip], C.c_int)
+_sig("orc_set_contacts", [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp])
 
 
 def project_contact(x9, w3, cp, cn, bary, lam=0.0):
@@ -182,6 +184,25 @@ class Sim:
     def set_order(self, order):
         o = np.ascontiguousarray(order, np.int32)
         lib.orc_set_order(self.s, p(o, C.c_int32))
+
+    def set_order_levels(self, order, level_off):
+        """Gauss-Seidel order cut into levels (the colours of a colour-ordered
+        solver): with set_threads > 1 the connected units of a level run on
+        separate threads, bit for bit the sequential pass over `order`.
+        Returns the number of units."""
+        o = np.ascontiguousarray(order, np.int32)
+        lo = np.ascontiguousarray(level_off, np.int32)
+        n = lib.orc_set_order_levels(self.s, p(o, C.c_int32), len(lo) - 1, p(lo, C.c_int32))
+        if n < 0:
+            raise ValueError("bad level offsets")
+        self._ncolor = len(lo) - 1
+        return n
+
+    def set_contacts(self, face, cp, cn, bary):
+        """Replace the contact set with the given contacts (multipliers 0)."""
+        f = np.ascontiguousarray(face, np.int32)
+        self._cset = (f, _d(cp).reshape(-1), _d(cn).reshape(-1), _d(bary).reshape(-1))
+        lib.orc_set_contacts(self.s, len(f), p(f, C.c_int32), p(self._cset[1]), p(self._cset[2]), p(self._cset[3]))
 
     def color_elements(self, cluster=True):
         """Greedy colouring of units (runs of consecutive elements over at most 8

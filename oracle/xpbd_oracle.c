@@ -677,6 +677,93 @@ ORC_API int orc_color_elements(orc_sim* s, int cluster)
     return ncol;
 }
 
+/* Union-find root of vertex v with path halving (parent[v] < 0: v is a root). */
+static int uf_root(int* parent, int v)
+{
+    while (parent[v] >= 0) {
+        if (parent[parent[v]] >= 0) parent[v] = parent[parent[v]];
+        v = parent[v];
+    }
+    return v;
+}
+
+/* Gauss-Seidel order given as levels: order[] visits the elements sequentially
+ * and level_off[0..nlevels] cuts it into ranges (the colours of a
+ * colour-ordered solver).  Within a level, elements are grouped into units =
+ * connected components of the shared-vertex graph restricted to that level;
+ * units keep their elements in order and share no vertex with each other, so
+ * projecting the units of a level concurrently (orc_set_threads > 1), each in
+ * order, is bit for bit the sequential pass over order[].  s->order becomes the
+ * level-major, unit-major arrangement of order[], which visits every pair of
+ * elements sharing a vertex in their relative order in order[].  Returns the
+ * number of units, or -1 on bad offsets. */
+ORC_API int orc_set_order_levels(orc_sim* s, const int* order, int nlevels, const int* level_off)
+{
+    const int nt = s->nt, nv = s->nv;
+    if (nlevels < 1 || level_off[0] != 0 || level_off[nlevels] != nt) return -1;
+    for (int l = 0; l < nlevels; l++)
+        if (level_off[l + 1] < level_off[l]) return -1;
+    int* parent = (int*)malloc(sizeof(int) * (size_t)(nv > 0 ? nv : 1));
+    for (int v = 0; v < nv; v++) parent[v] = -1;
+    int* comp = (int*)malloc(sizeof(int) * (size_t)(nt > 0 ? nt : 1));   /* unit of position i (level-local) */
+    int* root_unit = (int*)malloc(sizeof(int) * (size_t)(nv > 0 ? nv : 1));
+    for (int v = 0; v < nv; v++) root_unit[v] = -1;
+    int* unit_off = (int*)malloc(sizeof(int) * ((size_t)nt + 1));
+    int* color_off = (int*)malloc(sizeof(int) * ((size_t)nlevels + 1));
+    int* neword = (int*)malloc(sizeof(int) * (size_t)(nt > 0 ? nt : 1));
+    int nunit = 0, pos = 0;
+    for (int l = 0; l < nlevels; l++) {
+        const int b = level_off[l], en = level_off[l + 1];
+        color_off[l] = nunit;
+        for (int i = b; i < en; i++) {
+            const int* t = s->tets + 4 * order[i];
+            int r = uf_root(parent, t[0]);
+            for (int j = 1; j < 4; j++) {
+                const int q = uf_root(parent, t[j]);
+                if (q != r) parent[q] = r;
+            }
+        }
+        /* units numbered by first appearance; count their sizes */
+        const int u0 = nunit;
+        for (int i = b; i < en; i++) {
+            const int r = uf_root(parent, s->tets[4 * order[i]]);
+            if (root_unit[r] < 0) root_unit[r] = nunit++;
+            comp[i] = root_unit[r] - u0;
+        }
+        const int nu = nunit - u0;
+        int* cnt = (int*)calloc((size_t)nu + 1, sizeof(int));
+        for (int i = b; i < en; i++) cnt[comp[i] + 1]++;
+        for (int u = 0; u < nu; u++) cnt[u + 1] += cnt[u];
+        for (int u = 0; u < nu; u++) unit_off[u0 + u] = pos + cnt[u];
+        for (int i = b; i < en; i++) neword[pos + cnt[comp[i]]++] = order[i];
+        pos = en;
+        free(cnt);
+        /* reset the touched vertices for the next level */
+        for (int i = b; i < en; i++) {
+            const int* t = s->tets + 4 * order[i];
+            for (int j = 0; j < 4; j++) {
+                root_unit[t[j]] = -1;
+                parent[t[j]] = -1;
+            }
+        }
+    }
+    color_off[nlevels] = nunit;
+    unit_off[nunit] = nt;
+    memcpy(s->order, neword, sizeof(int) * (size_t)nt);
+    free(s->color_off);
+    free(s->unit_off);
+    s->color_off = color_off;
+    s->unit_off = unit_off;
+    s->nunit = nunit;
+    s->ncolor = nlevels;
+    s->order_set = 1;
+    free(parent);
+    free(comp);
+    free(root_unit);
+    free(neword);
+    return nunit;
+}
+
 ORC_API void orc_set_threads(orc_sim* s, int n) { s->nthreads = n; }
 
 /* After orc_color_elements: the order (nt), unit offsets (nunit + 1, positions)
@@ -783,8 +870,10 @@ ORC_API int orc_detect(orc_sim* s)
  * (VariadicVectorContainer insertion order).  A face rank, when set, permutes
  * that order: contacts are visited by increasing rank of their face, which is
  * how the colour-ordered device solver sequences them. */
-static const int* g_sort_rank;
-static const int* g_sort_face;
+/* sort keys of the qsort comparators, one per thread: oracle instances may be
+ * stepped concurrently from several threads */
+static __thread const int* g_sort_rank;
+static __thread const int* g_sort_face;
 static int cmp_contact(const void* a, const void* b)
 {
     const int ra = g_sort_rank[g_sort_face[*(const int*)a]];
@@ -812,6 +901,21 @@ ORC_API void orc_set_contact_order(orc_sim* s, const int* face_rank)
         s->face_rank = (int*)malloc(sizeof(int) * (size_t)(s->nf > 0 ? s->nf : 1));
         memcpy(s->face_rank, face_rank, sizeof(int) * s->nf);
     }
+    order_contacts(s);
+}
+
+/* Replace the contact set with n given contacts (face index, surface point p,
+ * normal nrm, barycentrics bary; 3 doubles each per contact), multipliers 0:
+ * the same clear-and-add as orc_detect (XPBDMeshObject.cpp:180-251), with the
+ * contacts supplied instead of detected.  Used to project a contact set that
+ * another detector (the device) produced from the same state, so that
+ * projection is compared independently of detection.  Visiting order: face
+ * rank if set (orc_set_contact_order), else the given order. */
+ORC_API void orc_set_contacts(orc_sim* s, int n, const int* face, const double* p, const double* nrm,
+                              const double* bary)
+{
+    s->nc = 0;
+    for (int c = 0; c < n; c++) add_contact(s, face[c], p + 3 * c, nrm + 3 * c, bary + 3 * c);
     order_contacts(s);
 }
 
@@ -975,7 +1079,7 @@ static void iterate(orc_sim* s)
  * The reference visits the element list in gather order; when a device order
  * is set (orc_set_order) the list is visited in that order instead, repeats of
  * an element back to back, which is how the colour-ordered device sequences it. */
-static const int* g_elem_rank;
+static __thread const int* g_elem_rank;
 static int cmp_elem(const void* a, const void* b)
 {
     const int ra = g_elem_rank[*(const int*)a], rb = g_elem_rank[*(const int*)b];

@@ -23,3 +23,11 @@ def run_pair(gpu, orc, steps, detect=False, sdf_motion=None, check_every=1, cont
         if (n + 1) % check_every == 0:
             errs.append(rel_err(gpu.positions(), orc.x()))
     return np.array(errs)
+
+
+def disp_err(x, ref, rest):
+    """max_i |x_i - ref_i| / max(|ref - rest|_inf, 1e-300): the position error in
+    units of the largest displacement from the rest shape, which does not hide
+    an error of one step's motion behind the size of the coordinates."""
+    ref = np.asarray(ref)
+    return float(np.max(np.abs(np.asarray(x) - ref)) / max(np.max(np.abs(ref - np.asarray(rest))), 1e-300))

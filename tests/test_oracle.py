@@ -656,3 +656,116 @@ def test_colour_ordered_run_equals_the_same_order_set_explicitly():
     a.step(10)
     b.step(10)
     assert np.array_equal(a.x(), b.x())
+
+
+def _levelled_order(m, rng, interleave):
+    """A colour-ordered Gauss-Seidel order with its colour boundaries, as a
+    colour-ordered device solver reports them: the oracle's own cluster
+    colouring, clusters shuffled within each colour, and (interleave) the
+    elements of two clusters alternated, which a sequential pass may do since
+    clusters of one colour share no vertex."""
+    s = orc.Sim(m, iters=1, mode="gs")
+    s.color_elements(cluster=True)
+    order, uoff, coff = s.colors()
+    new, level_off = [], [0]
+    for c in range(len(coff) - 1):
+        units = [list(order[uoff[u]:uoff[u + 1]]) for u in range(coff[c], coff[c + 1])]
+        rng.shuffle(units)
+        if interleave:
+            merged = []
+            for a, b in zip(units[0::2], units[1::2] + [[]]):
+                for k in range(max(len(a), len(b))):
+                    merged += a[k:k + 1] + b[k:k + 1]
+            units = [merged]
+        for u in units:
+            new += u
+        level_off.append(len(new))
+    return np.array(new, np.int32), np.array(level_off, np.int32)
+
+
+@pytest.mark.parametrize("interleave", [False, True])
+def test_levelled_order_runs_threaded_bit_for_bit(interleave):
+    """orc_set_order_levels groups each level into connected units; projecting
+    the units of a level on separate threads must reproduce the sequential pass
+    over the same order exactly (with contacts detected every step)."""
+    import workloads as W
+    m = W.grid_object(6, 5, 4, 0.01, pin_axis=2, noise=0.01, seed=8, faces=True)
+    order, level_off = _levelled_order(m, np.random.default_rng(3), interleave)
+    runs = []
+    for threads in (1, 4):
+        s = orc.Sim(m, iters=4, mode="gs")
+        if threads == 1:
+            s.set_order(order)
+        else:
+            n_units = s.set_order_levels(order, level_off)
+            assert n_units >= len(level_off) - 1
+            s.set_threads(threads)
+        s.set_sdf(orc.SDF_SPHERE, [0.03, 0.025, 0.04 + 0.02 - 0.004], [0, 0, 0, 1], [0.02, 0, 0])
+        most = 0
+        for _ in range(12):
+            s.step(1, detect=True)
+            most = max(most, len(s.contacts()[0]))
+        runs.append((s.x(), s.lambdas(), most))
+    assert runs[0][2] > 0
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_levelled_order_rejects_bad_offsets():
+    m = _mesh(2)
+    s = orc.Sim(m, iters=1, mode="gs")
+    with pytest.raises(ValueError):
+        s.set_order_levels(np.arange(m.nt), [0, m.nt - 1])
+
+
+def test_given_contacts_project_like_detected_ones():
+    """orc_set_contacts with the contact set orc_detect produced (same state)
+    gives the same step, bit for bit: the projection does not depend on where
+    the contacts came from."""
+    import workloads as W
+    m = W.grid_object(5, 5, 4, 0.01, pin_axis=2, noise=0.0, seed=9, faces=True)
+    sph = (orc.SDF_SPHERE, [0.025, 0.025, 0.04 + 0.02 - 0.003], [0, 0, 0, 1], [0.02, 0, 0])
+    a = orc.Sim(m, iters=5, mode="gs")
+    b = orc.Sim(m, iters=5, mode="gs")
+    for s in (a, b):
+        s.set_sdf(*sph)
+    a.step(1, detect=True)
+    b.set_state(a.x(), a.v())
+    a.detect()
+    face, cp, cn, bb = a.contacts()
+    assert len(face) > 0
+    # hand them over in reverse: the oracle orders contacts by face rank or input order
+    rank = np.arange(len(m.faces), dtype=np.int32)
+    a.set_contact_order(rank)
+    b.set_contact_order(rank)
+    b.set_contacts(face[::-1], cp[::-1], cn[::-1], bb[::-1])
+    got = b.contacts()
+    assert sorted(got[0]) == sorted(face)
+    a.step(1, detect=False)
+    b.step(1, detect=False)
+    assert np.array_equal(a.x(), b.x())
+
+
+def test_oracle_instances_step_concurrently():
+    """Separate oracle instances stepped from several threads (ctypes releases
+    the GIL) give the results of stepping them one after another: the oracle
+    keeps no shared mutable state across instances."""
+    from concurrent.futures import ThreadPoolExecutor
+    import workloads as W
+
+    def run(seed):
+        m = W.grid_object(10, 10, 2, 0.01, pin_axis=2, noise=0.02, seed=seed, faces=True)
+        s = orc.Sim(m, g=0.0, iters=1, mode="gs")
+        s.set_contact_order(np.arange(len(m.faces), dtype=np.int32)[::-1].copy())
+        s.set_local_collision_iters(1)
+        s.set_sdf(orc.SDF_BOX, [0.05, 0.05, 0.02 + 0.05 - 0.004], [0, 0, 0, 1], [0.1, 0.1, 0.1])
+        for _ in range(6):
+            s.step(1, detect=True)
+        return s.x()
+
+    seeds = list(range(48))
+    serial = [run(k) for k in seeds]
+    with ThreadPoolExecutor(8) as pool:
+        for _ in range(4):
+            threaded = list(pool.map(run, seeds))
+            for a, b in zip(serial, threaded):
+                assert np.array_equal(a, b)

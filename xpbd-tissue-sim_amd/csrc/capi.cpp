@@ -1213,6 +1213,14 @@ extern "C" int xpbd_get_tet_order(xpbd_handle* h, int32_t* out)
     return XPBD_OK;
 }
 
+extern "C" int xpbd_get_color_offsets(xpbd_handle* h, int32_t* out)
+{
+    if (!h || !out) return XPBD_E_INVALID_ARG;
+    for (int c = 0; c <= h->ncolors; c++)
+        out[c] = h->nclusters > 0 ? h->cl_start[h->color_off[c]] : h->color_off[c];
+    return XPBD_OK;
+}
+
 extern "C" int xpbd_get_face_rank(xpbd_handle* h, int32_t* out)
 {
     if (!h || !out) return XPBD_E_INVALID_ARG;

@@ -878,6 +878,13 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(uint32_t* __restrict__ p, 
 }
 
 // ---------------------------------------------------------------------------
+// FP64 code below (residual, energy, signed distances, detection, contact and
+// attachment projection) is compiled without FMA contraction: every product is
+// rounded before it is added, as in the FP64 restatement of the reference
+// arithmetic, so that detection from the same state takes the same branches
+// (Frank-Wolfe's vertex choice compares dot products that can tie exactly).
+#pragma clang fp contract(off)
+// ---------------------------------------------------------------------------
 // FP64 residual / energy (XPBDSolver.hpp:327-391, XPBDMeshObject.cpp:537-549).
 // The FP32 state is promoted and the constraint maths evaluated in double.
 // Partial sums use a fixed tree, so results are reproducible for a fixed grid.
@@ -1425,8 +1432,8 @@ __global__ __launch_bounds__(kBlock) void k_contacts_jacobi(const int4* __restri
     }
     clam[j] = lam;
 }
+#pragma clang fp contract(fast)
 
-// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // Fused patch Jacobi (fixed-order accumulation), one launch per iteration.
 // A shared vertex's value after the previous iteration is formed while it is
@@ -1636,6 +1643,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_fused_final(const int* __rest
 // contacts.  Attached vertices are distinct, so one thread per attachment is
 // the sequential order.
 // ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
 __device__ __forceinline__ void attach_dx(const double* x, const double* p, float w, double* lam, double* dx)
 {
     const double d[3] = { x[0] - p[0], x[1] - p[1], x[2] - p[2] };
@@ -1651,6 +1659,7 @@ __device__ __forceinline__ void attach_dx(const double* x, const double* p, floa
     *lam += dlam;
     for (int k = 0; k < 3; k++) dx[k] = (double)w * g[k] * dlam;
 }
+#pragma clang fp contract(fast)
 
 // Attachments a in [a0, n) by stride, each applied in place.
 __device__ __forceinline__ void attach_gs_pass(bool first, int a0, int stride, const int* __restrict__ av,

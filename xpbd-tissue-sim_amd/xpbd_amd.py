@@ -84,6 +84,7 @@ _sig("xpbd_last_step_residual", [_vp, _dp])
 _sig("xpbd_num_colors", [_vp, _ip])
 _sig("xpbd_get_tet_order", [_vp, _ip])
 _sig("xpbd_get_face_rank", [_vp, _ip])
+_sig("xpbd_get_color_offsets", [_vp, _ip])
 _sig("xpbd_set_sdf", [_vp, C.c_int32, _dp, _dp])
 _sig("xpbd_set_sdf_velocity", [_vp, _dp])
 _sig("xpbd_detect_contacts", [_vp])
@@ -352,6 +353,12 @@ class Solver:
     def face_rank(self):
         out = np.empty(self.nf, np.int32)
         self._c(lib.xpbd_get_face_rank(self.h, ptr(out, C.c_int32)))
+        return out
+
+    def color_offsets(self):
+        """Colour boundaries (ncolors + 1 positions) in the tet_order() array."""
+        out = np.empty(self.num_colors() + 1, np.int32)
+        self._c(lib.xpbd_get_color_offsets(self.h, ptr(out, C.c_int32)))
         return out
 
     def set_sdf(self, kind, pos, quat=(0, 0, 0, 1), params=(0, 0, 0), instance=None):
