@@ -1,20 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark: simulated steps/s and tet projections/s of the XPBD solver on one MI355X.
 
-Default workload: the heterogeneous tissue block (56^3 cells x 6 tets =
-1,053,696 tets, 185,193 vertices, base pinned), colour-ordered Gauss-Seidel
-with 20 iterations, dt = 1e-3, a sphere (r = 0.05 m) pressed down at 0.1 m/s
-with contacts re-detected every step.  Inputs are synthetic (seeded) and
-resident in HBM before timing; a step is one full time step (detect, predict,
-20 iterations, pin reset, velocity update) replayed from a HIP graph.
+Default workload (the JSON line's `value`): the heterogeneous tissue block
+(56^3 cells x 6 tets = 1,053,696 tets, 185,193 vertices, base pinned),
+colour-ordered Gauss-Seidel with 20 iterations, dt = 1e-3, a sphere
+(r = 0.05 m) pressed down at 0.1 m/s with contacts re-detected every step.
+Inputs are synthetic (seeded) and resident in HBM before timing; a step is one
+full time step (detect, predict, 20 iterations, pin reset, velocity update)
+replayed from a HIP graph.
 
-Prints one JSON line.  `roofline` covers the element-projection kernel
-(algorithmic bytes per solver iteration, 80 B/tet + 28 B/vertex, divided by
-the summed projection-kernel time per iteration, HIP events on the solver's
-stream); `cpu_baseline` times the FP64 oracle (single thread, same loop order
-as the reference CPU solver) on a bounded sample of the same workload, and
-`cpu_baseline_parallel` the same oracle on all host cores (OpenMP colour-ordered
-Gauss-Seidel over element clusters, or parallel Jacobi projection).
+Extra keys of the same line, each timed the same way with its own roofline:
+`beam_gs` and `beam_jacobi` (200x20x20-cell beam, 480,000 tets, 10
+iterations) and `batch` (256 cubes of 4..24 cells per axis, 10 GS iterations,
+a box collider each).
+
+`roofline` covers the element-projection kernel: algorithmic bytes per solver
+iteration (80 B/tet + 28 B/vertex) divided by the summed projection-kernel
+time per iteration (HIP events on the solver's stream).  `traffic` is the
+counter-measured HBM bytes per iteration of the same kernel shape, read from
+profiles/traffic_<workload>_<solver>.json when that file was recorded for the
+launch configuration the autotuner picked here (else null).  `cpu_baseline`
+times the FP64 oracle (single thread, same loop order as the reference CPU
+solver) on a bounded sample of the same workload, `cpu_baseline_parallel` the
+same oracle on all host cores.  `learned` compares the learned warm start with
+the plain solver on the same workload.
 """
 import argparse
 import json
@@ -29,6 +38,7 @@ sys.path.insert(0, os.path.join(ROOT, "xpbd-tissue-sim_amd"))
 HBM_PEAK_GBS = 8000.0
 KERNEL_NAMES = {"cluster": "k_gs_cluster x colours", "quad": "k_gs_cluster_quad x colours",
                 "simple": "k_gs_color x colours / k_jacobi_project", "patch": "k_jacobi_patch + k_jacobi_shared"}
+METRIC = "sim steps/sec & tet projections/sec at equal residual, learned vs plain, 1 GPU"
 
 
 def parse():
@@ -42,16 +52,30 @@ def parse():
     ap.add_argument("--variant", default="auto", choices=["auto", "simple", "cluster", "patch", "quad"])
     ap.add_argument("--nondeterministic", action="store_true", help="Jacobi: LDS float atomics")
     ap.add_argument("--timing-steps", type=int, default=5)
+    ap.add_argument("--extra-steps", type=int, default=100, help="timed steps of each extra workload (0: none)")
     ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--learned-steps", type=int, default=50, help="steps of the learned-vs-plain comparison (0: off)")
-    ap.add_argument("--learned-iters", type=int, default=3)
+    ap.add_argument("--learned-iters", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every step eagerly (kernel timing mode) instead of replaying HIP graphs")
-    ap.add_argument("--copy-calibration", type=float, default=0.0,
-                    help="MiB moved once by a plain device copy (scale for rocprof HBM byte counters)")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file (shared with xpbd_sim --tune-cache for counter runs)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xpbd-tissue-sim_amd", "weights", "warmstart.npz"))
     return ap.parse_args()
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"host_cpu": model or platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
 
 
 def build_workload(name, solver, iters, variant="auto", det=1):
@@ -89,6 +113,68 @@ def build_workload(name, solver, iters, variant="auto", det=1):
     return s, meshes, cfg, list(sdfs)
 
 
+def traffic_for(workload, solver, launch):
+    """Counter-measured bytes per iteration, if recorded for this launch configuration."""
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{workload}_{solver}.json")
+    if not os.path.exists(tfile):
+        return None, None, "no counter profile for this workload"
+    with open(tfile) as f:
+        tj = json.load(f)
+    rec = tj.get("launch_config")
+    if rec is None or list(rec) != list(launch):
+        return None, tj.get("kernel"), f"counter profile recorded for launch config {rec}, not {list(launch)}"
+    return tj.get("bytes_per_iteration"), tj.get("kernel"), None
+
+
+def measure(s, cfg, workload, steps, warmup, timing_steps, no_graph=False, tune_cache=None, variant="auto"):
+    """Autotune, time `steps` graph-replayed steps, then HIP-event kernel timing."""
+    if variant == "auto":
+        s.autotune(tune_cache)
+        cfg["autotuned"] = list(s.launch_config())
+    cfg["colors"] = s.num_colors()
+    T, V, iters = s.nt, s.nv, cfg["iterations"]
+    if no_graph:
+        s.set_kernel_timing(True)
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    steps_per_s = steps / el
+    # projection-kernel timing with HIP events on the solver stream (no graph)
+    s.set_kernel_timing(True)
+    s.step(timing_steps)
+    ms, nit, _ = s.kernel_stats()
+    s.set_kernel_timing(False)
+    bytes_iter = 80.0 * T + 28.0 * V
+    us_per_iter = 1e3 * ms / max(nit, 1)
+    achieved = bytes_iter / (us_per_iter * 1e-6) / 1e9
+    launch = list(s.launch_config())
+    traffic, traffic_kernel, note = traffic_for(workload, cfg["solver"], launch)
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": KERNEL_NAMES.get(launch[0], launch[0]) + " (one solver iteration)",
+            "launch_config": launch, "us_per_iteration": us_per_iter,
+            "algorithmic_bytes_per_iteration": bytes_iter, "iterations_timed": nit, "graph_replay": not no_graph}
+    if traffic_kernel:
+        roof["traffic_kernel"] = traffic_kernel
+    if note:
+        roof["traffic_note"] = note
+    return {"steps_per_s": steps_per_s, "ms_per_step": 1e3 / steps_per_s,
+            "tet_projections_per_s": steps_per_s * iters * T, "steps": steps, "roofline": roof}
+
+
+def _oracle_step_loop(o, sdf, cfg, nsteps):
+    t0 = time.perf_counter()
+    for n in range(nsteps):
+        if sdf is not None:
+            kind, pos, quat, prm, vel = sdf
+            o.set_sdf(kind, [pos[k] + vel[k] * n * cfg["dt"] for k in range(3)], quat, prm)
+        o.step(1, detect=sdf is not None)
+    return time.perf_counter() - t0
+
+
 def cpu_baseline_batch(meshes, cfg, sdfs, nsteps, stride=16):
     """FP64 oracle, one thread, on every stride-th instance of the batch; the
     reference steps objects one after another (Simulation.cpp:246-255), so the
@@ -108,12 +194,13 @@ def cpu_baseline_batch(meshes, cfg, sdfs, nsteps, stride=16):
     t_sample = sum(meshes[i].nt for i in idx)
     t_all = sum(m.nt for m in meshes)
     el_all = el * t_all / t_sample
-    return {"value": nsteps / el_all, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{nsteps} steps of {len(idx)} of the {len(meshes)} instances (every {stride}th, "
-                      f"{t_sample} of {t_all} tets), {cfg['iterations']} {cfg['solver']} iterations, box contacts, "
-                      f"one host core, FP64 oracle; whole-batch time extrapolated by element count",
-            "tet_proj_per_s": nsteps * cfg["iterations"] * t_sample / el,
-            "host": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
+    out = {"value": nsteps / el_all, "unit": "steps/s", "cores": 1, "kind": "port",
+           "sample": f"{nsteps} steps of {len(idx)} of the {len(meshes)} instances (every {stride}th, "
+                     f"{t_sample} of {t_all} tets), {cfg['iterations']} {cfg['solver']} iterations, box contacts, "
+                     f"one host core, FP64 oracle; whole-batch time extrapolated by element count",
+           "tet_proj_per_s": nsteps * cfg["iterations"] * t_sample / el}
+    out.update(host_info())
+    return out
 
 
 def cpu_baseline(meshes, cfg, sdf, nsteps):
@@ -124,21 +211,13 @@ def cpu_baseline(meshes, cfg, sdf, nsteps):
         return cpu_baseline_batch(meshes, cfg, sdf, nsteps)
     mesh = meshes[0]
     o = orc.Sim(mesh, dt=cfg["dt"], iters=cfg["iterations"], mode=cfg["solver"])
-    if sdf is not None:
-        kind, pos, quat, prm, vel = sdf
-        o.set_sdf(kind, pos, quat, prm)
-    t0 = time.perf_counter()
-    for n in range(nsteps):
-        if sdf is not None:
-            p = [pos[k] + vel[k] * n * cfg["dt"] for k in range(3)]
-            o.set_sdf(kind, p, quat, prm)
-        o.step(1, detect=sdf is not None)
-    el = time.perf_counter() - t0
-    return {"value": nsteps / el, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{nsteps} steps of the same workload ({mesh.nt} tets, {cfg['iterations']} "
-                      f"{cfg['solver']} iterations) on one host core, FP64 oracle",
-            "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el,
-            "host": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
+    el = _oracle_step_loop(o, sdf, cfg, nsteps)
+    out = {"value": nsteps / el, "unit": "steps/s", "cores": 1, "kind": "port",
+           "sample": f"{nsteps} steps of the same workload from rest ({mesh.nt} tets, {cfg['iterations']} "
+                     f"{cfg['solver']} iterations) on one host core, FP64 oracle",
+           "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el}
+    out.update(host_info())
+    return out
 
 
 def cpu_baseline_parallel(meshes, cfg, sdf, nsteps):
@@ -153,23 +232,15 @@ def cpu_baseline_parallel(meshes, cfg, sdf, nsteps):
     o = orc.Sim(mesh, dt=cfg["dt"], iters=cfg["iterations"], mode=cfg["solver"])
     ncol = o.color_elements(cluster=True) if cfg["solver"] == "gs" else 0
     o.set_threads(threads)
-    if sdf is not None:
-        kind, pos, quat, prm, vel = sdf
-        o.set_sdf(kind, pos, quat, prm)
-    t0 = time.perf_counter()
-    for n in range(nsteps):
-        if sdf is not None:
-            p = [pos[k] + vel[k] * n * cfg["dt"] for k in range(3)]
-            o.set_sdf(kind, p, quat, prm)
-        o.step(1, detect=sdf is not None)
-    el = time.perf_counter() - t0
+    el = _oracle_step_loop(o, sdf, cfg, nsteps)
     how = (f"colour-ordered GS, {ncol} colours of element clusters" if cfg["solver"] == "gs"
            else "Jacobi, projections in parallel")
-    return {"value": nsteps / el, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{nsteps} steps of the same workload ({mesh.nt} tets, {cfg['iterations']} "
-                      f"{cfg['solver']} iterations), FP64 oracle, {how}, {threads} OpenMP threads",
-            "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el,
-            "host": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
+    out = {"value": nsteps / el, "unit": "steps/s", "cores": threads, "kind": "port",
+           "sample": f"{nsteps} steps of the same workload from rest ({mesh.nt} tets, {cfg['iterations']} "
+                     f"{cfg['solver']} iterations), FP64 oracle, {how}, {threads} OpenMP threads",
+           "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el}
+    out.update(host_info())
+    return out
 
 
 MLP_FLOP_PER_VERTEX = 2 * (24 * 128 + 128 * 128 + 128 * 3)
@@ -177,11 +248,21 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def learned_vs_plain(a, net_path):
-    """Learned warm start (k iterations + network, fallback at 1.1x the plain
-    residual) against the plain solver on the same workload and steps."""
+    """Learned warm start against the plain solver on the same workload and
+    steps.  The plain run reads its end-of-solve residual back every step (the
+    reference for the fallback test, ratio 1.1); the learned runs take
+    a.learned_iters iterations per step:
+      with_fallback  network correction, multiplier start (the default);
+      no_correction  the same step with a network that outputs zero;
+      positions_start  network correction, multipliers from zero.
+    Each run reports its trajectory error against the plain run after the
+    last step, as max |x - x_plain| / max |x_plain| and in units of the plain
+    run's largest displacement from rest."""
     import numpy as np
     import train_warmstart as TW
-    s_plain, _, cfg, _ = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
+    import xpbd_amd as X
+    s_plain, meshes, cfg, _ = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
+    rest = np.concatenate([m.x0 for m in meshes])
     s_plain.set_residual_policy(True)
     r_plain, t_plain = [], 0.0
     for _ in range(a.learned_steps):
@@ -190,27 +271,25 @@ def learned_vs_plain(a, net_path):
         r_plain.append(s_plain.last_step_residual())
         t_plain += time.perf_counter() - t0
     x_plain = s_plain.positions()
-    trained_here = not os.path.exists(net_path)
-    if trained_here:
-        # no saved weights: train a small network on the device now
-        x, y = TW.build_dataset(2, 60)
-        import xpbd_amd as X
-        net = X.Network()
-        net.fit_scaling(x, y)
-        net.train(x, y, batch=8192, epochs=10, lr=2e-3, seed=7)
-    else:
-        net = TW.load(net_path)
+    s_plain.close()
+    disp = float(np.max(np.abs(x_plain - rest)))
+    net = TW.load(net_path)
+    zero = X.Network(np.zeros(X.MLP_PARAMS, np.float32))
+    zero.set_scaling(np.zeros(24), np.ones(24), 1.0)
     out = {"iterations": a.learned_iters, "plain_iterations": cfg["iterations"], "fallback_ratio": 1.1,
-           "steps": a.learned_steps, "weights": "trained in this run" if trained_here else os.path.relpath(net_path, ROOT),
-           "plain_steps_per_s_with_residual_readback": a.learned_steps / t_plain}
-    for mode, ref in (("with_fallback", r_plain), ("no_fallback", [0.0] * len(r_plain))):
+           "steps": a.learned_steps, "weights": os.path.relpath(net_path, ROOT),
+           "plain_steps_per_s_with_residual_readback": a.learned_steps / t_plain,
+           "plain_residual_median": float(np.median(r_plain))}
+    for mode, network, start in (("with_fallback", net, "multipliers"), ("no_correction", zero, "multipliers"),
+                                 ("positions_start", net, "positions")):
         s, _, _, _ = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
-        s.attach(net)
+        s.attach(network)
+        s.set_learned_start(start)
         s.set_kernel_timing(True)
         ratios, fb, t = [], 0, 0.0
         for n in range(a.learned_steps):
             t0 = time.perf_counter()
-            f, r = s.step_learned(a.learned_iters, ref[n], 1.1)
+            f, r = s.step_learned(a.learned_iters, r_plain[n], 1.1)
             t += time.perf_counter() - t0
             fb += int(f)
             ratios.append(r / max(r_plain[n], 1e-300))
@@ -218,11 +297,13 @@ def learned_vs_plain(a, net_path):
         x = s.positions()
         ratios = np.array(ratios)
         ms_inf = st["mlp_ms"] / max(st["mlp_calls"], 1)
-        out[mode] = {"steps_per_s": a.learned_steps / t, "fallback_rate": fb / a.learned_steps,
+        dx = float(np.max(np.abs(x - x_plain)))
+        out[mode] = {"start": start, "steps_per_s": a.learned_steps / t, "fallback_rate": fb / a.learned_steps,
                      "residual_ratio_median": float(np.median(ratios)), "residual_ratio_max": float(ratios.max()),
-                     "rel_pos_err_vs_plain_end": float(np.max(np.abs(x - x_plain)) / np.max(np.abs(x_plain))),
+                     "rel_pos_err_vs_plain_end": dx / float(np.max(np.abs(x_plain))),
+                     "traj_err_vs_plain_end_over_displacement": dx / disp,
                      "mlp_ms_per_inference": ms_inf}
-        if mode == "no_fallback":
+        if mode == "with_fallback":
             flop = MLP_FLOP_PER_VERTEX * s.nv
             out["mlp_tflops"] = flop / (ms_inf * 1e-3) / 1e12
             out["mlp_mfma_util"] = out["mlp_tflops"] / MFMA_BF16_PEAK_TFLOPS
@@ -239,79 +320,38 @@ def main():
                                    "described in DESIGN.md but not built)"}))
         sys.exit(2)
     import xpbd_amd as X  # noqa: F401  (fails loudly if the HIP library is missing)
-    if a.copy_calibration > 0:
-        # a copy of known size, so that FETCH_SIZE / WRITE_SIZE per dispatch can be scaled
-        import torch
-        n = int(a.copy_calibration * (1 << 20)) // 4
-        src = torch.ones(n, device="cuda", dtype=torch.float32)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_each_way": 4 * n}), file=sys.stderr)
     s, meshes, cfg, sdf = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
-    if a.variant == "auto":
-        # time the variants that share this handle's layout and keep the fastest
-        s.autotune()
-        cfg["autotuned"] = list(s.launch_config())
     cfg["variant"] = a.variant
     cfg["deterministic"] = not a.nondeterministic
-    cfg["colors"] = s.num_colors()
-    T, V = s.nt, s.nv
-    iters = cfg["iterations"]
-
-    if a.no_graph:
-        s.set_kernel_timing(True)
-    s.step(a.warmup)
-    s.sync()
-    t0 = time.perf_counter()
-    s.step(a.steps)
-    s.sync()
-    el = time.perf_counter() - t0
-    steps_per_s = a.steps / el
-    tet_proj_per_s = steps_per_s * iters * T
-
-    # projection-kernel timing with HIP events on the solver stream (no graph)
-    s.set_kernel_timing(True)
-    s.step(a.timing_steps)
-    ms, nit, nlaunch = s.kernel_stats()
-    s.set_kernel_timing(False)
-    bytes_iter = 80.0 * T + 28.0 * V
-    us_per_iter = 1e3 * ms / max(nit, 1)
-    achieved = bytes_iter / (us_per_iter * 1e-6) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{a.workload}_{cfg['solver']}.json")
-    traffic_kernel = None
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            tj = json.load(f)
-        traffic, traffic_kernel = tj.get("bytes_per_iteration"), tj.get("kernel")
-
+    m = measure(s, cfg, a.workload, a.steps, a.warmup, a.timing_steps, a.no_graph, a.tune_cache, a.variant)
     res = s.residual()
+    s.close()
     out = {
-        "metric": "sim steps/sec & tet projections/sec at equal residual, learned vs plain, 1 GPU",
-        "value": steps_per_s,
+        "metric": METRIC,
+        "value": m["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 / steps_per_s,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": cfg,
-        "tet_projections_per_s": tet_proj_per_s,
+        "tet_projections_per_s": m["tet_projections_per_s"],
         "residual_after_run": res,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": KERNEL_NAMES.get(s.variant()[0], s.variant()[0]) + " (one solver iteration)",
-                     "us_per_iteration": us_per_iter, "algorithmic_bytes_per_iteration": bytes_iter,
-                     "iterations_timed": nit, "graph_replay": not a.no_graph,
-                     "traffic_kernel": traffic_kernel},
+        "roofline": m["roofline"],
     }
+    if a.extra_steps > 0 and a.workload == "block":
+        for key, wl, solver in (("beam_gs", "beam", "gs"), ("beam_jacobi", "beam", "jacobi"), ("batch", "batch", "gs")):
+            se, _, ce, _ = build_workload(wl, solver, None, "auto", 1)
+            me = measure(se, ce, wl, a.extra_steps, 10, a.timing_steps, tune_cache=a.tune_cache)
+            se.close()
+            me["config"] = ce
+            out[key] = me
     if a.learned_steps > 0 and a.workload != "batch":
-        s.close()
         out["learned"] = learned_vs_plain(a, a.weights)
     if not a.no_cpu:
         n = a.cpu_steps if a.cpu_steps is not None else {"block": 10, "batch": 40}.get(a.workload, 20)

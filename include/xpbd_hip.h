@@ -347,14 +347,30 @@ int xpbd_features(xpbd_handle* h, float* out);
 /* Attach (or detach with NULL) a network to the handle for learned steps. */
 int xpbd_attach_mlp(xpbd_handle* h, xpbd_mlp* m);
 /* One learned step: predict, add the network's correction to the predicted
- * positions of the free vertices, then iters solver iterations (multipliers
- * from zero, as in XPBDSolver.hpp:66-97).  If ||r||_2 (r = C + alpha~ lambda at
+ * positions of the free vertices, start the solve as xpbd_set_learned_start
+ * selects, then iters solver iterations.  If ||r||_2 (r = C + alpha~ lambda at
  * the end of the solve) exceeds fallback_ratio * ref_residual, the step is
  * redone from the saved start-of-step state with the handle's full iteration
  * count, which reproduces the plain step exactly; *fell_back reports it and
  * *residual the final ||r||_2.  ref_residual <= 0 disables the fallback. */
 int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_residual, double fallback_ratio,
                       int32_t* fell_back, double* residual);
+
+/* Start of the learned step's solve.
+ * POSITIONS: every multiplier from zero, as the reference solve
+ *   (XPBDSolver.hpp:66-77); the network only moves the starting positions.
+ * MULTIPLIERS (default): the contacts are projected once from the corrected
+ *   positions (contact multipliers from zero), then every element's
+ *   multipliers start at lambda = -C / alpha~ there (C in FP64), the values
+ *   that satisfy its constraints; attachment multipliers start at zero and
+ *   the iterations keep all of them.  The elastic residual then starts at the
+ *   rounding of the FP32 multipliers whatever the correction, so the
+ *   residual test no longer judges the correction: the trajectory is as
+ *   accurate as the network (the positions are not moved by
+ *   M^-1 grad C^T lambda, the implicit-Euler momentum balance). */
+#define XPBD_LEARNED_START_POSITIONS 0
+#define XPBD_LEARNED_START_MULTIPLIERS 1
+int xpbd_set_learned_start(xpbd_handle* h, int32_t mode);
 /* Counters of the learned steps on h: steps taken, fallbacks, and (with kernel
  * timing on) the summed milliseconds and number of network inferences. */
 int xpbd_learned_stats(xpbd_handle* h, int64_t* steps, int64_t* fallbacks, double* mlp_ms,

@@ -79,6 +79,8 @@ _sig("orc_set_threads", [C.c_void_p, C.c_int], None)
 _sig("orc_get_colors", [C.c_void_p, _ip, _ip, _ip], C.c_int)
 _sig("orc_set_order_levels", [C.c_void_p, _ip, C.c_int, _ip], C.c_int)
 _sig("orc_set_contacts", [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp])
+_sig("orc_momentum_residual", [C.c_void_p, _dp, _dp], C.c_double)
+_sig("orc_step_warm", [C.c_void_p, C.c_int, _dp, C.c_int])
 
 
 def project_contact(x9, w3, cp, cn, bary, lam=0.0):
@@ -299,6 +301,20 @@ class Sim:
         """One step whose multipliers start at lam0 (T x 2) with the consistent position start."""
         l0 = None if lam0 is None else _d(lam0).reshape(-1)
         lib.orc_step_dual_warm(self.s, 1 if detect else 0, p(l0), iters)
+
+    def step_warm(self, corr, iters, detect=False):
+        """One step started at the predicted positions x~ + corr (V x 3, or None)
+        with every element's multipliers at -C / alpha~ there, then `iters`
+        iterations without a reset (the device's learned step)."""
+        c = None if corr is None else _d(corr).reshape(-1)
+        lib.orc_step_warm(self.s, 1 if detect else 0, p(c), iters)
+
+    def momentum_residual(self, xt, vec=False):
+        """||m (x - xt) - sum grad C^T lambda|| over free vertices after a step
+        whose inertial prediction was xt (V x 3)."""
+        out = np.empty((self.nv, 3)) if vec else None
+        n = lib.orc_momentum_residual(self.s, p(_d(xt)), p(out))
+        return (n, out) if vec else n
 
     def step_corrected(self, corr, iters, detect=False):
         """One step with `corr` (V x 3) added to the predicted positions before `iters` iterations."""

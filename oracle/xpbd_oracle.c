@@ -1389,6 +1389,109 @@ ORC_API double orc_residual(const orc_sim* s, double* vec)
     return sqrt(acc);
 }
 
+/* Momentum balance of the last solve (implicit Euler): for every free vertex i,
+ *     R_i = m_i (x_i - xt_i) - sum_e grad_i C_e(x)^T lambda_e   (Dev and Hyd),
+ * with xt the inertial prediction of the step (3*nv, given) and lambda the
+ * multipliers left by the solve (alpha~ lambda is the constraint force times
+ * dt^2, XPBDSolver.hpp:66-97 with CombinedNeohookeanConstraintProjector.cpp:98-179).
+ * Pinned vertices are left out (their constraint force is the support's).
+ * Returns ||R||_2; vec (3*nv, optional) receives R (zero on pinned vertices).
+ * Gauss-Seidel XPBD keeps R = 0 only up to the change of the gradients along
+ * the iterates; a start that sets lambda without moving x by M^-1 grad C^T
+ * lambda violates it by the difference. */
+ORC_API double orc_momentum_residual(const orc_sim* s, const double* xt, double* vec)
+{
+    const int nv = s->nv;
+    double* R = (double*)calloc(3 * (size_t)nv, sizeof(double));
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) R[3 * i + k] = s->m[i] * (s->x[3 * i + k] - xt[3 * i + k]);
+    for (int e = 0; e < s->nt; e++) {
+        const int* t = s->tets + 4 * e;
+        double x12[12], C[2], g[24];
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 3; k++) x12[3 * j + k] = s->x[3 * t[j] + k];
+        orc_eval_tet(x12, s->Q + 9 * e, s->gam[e], C, g);
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 3; k++)
+                R[3 * t[j] + k] -= g[3 * j + k] * s->lam[2 * e] + g[12 + 3 * j + k] * s->lam[2 * e + 1];
+    }
+    double acc = 0;
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) {
+            if (s->pin[i]) R[3 * i + k] = 0;
+            acc += R[3 * i + k] * R[3 * i + k];
+        }
+    if (vec) memcpy(vec, R, sizeof(double) * 3 * (size_t)nv);
+    free(R);
+    return sqrt(acc);
+}
+
+/* One contact pass from the current positions, as in iterate(): Gauss-Seidel
+ * applies each contact's correction at once (corder), Jacobi sums all of them
+ * from the same positions afterwards. */
+static void contact_pass(orc_sim* s)
+{
+    if (s->nc <= 0) return;
+    double* buf = s->mode == ORC_JACOBI ? (double*)calloc(9 * (size_t)s->nc, sizeof(double)) : NULL;
+    for (int ci = 0; ci < s->nc; ci++) {
+        const int c = buf ? ci : s->corder[ci];
+        const int* f = s->faces + 3 * s->c_face[c];
+        const double* p[3] = { s->x + 3 * f[0], s->x + 3 * f[1], s->x + 3 * f[2] };
+        const double w[3] = { s->w[f[0]], s->w[f[1]], s->w[f[2]] };
+        double dx[9];
+        if (project_contact(p, w, s->c_p + 3 * c, s->c_n + 3 * c, s->c_bary + 3 * c, s->c_lam + c, buf ? buf + 9 * c : dx) &&
+            !buf)
+            for (int i = 0; i < 3; i++)
+                for (int k = 0; k < 3; k++) s->x[3 * f[i] + k] += dx[3 * i + k];
+    }
+    if (buf) {
+        for (int c = 0; c < s->nc; c++) {
+            const int* f = s->faces + 3 * s->c_face[c];
+            for (int i = 0; i < 3; i++)
+                for (int k = 0; k < 3; k++) s->x[3 * f[i] + k] += buf[9 * c + 3 * i + k];
+        }
+        free(buf);
+    }
+}
+
+/* Multiplier warm start from predicted positions (the device's learned step):
+ * the positions start at the prediction xl = xt + corr (corr: 3*nv, NULL =
+ * zero; pinned vertices take corr as given), the contacts are projected once
+ * there from zero multipliers (contact_pass), then every element's
+ * multipliers are set to the values that satisfy its constraints,
+ *     lambda0_e = -C_e(x) / alpha~_e,
+ * attachment multipliers to 0, and iters iterations follow as in orc_step
+ * without a reset.  The start has zero elastic residual.  Unlike
+ * orc_step_learned it does not move the positions by M^-1 grad C^T lambda0, so
+ * the momentum balance holds only as well as corr predicts the solved step
+ * (see orc_momentum_residual). */
+ORC_API void orc_step_warm(orc_sim* s, int detect, const double* corr, int iters)
+{
+    if (detect) orc_detect(s);
+    const int nv = s->nv;
+    const double dt = s->dt;
+    move_inertially(s);
+    if (corr)
+        for (int i = 0; i < 3 * nv; i++) s->x[i] += corr[i];
+    for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
+    contact_pass(s);
+    for (int e = 0; e < s->nt; e++) {
+        const int* t = s->tets + 4 * e;
+        double F[9];
+        compute_F(s->x + 3 * t[0], s->x + 3 * t[1], s->x + 3 * t[2], s->x + 3 * t[3], s->Q + 9 * e, F);
+        s->lam[2 * e] = -dev_eval(F) / (s->aD[e] / s->ts);
+        s->lam[2 * e + 1] = -hyd_eval(F, s->gam[e]) / (s->aH[e] / s->ts);
+    }
+    for (int a = 0; a < s->na; a++) s->att_lam[a] = 0;
+    for (int it = 0; it < iters; it++) iterate(s);
+    s->last_res = orc_residual(s, NULL);
+    local_collision_iterations(s);
+    for (int i = 0; i < nv; i++)
+        if (s->pin[i])
+            for (int k = 0; k < 3; k++) s->x[3 * i + k] = s->xp[3 * i + k];
+    for (int i = 0; i < 3 * nv; i++) s->v[i] = (s->x[i] - s->xp[i]) / dt;
+}
+
 /* a12 total strain energy: XPBDMeshObject.cpp:537-549 (all Dev, then all Hyd,
  *     sum of C^2 / alpha, no 1/2 factor). */
 ORC_API double orc_energy(const orc_sim* s)

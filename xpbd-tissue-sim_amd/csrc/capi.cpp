@@ -669,8 +669,11 @@ extern "C" const char* xpbd_last_error(const xpbd_handle* h)
 // ---------------------------------------------------------------------------
 namespace xpbd {
 
-// Enqueue the solver iterations of one step (no predict / finish).
-int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
+// Enqueue the solver iterations of one step (no predict / finish).  With
+// reset_lambda (the reference solve, XPBDSolver.hpp:66-77) the first iteration
+// starts every multiplier at zero; without it every multiplier (element,
+// contact, attachment) keeps the value the caller left (a warm start).
+int enqueue_iterations(xpbd_handle* h, int iters, bool timed, bool reset_lambda)
 {
     hipStream_t s = h->stream;
     TetArrays ta = tet_arrays(h);
@@ -712,7 +715,7 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
             A.bwr = h->d_sbuf + (1 - par) * nsh;
             A.first_iter = it == 0;
             if (timed) timed_begin(h);
-            e = launch_jacobi_fused(s, ta, A, it == 0, con);
+            e = launch_jacobi_fused(s, ta, A, it == 0 && reset_lambda, con);
             if (e != hipSuccess) return hip_err(h, e, "jacobi_fused launch");
             if (timed) {
                 timed_end(h);
@@ -728,7 +731,7 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed)
         return XPBD_OK;
     }
     for (int it = 0; it < iters; it++) {
-        const bool first = it == 0;
+        const bool first = it == 0 && reset_lambda;
         if (h->params.solver == XPBD_SOLVER_GS_COLORED) {
             // one timed span covers the colour launches of the iteration (back to back on the stream)
             if (timed) timed_begin(h);

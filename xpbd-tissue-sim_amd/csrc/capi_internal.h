@@ -136,7 +136,7 @@ void invalidate_graphs(xpbd_handle* h);
 TetArrays tet_arrays(xpbd_handle* h);
 PatchArrays patch_arrays(xpbd_handle* h);
 bool contacts_enabled(const xpbd_handle* h);
-int enqueue_iterations(xpbd_handle* h, int iters, bool timed);
+int enqueue_iterations(xpbd_handle* h, int iters, bool timed, bool reset_lambda = true);
 int enqueue_detect(xpbd_handle* h);
 int enqueue_predict(xpbd_handle* h);
 int enqueue_finish(xpbd_handle* h);

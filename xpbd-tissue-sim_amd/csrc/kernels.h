@@ -35,6 +35,8 @@ hipError_t launch_predict(hipStream_t s, float4* pos, const float4* vel, int nv,
 hipError_t launch_predict_first_order(hipStream_t s, float4* pos, const float* dz1, int nv);
 hipError_t launch_finish(hipStream_t s, float4* pos, float4* lo, float4* vel, int nv, float dt,
                          unsigned long long* step_counter);
+// lambda = -C / alpha~ of every element at the current positions (learned step start)
+hipError_t launch_lambda_start(hipStream_t s, const TetArrays& ta, const float4* pos, const float4* lo, int nt);
 hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int begin, int end, bool first);
 // Cluster Gauss-Seidel over clusters [cbeg, cend) of one colour: cverts holds 8
 // vertex ids per cluster (-1 padded), cstart the tet range of each cluster,

@@ -908,6 +908,8 @@ struct Learned {
     double mlp_ms = 0;
     int64_t mlp_calls = 0;
     int64_t steps = 0, fallbacks = 0;
+    // XPBD_LEARNED_START_*: how the solve after the network's correction starts
+    int start = XPBD_LEARNED_START_MULTIPLIERS;
 };
 
 static void free_learned(Learned* L)
@@ -1025,6 +1027,31 @@ static int enqueue_store_cprev(xpbd_handle* h)
     return e == hipSuccess ? XPBD_OK : hip_err(h, e, "correction launch");
 }
 
+// Multiplier start of the learned step, on the corrected positions: the
+// contacts are projected once from zero multipliers (a Gauss-Seidel contact
+// pass, or the Jacobi contact sum), so that the start does not penetrate the
+// collider, then every element's multipliers are set to -C / alpha~ there.
+// Attachment multipliers start at zero.  The iterations that follow keep all
+// of them.
+static int enqueue_multiplier_start(xpbd_handle* h)
+{
+    hipStream_t s = h->stream;
+    hipError_t e = hipSuccess;
+    if (h->natt > 0) e = launch_fill_zero(s, h->d_att_lam, sizeof(double) * (size_t)h->natt);
+    if (e == hipSuccess && contacts_enabled(h)) {
+        if (h->params.solver == XPBD_SOLVER_GS_COLORED) {
+            e = launch_contacts_gs(s, h->d_faces, h->d_fci_off, h->nfcolors, h->ninst, h->d_cpos, h->d_clist,
+                                   h->d_rest, h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, true);
+        } else {
+            e = launch_contacts_jacobi(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
+                                       h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, true);
+            if (e == hipSuccess) e = launch_jacobi_apply(s, h->d_pos, h->d_acc, h->nv);
+        }
+    }
+    if (e == hipSuccess) e = launch_lambda_start(s, tet_arrays(h), h->d_pos, h->d_lo, h->nt);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "multiplier start");
+}
+
 }  // namespace xpbd
 
 using namespace xpbd;
@@ -1127,7 +1154,9 @@ extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_resid
     if (e != hipSuccess) return hip_err(h, e, "mlp launch");
     hipLaunchKernelGGL(k_apply_corr, dim3(nb(nv)), dim3(256), 0, s, h->d_pos, h->d_vel, L->d_corr, nv);
     if ((e = hipGetLastError()) != hipSuccess) return hip_err(h, e, "apply launch");
-    if ((rc = enqueue_iterations(h, iters, false))) return rc;
+    const bool warm = L->start == XPBD_LEARNED_START_MULTIPLIERS;
+    if (warm && (rc = enqueue_multiplier_start(h))) return rc;
+    if ((rc = enqueue_iterations(h, iters, false, !warm))) return rc;
     double r = 0;
     if ((rc = device_residual(h, &r, nullptr))) return rc;
     if (h->timing) {
@@ -1161,6 +1190,17 @@ extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_resid
     L->steps++;
     if (fell_back) *fell_back = fb ? 1 : 0;
     if (residual) *residual = r;
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_set_learned_start(xpbd_handle* h, int32_t mode)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (mode != XPBD_LEARNED_START_POSITIONS && mode != XPBD_LEARNED_START_MULTIPLIERS)
+        return set_err(h, XPBD_E_INVALID_ARG, "unknown learned start mode");
+    int rc = ensure_learned(h);
+    if (rc) return rc;
+    h->learned->start = mode;
     return XPBD_OK;
 }
 

@@ -17,6 +17,9 @@
 // --solver-type S   override every object's solver-type (a config solver-type name)
 // --autotune        time the kernel variants of every object after set-up and keep
 //                   the fastest (xpbd_autotune)
+// --tune-cache FILE with --autotune: reuse (and append to) the tuning cache FILE,
+//                   so a run takes the kernel shape an earlier run chose for the
+//                   same problem shape
 // --eager           launch every step without the HIP graph (kernel timing mode);
 //                   needed by profilers that cannot follow graph launches
 // --copy-calibration MB
@@ -56,7 +59,7 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "usage: %s CONFIG.yaml [--beam] [--steps N] [--dump-positions FILE] "
                              "[--dump-order FILE] [--dump-face-rank FILE] [--dump-rest FILE] [--eager] [--copy-calibration MB] "
                              "[--print-config] [--grasp R] [--cursor X,Y,Z] [--cursor-velocity VX,VY,VZ] "
-                             "[--dump-attachments FILE] [--autotune]\n", argv[0]);
+                             "[--dump-attachments FILE] [--autotune] [--tune-cache FILE]\n", argv[0]);
         return 2;
     }
     std::string cfg_path = argv[1], dump_pos, dump_order, dump_rank, dump_rest, dump_att;
@@ -69,7 +72,7 @@ int main(int argc, char** argv)
     };
     bool beam = false, print_only = false, load_meshes = false, eager = false;
     double copy_mb = 0;
-    std::string solver_override;
+    std::string solver_override, tune_cache;
     long steps = -1;
     for (int i = 2; i < argc; i++) {
         const std::string a = argv[i];
@@ -79,6 +82,7 @@ int main(int argc, char** argv)
         else if (a == "--eager") eager = true;
         else if (a == "--autotune") autotune = true;
         else if (a == "--solver-type" && i + 1 < argc) solver_override = argv[++i];
+        else if (a == "--tune-cache" && i + 1 < argc) tune_cache = argv[++i];
         else if (a == "--copy-calibration" && i + 1 < argc) copy_mb = std::atof(argv[++i]);
         else if (a == "--steps" && i + 1 < argc) steps = std::atol(argv[++i]);
         else if (a == "--dump-positions" && i + 1 < argc) dump_pos = argv[++i];
@@ -151,7 +155,7 @@ int main(int argc, char** argv)
         sim->setup();
         if (autotune)
             for (auto& o : sim->objects()) {
-                const int rc = xpbd_autotune(o->solver().handle(), nullptr);
+                const int rc = xpbd_autotune(o->solver().handle(), tune_cache.empty() ? nullptr : tune_cache.c_str());
                 if (rc != XPBD_OK) throw Sim::SimError(rc, "xpbd_autotune");
                 int32_t v = 0, b = 0, x = 0;
                 xpbd_get_variant(o->solver().handle(), &v, &b, &x);

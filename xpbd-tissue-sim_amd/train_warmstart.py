@@ -79,6 +79,9 @@ def save(path, net: X.Network, mean, std, scale, meta):
     net.save(os.path.splitext(path)[0] + ".bin")
 
 
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart.npz")
+
+
 def load(path) -> X.Network:
     z = np.load(path, allow_pickle=False)
     net = X.Network(z["params"])

@@ -85,6 +85,7 @@ _sig("xpbd_num_colors", [_vp, _ip])
 _sig("xpbd_get_tet_order", [_vp, _ip])
 _sig("xpbd_get_face_rank", [_vp, _ip])
 _sig("xpbd_get_color_offsets", [_vp, _ip])
+_sig("xpbd_set_learned_start", [_vp, C.c_int32])
 _sig("xpbd_set_sdf", [_vp, C.c_int32, _dp, _dp])
 _sig("xpbd_set_sdf_velocity", [_vp, _dp])
 _sig("xpbd_detect_contacts", [_vp])
@@ -443,6 +444,11 @@ class Solver:
         self._c(lib.xpbd_collect_training(self.h, nsteps, cap, ptr(x, C.c_float), ptr(y, C.c_float),
                                           C.byref(rows)))
         return x[:rows.value], y[:rows.value]
+
+    def set_learned_start(self, mode: str):
+        """'multipliers' (default: element multipliers at -C / alpha~ at the
+        corrected positions) or 'positions' (multipliers from zero)."""
+        self._c(lib.xpbd_set_learned_start(self.h, {"positions": 0, "multipliers": 1}[mode]))
 
     def step_learned(self, iters: int, ref_residual: float = 0.0, ratio: float = 1.1):
         """One learned step; returns (fell_back, residual)."""
