@@ -300,6 +300,12 @@ int xpbd_color_items(int32_t num_vertices, int32_t num_items, int32_t verts_per_
  * NULL arrays to get the counts, then again with arrays of that size. */
 int xpbd_read_msh(const char* path, int32_t* num_vertices, int32_t* num_tets, int32_t* num_tris,
                   double* x, int32_t* tets, int32_t* tris);
+/* Element classes file (element-classes-filename, XPBDMeshObjectConfig.hpp:63;
+ * read by XPBDMeshObject.cpp:112-149 with FileUtils::readVectorFromFile<int>):
+ * one integer per line, up to the first line without one.  *count receives
+ * the number read; classes (may be NULL) the first min(cap, count) of them.
+ * Each class indexes the object's material list (XPBDMeshObject.cpp:287-300). */
+int xpbd_read_element_classes(const char* path, int32_t cap, int32_t* classes, int32_t* count);
 
 /* ---- learned warm start ----------------------------------------------------- */
 

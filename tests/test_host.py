@@ -149,3 +149,50 @@ def test_null_handles_are_rejected():
     assert X.lib.xpbd_step(None, 1) == -1
     assert X.lib.xpbd_destroy(None) == 0
     assert X.lib.xpbd_last_error(None) is not None
+
+
+@pytest.mark.parametrize("name,nv,nt,nf", [("Prostate.msh", 2472, 9127, 3594),
+                                           ("tissue_block1x1x0.1_4subdiv.msh", 5417, 20952, 7680),
+                                           ("neuroma_segmentation_decimated_2000.msh", 1276, 4448, 2000),
+                                           ("2x1x1_block16.msh", 9537, 40960, 5120)])
+def test_reference_tissue_meshes_read(name, nv, nt, nf, tmp_path):
+    """Tissue meshes shipped with the reference (resource/tissue,
+    resource/demos/palpation), read by xpbd_read_msh: counts, element volumes
+    positive after orientation, the boundary of the tets is closed (every edge
+    of the extracted surface is shared by two faces) and covers the file's
+    triangles."""
+    import gzip
+    p = tmp_path / name
+    p.write_bytes(gzip.open(os.path.join(GOLDEN, "msh", name + ".gz")).read())
+    x, t, f = X.read_msh(str(p))
+    assert (len(x), len(t), len(f)) == (nv, nt, nf)
+    a, b, c, d = (x[t[:, k]] for k in range(4))
+    vol = np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a) / 6.0
+    assert np.all(np.abs(vol) > 0)
+    s = X.surface_faces(len(x), t)
+    edges = np.sort(np.concatenate([s[:, [0, 1]], s[:, [1, 2]], s[:, [2, 0]]]), axis=1)
+    _, counts = np.unique(edges, axis=0, return_counts=True)
+    assert np.all(counts == 2)
+    on_surface = {tuple(sorted(tri)) for tri in s.tolist()}
+    assert sum(tuple(sorted(tri)) in on_surface for tri in f.tolist()) >= 0.99 * len(f)
+
+
+def test_msh_without_nodes_section_is_refused(tmp_path):
+    """A file with a $MeshFormat header but no $Nodes / $Elements sections (as
+    resource/tissue/liver.msh, whose body is a bare node and element list) is
+    not read as an empty mesh."""
+    p = tmp_path / "bare.msh"
+    p.write_text("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n1 3\n1 0 0 0\n2 1 0 0\n3 0 1 0\n")
+    with pytest.raises(X.XpbdError, match="Nodes"):
+        X.read_msh(str(p))
+
+
+def test_element_classes_reader(tmp_path):
+    p = tmp_path / "c.txt"
+    p.write_text("0\n2\n1\n  7\nnot a class\n3\n")
+    assert X.read_element_classes(str(p)).tolist() == [0, 2, 1, 7]   # stops at the first non-integer line
+    E, nu, rho = X.materials_from_classes([0, 2, 1, 7, -1], [{"E": 1.0, "nu": 0.4, "density": 10.0},
+                                                             {"E": 2.0, "nu": 0.45, "density": 20.0}])
+    assert E.tolist() == [1.0, 2.0, 2.0, 2.0, 2.0] and rho.tolist() == [10.0, 20.0, 20.0, 20.0, 20.0]
+    with pytest.raises(X.XpbdError):
+        X.read_element_classes(str(tmp_path / "missing.txt"))

@@ -4,6 +4,7 @@ import json
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -78,7 +79,8 @@ def test_reference_style_config(tmp_path):
     assert o == {"name": "Cube1", "solver_type": "Parallel-Jacobi", "iters": 7, "local_collision_iters": 3,
                  "E": 6e4, "nu": 0.45,
                  "density": 1100, "collisions": False, "source": "mesh.msh", "position": [0, 0, 2],
-                 "first_order": False, "damping_multiplier": 1, "adjust_damping": False}
+                 "first_order": False, "damping_multiplier": 1, "adjust_damping": False,
+                 "materials": [{"name": "Soft Tissue", "E": 6e4, "nu": 0.45, "density": 1100}], "element_classes": ""}
 
 
 @pytest.mark.parametrize("bad,msg", [
@@ -152,3 +154,91 @@ objects:
     assert r.returncode == 0, r.stderr
     o = json.loads(r.stdout)["objects"][0]
     assert o["first_order"] is True and o["damping_multiplier"] == 20000 and o["adjust_damping"] is True
+
+
+def _gunzip(name, tmp_path):
+    import gzip
+    src = os.path.join(ROOT, "tests", "golden", "msh", name + ".gz")
+    dst = tmp_path / name
+    dst.write_bytes(gzip.open(src).read())
+    return str(dst)
+
+
+PALPATION_MATERIALS = """
+materials:
+  -
+    name: "Soft Tissue"
+    density: 1000
+    E: 6e4
+    nu: 0.45
+  - {name: "Stiffer Tissue", density: 1030, E: 3e5, nu: 0.49}
+"""
+
+
+def test_element_classes_select_materials(tmp_path):
+    """The reference's palpation demo: a 40,960-element block whose classes file
+    marks 6,161 elements as class 1 (resource/demos/palpation, config
+    config/demos/palpation/palpation.yaml).  Class c takes the object's c-th
+    material (XPBDMeshObject.cpp:287-300); a vertex's class is the largest of its
+    elements' (XPBDMeshObject.cpp:135-148)."""
+    msh = _gunzip("2x1x1_block16.msh", tmp_path)
+    cls = _gunzip("2x1x1_block16_element_classes.txt", tmp_path)
+    cfg = PALPATION_MATERIALS + f"""
+objects:
+  - name: "Cube1"
+    type: "XPBDMeshObject"
+    filename: "{msh}"
+    element-classes-filename: "{cls}"
+    materials: ["Soft Tissue", "Stiffer Tissue"]
+    max-size: 0.025
+"""
+    r = run(cfg, tmp_path, "--print-config", "--load-meshes")
+    assert r.returncode == 0, r.stderr
+    o = json.loads(r.stdout)["objects"][0]
+    assert [m["name"] for m in o["materials"]] == ["Soft Tissue", "Stiffer Tissue"]
+    assert o["materials"][1] == {"name": "Stiffer Tissue", "E": 3e5, "nu": 0.49, "density": 1030}
+    assert o["elements"] == 40960 and o["elements_per_material"] == [34799, 6161]
+    # independent count from the files: vertices of a class-1 element are class 1
+    import xpbd_amd as X
+    x, t, _ = X.read_msh(msh)
+    c = X.read_element_classes(cls)
+    v1 = np.zeros(len(x), bool)
+    v1[t[c == 1].ravel()] = True
+    assert o["vertices_per_class"] == [int((~v1).sum()), int(v1.sum())]
+
+
+def test_classes_past_the_material_list_take_the_last_material(tmp_path):
+    """One material for two classes: the reference warns and uses the last
+    material for every class past the end of the list."""
+    msh = _gunzip("2x1x1_block16.msh", tmp_path)
+    cls = _gunzip("2x1x1_block16_element_classes.txt", tmp_path)
+    cfg = PALPATION_MATERIALS + f"""
+objects:
+  - name: "Cube1"
+    type: "XPBDMeshObject"
+    filename: "{msh}"
+    element-classes-filename: "{cls}"
+    materials: ["Stiffer Tissue"]
+"""
+    r = run(cfg, tmp_path, "--print-config", "--load-meshes")
+    assert r.returncode == 0, r.stderr
+    assert "WARNING" in r.stderr
+    assert json.loads(r.stdout)["objects"][0]["elements_per_material"] == [40960]
+
+
+@pytest.mark.parametrize("bad,msg", [("materials: [\"Nope\"]", "no material named 'Nope'"),
+                                     ("element-classes-filename: \"SHORT\"", "entries for 40960 elements")])
+def test_material_and_class_errors(tmp_path, bad, msg):
+    msh = _gunzip("2x1x1_block16.msh", tmp_path)
+    short = tmp_path / "short.txt"
+    short.write_text("0\n1\n1\nnot a number\n0\n")
+    cfg = PALPATION_MATERIALS + f"""
+objects:
+  - name: "Cube1"
+    type: "XPBDMeshObject"
+    filename: "{msh}"
+    {bad.replace("SHORT", str(short))}
+"""
+    r = run(cfg, tmp_path, "--print-config", "--load-meshes")
+    assert r.returncode == 1
+    assert msg in r.stderr, r.stderr

@@ -769,3 +769,60 @@ def test_oracle_instances_step_concurrently():
             threaded = list(pool.map(run, seeds))
             for a, b in zip(serial, threaded):
                 assert np.array_equal(a, b)
+
+
+def _tissue(name, size, E=6e4, nu=0.45):
+    """A reference tissue mesh (tests/golden/msh) scaled to `size`, base 5% pinned."""
+    import gzip
+    import os
+    import tempfile
+    import xpbd_amd as X
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "msh", name + ".gz")
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, name)
+        with open(p, "wb") as f:
+            f.write(gzip.open(src).read())
+        x, t, _ = X.read_msh(p)
+    x = (x - x.min(axis=0)) * (size / np.max(x.max(axis=0) - x.min(axis=0)))
+    z = x[:, 2]
+    pinned = (z <= z.min() + 0.05 * (z.max() - z.min())).astype(np.uint8)
+    nt = len(t)
+    return X.Mesh(x0=x, tets=t, E=np.full(nt, E), nu=np.full(nt, nu), rho=np.full(nt, 1000.0), pinned=pinned,
+                  faces=X.surface_faces(len(x), t)), x, t
+
+
+def test_reference_jacobi_diverges_on_unstructured_meshes():
+    """The reference Jacobi adds every element's correction to a vertex with no
+    relaxation (XPBDJacobiSolver.hpp:79-114).  On an unstructured mesh at its
+    5 cm demo scale (neuroma_segmentation_decimated_2000.msh, E = 6e4, 5 mm/s
+    velocity noise) the FP64 oracle's positions leave the object's size after
+    one step, while Gauss-Seidel from the same start stays within it."""
+    m, x, _ = _tissue("neuroma_segmentation_decimated_2000.msh", 0.05)
+    m.v0 = np.random.default_rng(1).normal(0.0, 0.005, x.shape)   # 5 mm/s velocity noise
+    jac = orc.Sim(m, iters=10, mode="jacobi")
+    gs = orc.Sim(m, iters=10, mode="gs")
+    jac.step(1)
+    gs.step(1)
+    assert np.max(np.abs(jac.x() - x)) > 1.0
+    assert np.max(np.abs(gs.x() - x)) < 0.01
+
+
+@pytest.mark.parametrize("name,worst", [("Prostate.msh", 1e3), ("tissue_block1x1x0.1_4subdiv.msh", 500),
+                                        ("neuroma_segmentation_decimated_2000.msh", 500), ("2x1x1_block16.msh", 500)])
+def test_sliver_shape_factor_of_the_tissue_meshes(name, worst):
+    """How far an element amplifies displacement rounding into the strain:
+    |Q|_F times its longest edge (about 2 for a regular tet, unbounded for a
+    sliver).  Prostate.msh has slivers (factor > 1e3: three digits of an FP32
+    displacement lost in F); the other meshes stay small."""
+    m, x, t = _tissue(name, 0.5)
+    Q, vol = orc.Sim(m).rest()
+    P = x[t]
+    edges = np.stack([P[:, a] - P[:, b] for a, b in ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))], axis=1)
+    L = np.linalg.norm(edges, axis=2).max(axis=1)
+    k = np.linalg.norm(Q, axis=1) * L
+    print(f"{name}: shape factor median {np.median(k):.2f}, max {k.max():.3g}; smallest volume / median "
+          f"{vol.min() / np.median(vol):.2e}")
+    if worst >= 1e3:
+        assert k.max() > worst
+    else:
+        assert k.max() < worst

@@ -1451,3 +1451,14 @@ extern "C" int xpbd_read_msh(const char* path, int32_t* num_vertices, int32_t* n
     if (tris) std::memcpy(tris, vf.data(), sizeof(int32_t) * vf.size());
     return XPBD_OK;
 }
+
+extern "C" int xpbd_read_element_classes(const char* path, int32_t cap, int32_t* classes, int32_t* count)
+{
+    if (!path || !count || cap < 0 || (cap > 0 && !classes)) return XPBD_E_INVALID_ARG;
+    std::vector<int32_t> v;
+    std::string err;
+    if (read_int_lines(path, v, err)) return set_err(nullptr, XPBD_E_IO, err);
+    *count = (int32_t)v.size();
+    if (classes) std::memcpy(classes, v.data(), sizeof(int32_t) * std::min<size_t>((size_t)cap, v.size()));
+    return XPBD_OK;
+}

@@ -383,7 +383,7 @@ int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int3
     std::string line;
     std::unordered_map<long long, int32_t> tag2id;
     x.clear(); tets.clear(); tris.clear();
-    bool have_format = false;
+    bool have_format = false, have_nodes = false, have_elements = false;
     while (std::getline(in, line)) {
         if (line.rfind("$MeshFormat", 0) == 0) {
             std::getline(in, line);
@@ -393,8 +393,10 @@ int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int3
             if (ver < 4.0 || ver >= 5.0 || ftype != 0) { err = "only GMSH 4.x ASCII is supported"; return -2; }
             have_format = true;
         } else if (line.rfind("$Nodes", 0) == 0) {
+            have_nodes = true;
             long long nblocks, nnodes, mintag, maxtag;
             in >> nblocks >> nnodes >> mintag >> maxtag;
+            if (!in || nblocks < 0 || nnodes < 0) { err = "malformed $Nodes header"; return -6; }
             x.reserve(3 * (size_t)nnodes);
             for (long long b = 0; b < nblocks; b++) {
                 int edim, etag, param; long long n;
@@ -410,10 +412,13 @@ int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int3
                     tag2id[tags[i]] = (int32_t)(x.size() / 3);
                     x.insert(x.end(), p, p + 3);
                 }
+                if (!in) { err = "malformed $Nodes block"; return -6; }
             }
         } else if (line.rfind("$Elements", 0) == 0) {
+            have_elements = true;
             long long nblocks, nel, mintag, maxtag;
             in >> nblocks >> nel >> mintag >> maxtag;
+            if (!in || nblocks < 0 || nel < 0) { err = "malformed $Elements header"; return -6; }
             for (long long b = 0; b < nblocks; b++) {
                 int edim, etag, etype; long long n;
                 in >> edim >> etag >> etype >> n;
@@ -433,10 +438,33 @@ int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int3
                         }
                     }
                 }
+                if (!in) { err = "malformed $Elements block"; return -6; }
             }
         }
     }
     if (!have_format) { err = "missing $MeshFormat"; return -5; }
+    // a file without these sections is not a GMSH 4.1 mesh (the gmsh loader of
+    // MeshUtils.cpp:92-218 reads both), however its body is laid out
+    if (!have_nodes) { err = "missing $Nodes section"; return -7; }
+    if (!have_elements) { err = "missing $Elements section"; return -7; }
+    return 0;
+}
+
+// One integer per line until the first line that does not start with one
+// (FileUtils::readVectorFromFile<int>, include/utils/FileUtils.hpp:12-30).
+int read_int_lines(const std::string& path, std::vector<int32_t>& out, std::string& err)
+{
+    std::ifstream in(path);
+    if (!in) { err = "cannot open " + path; return -1; }
+    out.clear();
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        long long v;
+        if (!(ss >> v)) break;
+        if (v < INT32_MIN || v > INT32_MAX) { err = "value out of range in " + path; return -2; }
+        out.push_back((int32_t)v);
+    }
     return 0;
 }
 

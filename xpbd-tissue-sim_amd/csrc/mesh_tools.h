@@ -75,5 +75,8 @@ int build_patches(int nv, int nt, const int32_t* tets, const double* x, const in
 // 4-node tetrahedra (element type 4).  Triangles (type 2) are returned as faces.
 int read_msh41(const std::string& path, std::vector<double>& x,
                std::vector<int32_t>& tets, std::vector<int32_t>& tris, std::string& err);
+// Integers one per line, stopping at the first line without one (element
+// class files, FileUtils::readVectorFromFile<int>).
+int read_int_lines(const std::string& path, std::vector<int32_t>& out, std::string& err);
 
 }  // namespace xpbd

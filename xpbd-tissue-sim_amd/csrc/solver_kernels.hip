@@ -565,7 +565,10 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT, LEAN>& L, const Te
             const float A00 = atD + s00, A11 = atH + s11, A01 = s01;
             const float b0 = -CD - atD * l0;
             const float b1 = -CH - atH * l1;
-            const float inv_det = xrcp<float>(A00 * A11 - A01 * A01);
+            // SPD in exact arithmetic; a non-positive FP32 determinant (slivers) skips
+            // the projection, as in project_tet_F
+            const float det = A00 * A11 - A01 * A01;
+            const float inv_det = det > 0.f ? xrcp<float>(det) : 0.f;
             const float d0 = (b0 * A11 - b1 * A01) * inv_det;
             const float d1 = (b1 * A00 - b0 * A01) * inv_det;
             // the four lanes hold the same multipliers and store the same value

@@ -169,7 +169,12 @@ XPBD_HD void project_tet_F(const R* F, const R* w, const R* Q, R atD, R atH, R g
     const R A00 = atD + s00, A11 = atH + s11, A01 = s01;
     const R b0 = -CD - atD * lam[0];
     const R b1 = -CH - atH * lam[1];
-    const R inv_det = xrcp<R>(A00 * A11 - A01 * A01);
+    // A is a Gram matrix plus the positive compliances, so det > 0 in exact
+    // arithmetic; on sliver elements (compliance ~1e-6 of the Gram terms, whose
+    // determinant cancels at rest) FP32 rounding can make it <= 0, and the
+    // projection is then skipped instead of dividing by it
+    const R det = A00 * A11 - A01 * A01;
+    const R inv_det = det > R(0) ? xrcp<R>(det) : R(0);
     const R d0 = (b0 * A11 - b1 * A01) * inv_det;
     const R d1 = (b1 * A00 - b0 * A01) * inv_det;
     lam[0] += d0;

@@ -40,6 +40,21 @@ void opt(const yaml_lite::Node& n, const char* key, T& out, F conv)
 }
 }  // namespace
 
+// ElasticMaterialConfig.hpp:20-24 (name, density, E, nu, mu-s, mu-k)
+static ElasticMaterialConfig material_from_node(const yaml_lite::Node& m)
+{
+    ElasticMaterialConfig c;
+    auto S = [](const yaml_lite::Node& x) { return x.as_string(); };
+    auto D = [](const yaml_lite::Node& x) { return x.as_double(); };
+    opt(m, "name", c.name, S);
+    opt(m, "density", c.density, D);
+    opt(m, "E", c.E, D);
+    opt(m, "nu", c.nu, D);
+    opt(m, "mu-s", c.mu_s, D);
+    opt(m, "mu-k", c.mu_k, D);
+    return c;
+}
+
 XPBDMeshObjectConfig XPBDMeshObjectConfig::from_node(const yaml_lite::Node& n)
 {
     XPBDMeshObjectConfig c;
@@ -71,14 +86,12 @@ XPBDMeshObjectConfig XPBDMeshObjectConfig::from_node(const yaml_lite::Node& n)
     if (const yaml_lite::Node* m = n.find("position")) c.position = vec3(*m, "position");
     if (const yaml_lite::Node* m = n.find("velocity")) c.velocity = vec3(*m, "velocity");
     if (const yaml_lite::Node* m = n.find("rotation")) c.rotation = vec3(*m, "rotation");
-    if (const yaml_lite::Node* m = n.find("material")) {
-        opt(*m, "name", c.material.name, S);
-        opt(*m, "density", c.material.density, D);
-        opt(*m, "E", c.material.E, D);
-        opt(*m, "nu", c.material.nu, D);
-        opt(*m, "mu-s", c.material.mu_s, D);
-        opt(*m, "mu-k", c.material.mu_k, D);
+    if (const yaml_lite::Node* m = n.find("material")) c.material = material_from_node(*m);
+    if (const yaml_lite::Node* m = n.find("materials")) {
+        if (m->kind != yaml_lite::Node::Seq) throw Sim::SimError(XPBD_E_INVALID_ARG, "materials must be a list of names");
+        for (const yaml_lite::Node& x : m->seq) c.material_names.push_back(x.as_string());
     }
+    opt(n, "element-classes-filename", c.element_classes_filename, S);
     opt(n, "collisions", c.collisions, B);
     if (const yaml_lite::Node* m = n.find("num-solver-iters")) c.num_solver_iters = (int)m->as_int();
     if (const yaml_lite::Node* m = n.find("num-local-collision-iters"))
@@ -120,6 +133,10 @@ SimulationConfig SimulationConfig::from_node(const yaml_lite::Node& n)
     opt(n, "logging", c.logging, B);
     opt(n, "logging-output-folder", c.logging_output_folder, S);
     if (!(c.time_step > 0)) throw Sim::SimError(XPBD_E_INVALID_ARG, "time-step must be > 0");
+    if (const yaml_lite::Node* mats = n.find("materials")) {
+        if (mats->kind != yaml_lite::Node::Seq) throw Sim::SimError(XPBD_E_INVALID_ARG, "materials must be a list");
+        for (const yaml_lite::Node& m : mats->seq) c.materials.push_back(material_from_node(m));
+    }
     if (const yaml_lite::Node* objs = n.find("objects")) {
         if (objs->kind != yaml_lite::Node::Seq) throw Sim::SimError(XPBD_E_INVALID_ARG, "objects must be a list");
         for (const yaml_lite::Node& o : objs->seq) {
@@ -140,6 +157,18 @@ SimulationConfig SimulationConfig::from_node(const yaml_lite::Node& n)
                 c.xpbd_objects.push_back(XPBDMeshObjectConfig::from_node(o));
             }
         }
+    }
+    // resolve each object's material names against the top-level list
+    for (XPBDMeshObjectConfig& o : c.xpbd_objects) {
+        o.materials.clear();
+        for (const std::string& name : o.material_names) {
+            auto it = std::find_if(c.materials.begin(), c.materials.end(),
+                                   [&](const ElasticMaterialConfig& m) { return m.name == name; });
+            if (it == c.materials.end())
+                throw Sim::SimError(XPBD_E_INVALID_ARG, "object '" + o.name + "': no material named '" + name + "'");
+            o.materials.push_back(*it);
+        }
+        if (o.materials.empty()) o.materials.push_back(o.material);
     }
     return c;
 }
@@ -358,9 +387,48 @@ void XPBDMeshObject::loadAndConfigureMesh()
     _v0.assign(3 * (size_t)nv, 0.0);
     for (int i = 0; i < nv; i++)
         for (int k = 0; k < 3; k++) _v0[3 * i + k] = _cfg.velocity[k];
-    _E.assign(nt, _cfg.material.E);
-    _nu.assign(nt, _cfg.material.nu);
-    _rho.assign(nt, _cfg.material.density);
+    // element classes and materials, XPBDMeshObject.cpp:107-149 and 287-300
+    if (_cfg.materials.empty()) _cfg.materials.push_back(_cfg.material);
+    _elem_class.clear();
+    _vert_class.clear();
+    if (_cfg.element_classes_filename) {
+        const char* path = _cfg.element_classes_filename->c_str();
+        int32_t n = 0;
+        int rc = xpbd_read_element_classes(path, 0, nullptr, &n);
+        if (rc != XPBD_OK) throw SimError(rc, "cannot read element classes " + *_cfg.element_classes_filename);
+        if (n != nt)
+            throw SimError(XPBD_E_INVALID_ARG, "element classes file " + *_cfg.element_classes_filename + " has " +
+                                                   std::to_string(n) + " entries for " + std::to_string(nt) + " elements");
+        _elem_class.resize(nt);
+        rc = xpbd_read_element_classes(path, nt, _elem_class.data(), &n);
+        if (rc != XPBD_OK) throw SimError(rc, "cannot read element classes " + *_cfg.element_classes_filename);
+        _vert_class.assign(nv, 0);
+        for (int e = 0; e < nt; e++)
+            for (int j = 0; j < 4; j++) {
+                int& vc = _vert_class[_tets[4 * (size_t)e + j]];
+                vc = std::max(vc, _elem_class[e]);
+            }
+    }
+    const int nmat = (int)_cfg.materials.size();
+    _elem_mat.assign(nt, 0);
+    _E.resize(nt);
+    _nu.resize(nt);
+    _rho.resize(nt);
+    bool warned = false;
+    for (int e = 0; e < nt; e++) {
+        int m = elementClass(e);
+        if (static_cast<unsigned>(m) >= static_cast<unsigned>(nmat)) {
+            if (!warned)
+                std::fprintf(stderr, "WARNING: only %d materials were specified, but element %d has class %d; "
+                                     "such elements use the last material\n", nmat, e, m);
+            warned = true;
+            m = nmat - 1;
+        }
+        _elem_mat[e] = m;
+        _E[e] = _cfg.materials[m].E;
+        _nu[e] = _cfg.materials[m].nu;
+        _rho[e] = _cfg.materials[m].density;
+    }
     if (_cfg.fix_min_axis >= 0) {
         const AABB b = bbox();
         for (int v : getVerticesWithAxisValue(_cfg.fix_min_axis, b.min[_cfg.fix_min_axis])) _fixed[v] = 1;

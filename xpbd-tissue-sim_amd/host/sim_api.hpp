@@ -64,7 +64,14 @@ struct XPBDMeshObjectConfig {
     std::optional<double> max_size;
     std::optional<Sim::Vec3r> size;
     Sim::Vec3r position{ 0, 0, 0 }, velocity{ 0, 0, 0 }, rotation{ 0, 0, 0 };
+    // elastic materials: "materials: [names]" refers to the simulation's
+    // top-level material list (XPBDMeshObjectConfig.hpp:62, 112); an inline
+    // "material: {...}" gives a one-material list.  Element class c uses
+    // materials[c] (the last one for classes past the end, XPBDMeshObject.cpp:287-300).
     ElasticMaterialConfig material;
+    std::vector<std::string> material_names;
+    std::vector<ElasticMaterialConfig> materials;         // resolved list (at least one)
+    std::optional<std::string> element_classes_filename;  // element-classes-filename (:63)
     bool collisions = false;
     int num_solver_iters = 1;
     int num_local_collision_iters = 0;  // num-local-collision-iters (XPBDMeshObjectConfig.hpp:68)
@@ -87,6 +94,7 @@ struct SimulationConfig {
     std::string sim_mode = "AFAP", visualization = "None";
     bool logging = false;
     std::string logging_output_folder = ".";
+    std::vector<ElasticMaterialConfig> materials;  // top-level "materials:" list
     std::vector<XPBDMeshObjectConfig> xpbd_objects;
     std::vector<RigidObjectConfig> rigid_objects;
 
@@ -183,6 +191,13 @@ public:
     int numVertices() const { return (int)_x0.size() / 3; }
     int numElements() const { return (int)_tets.size() / 4; }
     int numFaces() const { return (int)_faces.size() / 3; }  // surface faces (objects with collisions)
+    // element class (0 without an element classes file) and vertex class, the
+    // largest class of the vertex's elements (XPBDMeshObject.cpp:112-149)
+    int elementClass(int e) const { return _elem_class.empty() ? 0 : _elem_class.at(e); }
+    int vertexClass(int v) const { return _vert_class.empty() ? 0 : _vert_class.at(v); }
+    // material of each element after the class lookup (an index into config().materials)
+    int elementMaterialIndex(int e) const { return _elem_mat.at(e); }
+    const Config::ElasticMaterialConfig& elementMaterial(int e) const { return _cfg.materials.at(_elem_mat.at(e)); }
     const std::vector<double>& restVertices() const { return _x0; }
     const std::vector<double>& vertices() const { return _x; }  // refreshed after each update
     Solver::XPBDGPUSolver& solver() { return *_solver; }
@@ -196,7 +211,7 @@ private:
     Config::XPBDMeshObjectConfig _cfg;
     double _dt, _g;
     std::vector<double> _x0, _x, _v0, _E, _nu, _rho, _masses;
-    std::vector<int32_t> _tets, _faces;
+    std::vector<int32_t> _tets, _faces, _elem_class, _vert_class, _elem_mat;
     std::vector<uint8_t> _fixed;
     struct Attachment {
         int v;

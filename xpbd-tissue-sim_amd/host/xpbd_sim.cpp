@@ -34,11 +34,13 @@
 //                   object 0's grasped vertex ids, raw int32
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include "sim_api.hpp"
 
@@ -117,6 +119,12 @@ int main(int argc, char** argv)
                             o.grid ? "grid" : o.filename.c_str(), o.position[0], o.position[1], o.position[2],
                             o.first_order ? "true" : "false", o.damping_multiplier,
                             o.adjust_damping_to_material ? "true" : "false");
+                std::printf(", \"materials\": [");
+                for (size_t k = 0; k < o.materials.size(); k++)
+                    std::printf("%s{\"name\": \"%s\", \"E\": %.9g, \"nu\": %.9g, \"density\": %.9g}", k ? ", " : "",
+                                o.materials[k].name.c_str(), o.materials[k].E, o.materials[k].nu, o.materials[k].density);
+                std::printf("], \"element_classes\": \"%s\"",
+                            o.element_classes_filename ? o.element_classes_filename->c_str() : "");
                 if (load_meshes) {
                     // host-side mesh set-up only (no device call)
                     Sim::XPBDMeshObject m(o, cfg.time_step, cfg.g_accel);
@@ -126,6 +134,19 @@ int main(int argc, char** argv)
                                 "\"bbox\": [[%.9g, %.9g, %.9g], [%.9g, %.9g, %.9g]]",
                                 m.numVertices(), m.numElements(), m.numFaces(), b.min[0], b.min[1], b.min[2], b.max[0],
                                 b.max[1], b.max[2]);
+                    // elements per material after the class lookup, vertices per vertex class
+                    std::vector<long> per_mat(m.config().materials.size(), 0), per_vclass;
+                    for (int e = 0; e < m.numElements(); e++) per_mat[m.elementMaterialIndex(e)]++;
+                    for (int v = 0; v < m.numVertices(); v++) {
+                        const int c = std::max(0, m.vertexClass(v));
+                        if ((size_t)c >= per_vclass.size()) per_vclass.resize(c + 1, 0);
+                        per_vclass[c]++;
+                    }
+                    std::printf(", \"elements_per_material\": [");
+                    for (size_t k = 0; k < per_mat.size(); k++) std::printf("%s%ld", k ? ", " : "", per_mat[k]);
+                    std::printf("], \"vertices_per_class\": [");
+                    for (size_t k = 0; k < per_vclass.size(); k++) std::printf("%s%ld", k ? ", " : "", per_vclass[k]);
+                    std::printf("]");
                 }
                 std::printf("}");
             }

@@ -142,7 +142,7 @@ Node Parser::parse_block(size_t& i, int indent)
             if (rest.empty()) {
                 if (i < L.size() && L[i].indent > indent) n.seq.push_back(parse_block(i, L[i].indent));
                 else n.seq.push_back(Node());
-            } else if (key_colon(rest) != std::string::npos && rest.front() != '[') {
+            } else if (key_colon(rest) != std::string::npos && rest.front() != '[' && rest.front() != '{') {
                 // "- key: value" starts a mapping whose further keys are indented past the dash
                 const int item_indent = indent + 2;
                 std::vector<Line> sub{ { item_indent, rest, ln.number } };

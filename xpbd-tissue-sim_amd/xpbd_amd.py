@@ -102,6 +102,7 @@ _sig("xpbd_mesh_grid", [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
 _sig("xpbd_mesh_surface", [C.c_int32, C.c_int32, _ip, _ip, _ip])
 _sig("xpbd_color_items", [C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _ip])
 _sig("xpbd_read_msh", [C.c_char_p, _ip, _ip, _ip, _dp, _ip, _ip])
+_sig("xpbd_read_element_classes", [C.c_char_p, C.c_int32, _ip, _ip])
 _sig("xpbd_mlp_create", [C.POINTER(_vp)])
 _sig("xpbd_mlp_destroy", [_vp])
 _sig("xpbd_mlp_num_params", [_ip])
@@ -175,6 +176,27 @@ def read_msh(path: str):
     _check(lib.xpbd_read_msh(path.encode(), C.byref(nv), C.byref(nt), C.byref(nf), ptr(x, C.c_double),
                              ptr(t, C.c_int32), ptr(f, C.c_int32)))
     return x, t, f
+
+
+def read_element_classes(path: str) -> np.ndarray:
+    """Element classes file: one integer per line (element-classes-filename)."""
+    n = C.c_int32()
+    _check(lib.xpbd_read_element_classes(path.encode(), 0, ptr(None, C.c_int32), C.byref(n)))
+    out = np.empty(n.value, np.int32)
+    _check(lib.xpbd_read_element_classes(path.encode(), n.value, ptr(out, C.c_int32), C.byref(n)))
+    return out
+
+
+def materials_from_classes(classes, materials):
+    """Per-element (E, nu, rho) from element classes and a list of materials
+    (dicts with E, nu, density), as XPBDMeshObject.cpp:287-300 maps them: a
+    class at or past the end of the list takes the last material."""
+    cls = np.asarray(classes, np.int64)
+    idx = np.where((cls < 0) | (cls >= len(materials)), len(materials) - 1, cls)
+    E = np.array([m["E"] for m in materials], np.float64)[idx]
+    nu = np.array([m["nu"] for m in materials], np.float64)[idx]
+    rho = np.array([m["density"] for m in materials], np.float64)[idx]
+    return E, nu, rho
 
 
 # ---------------------------------------------------------------------------
