@@ -90,7 +90,10 @@ typedef struct xpbd_params {
     double g;                 /* gravity magnitude, acts along -z [m/s^2] */
     int32_t iters;            /* solver iterations per step */
     int32_t solver;           /* XPBD_SOLVER_* */
-    int32_t deterministic;    /* Jacobi only: 1 = fixed-order accumulation */
+    int32_t deterministic;    /* Jacobi with the PATCH variant: 1 = every correction (elements,
+                               * contacts, attachments, local passes) summed in an order fixed
+                               * at set-up, bitwise reproducible; 0 = LDS / global float
+                               * atomics.  The SIMPLE variant always uses float atomics. */
     int32_t collision_every;  /* re-detect contacts every k steps; 0 = never */
     int32_t variant;          /* XPBD_VARIANT_* */
     /* First-order XPBD (FirstOrderXPBDMeshObject; XPBDMeshObject.cpp:277-348,

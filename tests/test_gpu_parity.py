@@ -127,6 +127,35 @@ def test_end_of_solve_residual_matches_oracle():
         assert abs(rg - ro) <= 1e-3 * ro + 1e-6
 
 
+@pytest.mark.parametrize("extras", ["contacts", "contacts, attachments, local passes"])
+def test_deterministic_patch_jacobi_with_contacts_is_bitwise_reproducible(extras):
+    """The fixed-order Jacobi mode sums contact, attachment and local-pass
+    corrections in an order fixed at set-up too (per-face and per-element
+    slots gathered per vertex), so two runs of a block pressed by a sphere give
+    the same bits."""
+    n = 8
+    mesh = W.grid_object(n, n, n, 0.01, E=1e3, nu=0.45, pin_axis=2, noise=0.01, seed=23, faces=True)
+    L = 0.01 * n
+    local = 2 if "local" in extras else 0
+    runs = []
+    for _ in range(2):
+        gpu = X.Solver(mesh, iters=10, solver="jacobi", variant="patch", deterministic=1,
+                       local_collision_iters=local)
+        gpu.set_sdf(X.SDF_SPHERE, (L / 2, L / 2, L + 0.02 - 0.001), (0, 0, 0, 1), (0.02, 0, 0))
+        gpu.set_sdf_velocity((0, 0, -0.05))
+        if "attachments" in extras:
+            top = np.where(np.abs(mesh.x0[:, 2] - L) < 1e-9)[0][:5]
+            gpu.set_attachments(top, mesh.x0[top] + np.array([0.0, 0.002, 0.001]))
+        most = 0
+        for _ in range(25):
+            gpu.step(1)
+            most = max(most, len(gpu.contacts()[0]))
+        assert most > 0
+        runs.append((gpu.positions(), gpu.lambdas()))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+
+
 def test_deterministic_patch_jacobi_is_bitwise_reproducible():
     mesh = W.cube(12, seed=21, E=1e3, nu=0.45)
     runs = []
@@ -548,14 +577,40 @@ def _single_tet():
                   rho=np.array([1000.0]), v0=v0)
 
 
+def _inverted_start(mesh, depth=0.3):
+    """Positions where one vertex of a free element is pushed through the plane
+    of its opposite face, `depth` times its height: that element starts with
+    J = det F <= 0, the Taylor branch of HydrostaticConstraint.hpp:136-151."""
+    x = mesh.x0.copy()
+    pinned = np.zeros(mesh.nv, bool) if mesh.pinned is None else mesh.pinned.astype(bool)
+    for t in mesh.tets:
+        if not pinned[t[0]]:
+            break
+    a, b, c, d = (x[i] for i in t)
+    n = np.cross(c - b, d - b)
+    n /= np.linalg.norm(n)
+    h = np.dot(a - b, n)
+    x[t[0]] = a - (1.0 + depth) * h * n
+    Q = np.linalg.inv(np.stack([mesh.x0[t[0]] - mesh.x0[t[3]], mesh.x0[t[1]] - mesh.x0[t[3]],
+                                mesh.x0[t[2]] - mesh.x0[t[3]]], axis=1))
+    F = np.stack([x[t[0]] - x[t[3]], x[t[1]] - x[t[3]], x[t[2]] - x[t[3]]], axis=1) @ Q
+    assert np.linalg.det(F) <= 0.0
+    return x
+
+
 @pytest.mark.parametrize("solver,variant,det", VARIANTS)
-@pytest.mark.parametrize("case", ["single tet", "ragged 3x5x7", "no iterations", "all pinned"])
+@pytest.mark.parametrize("case", ["single tet", "ragged 3x5x7", "no iterations", "all pinned", "inverted element"])
 def test_edge_cases_match_oracle(case, solver, variant, det):
     """The smallest mesh (one element, moving vertices), sizes that fill no patch,
     cluster chunk or colour evenly, a solve with zero iterations (free flight),
-    and a fully pinned object (nothing moves), for every kernel variant."""
+    a fully pinned object (nothing moves), and a start with an inverted element
+    (J <= 0, the hydrostatic constraint's Taylor branch), for every kernel variant."""
     iters = 10
-    if case == "single tet":
+    x_start = None
+    if case == "inverted element":
+        mesh = W.grid_object(3, 3, 3, 0.01, E=3e3, nu=0.45, pin_axis=2, noise=0.0, seed=8)
+        x_start = _inverted_start(mesh)
+    elif case == "single tet":
         mesh = _single_tet()
     elif case == "ragged 3x5x7":
         mesh = W.grid_object(3, 5, 7, 0.01, E=3e3, nu=0.45, pin_axis=1, noise=0.02, seed=5)
@@ -565,6 +620,9 @@ def test_edge_cases_match_oracle(case, solver, variant, det):
         mesh = W.grid_object(3, 3, 3, 0.01, E=3e3, nu=0.45, pin_axis=None, noise=0.02, seed=7)
         mesh.pinned = np.ones(mesh.nv, np.uint8)
     gpu, o = _pair(mesh, solver, iters, variant=variant, det=det)
+    if x_start is not None:
+        gpu.set_state(x_start, None)
+        o.set_state(gpu.positions(), gpu.velocities())
     errs = run_pair(gpu, o, 50)
     print(f"{case} {solver}/{variant}/det{det}: max rel err {errs.max():.3e}")
     assert np.all(np.isfinite(errs))

@@ -460,6 +460,25 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     rc |= dalloc(h, &h->d_cn, 3 * (size_t)nf);
     rc |= dalloc(h, &h->d_cb, 3 * (size_t)nf);
     rc |= dalloc(h, &h->d_clam, nf);
+    // deterministic patch Jacobi: contact corrections per face slot, gathered per
+    // vertex over (face slot * 3 + corner) in ascending face slot
+    std::vector<int32_t> vf_off, vf;
+    if (P->solver == XPBD_SOLVER_JACOBI && nf > 0) {
+        vf_off.assign((size_t)nv + 1, 0);
+        vf.resize(3 * (size_t)nf);
+        // solver face slot j holds caller face fperm[j] (4 ints per face: a, b, c, instance)
+        auto corner = [&](int j, int c) { return faces[4 * (size_t)h->fperm[j] + c]; };
+        for (int j = 0; j < nf; j++)
+            for (int c = 0; c < 3; c++) vf_off[corner(j, c) + 1]++;
+        for (int v = 0; v < nv; v++) vf_off[v + 1] += vf_off[v];
+        std::vector<int32_t> fill(vf_off.begin(), vf_off.end() - 1);
+        for (int j = 0; j < nf; j++)
+            for (int c = 0; c < 3; c++) vf[fill[corner(j, c)]++] = 3 * j + c;
+        rc |= dalloc(h, &h->d_cdx, 9 * (size_t)nf);
+        rc |= dalloc(h, &h->d_vf_off, vf_off.size());
+        rc |= dalloc(h, &h->d_vf, vf.size());
+        if (P->local_collision_iters > 0) rc |= dalloc(h, &h->d_edx, 12 * (size_t)nt);
+    }
     // local collision iterations: elements attached to each vertex, as solver positions
     std::vector<int32_t> vt_off, vt_pos;
     if (P->local_collision_iters > 0 && nf > 0) {
@@ -609,6 +628,10 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     }
     if (nf > 0) UP(h->d_faces, hf.data(), sizeof(int4) * nf);
     UP(h->d_fci_off, h->fci_off.data(), sizeof(int32_t) * h->fci_off.size());
+    if (h->d_vf_off) {
+        UP(h->d_vf_off, vf_off.data(), sizeof(int32_t) * vf_off.size());
+        UP(h->d_vf, vf.data(), sizeof(int32_t) * vf.size());
+    }
     if (h->d_vt_off) {
         UP(h->d_vt_off, vt_off.data(), sizeof(int32_t) * vt_off.size());
         UP(h->d_vt_pos, vt_pos.data(), sizeof(int32_t) * vt_pos.size());
@@ -668,6 +691,29 @@ extern "C" const char* xpbd_last_error(const xpbd_handle* h)
 // time step
 // ---------------------------------------------------------------------------
 namespace xpbd {
+
+// Patch Jacobi in its fixed-order mode: every accumulation (elements, contacts,
+// attachments, local passes) is summed in an order fixed at set-up.
+static bool jacobi_fixed_order(const xpbd_handle* h)
+{
+    return h->params.solver == XPBD_SOLVER_JACOBI && h->variant == XPBD_VARIANT_PATCH &&
+           h->params.deterministic != 0 && h->d_vf_off != nullptr;
+}
+
+int enqueue_contacts_jacobi(xpbd_handle* h, bool first)
+{
+    hipError_t e;
+    if (jacobi_fixed_order(h)) {
+        e = launch_contacts_jacobi_det(h->stream, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos,
+                                       h->d_lo, h->d_cdx, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
+        if (e == hipSuccess)
+            e = launch_contact_gather(h->stream, h->nv, h->d_vf_off, h->d_vf, h->d_flag, h->d_cdx, h->d_acc);
+    } else {
+        e = launch_contacts_jacobi(h->stream, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
+                                   h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
+    }
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "contacts_jacobi launch");
+}
 
 // Enqueue the solver iterations of one step (no predict / finish).  With
 // reset_lambda (the reference solve, XPBDSolver.hpp:66-77) the first iteration
@@ -767,9 +813,8 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed, bool reset_lambda)
             }
             if (con) {
                 // contact corrections read the pre-iteration positions, so they run first
-                e = launch_contacts_jacobi(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
-                                           h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
-                if (e != hipSuccess) return hip_err(h, e, "contacts_jacobi launch");
+                int rc = enqueue_contacts_jacobi(h, first);
+                if (rc) return rc;
             }
             if (timed) timed_begin(h);
             e = launch_jacobi_patch(s, ta, patch_arrays(h), h->d_pos, reinterpret_cast<float4*>(h->d_acc), h->nt,
@@ -782,9 +827,8 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed, bool reset_lambda)
             if (e != hipSuccess) return hip_err(h, e, "jacobi_project launch");
             if (timed) timed_end(h);
             if (con) {
-                e = launch_contacts_jacobi(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
-                                           h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
-                if (e != hipSuccess) return hip_err(h, e, "contacts_jacobi launch");
+                int rc = enqueue_contacts_jacobi(h, first);
+                if (rc) return rc;
             }
             if (h->natt > 0) {
                 e = launch_attach_jacobi(s, h->d_att_v, h->d_att_p, h->d_att_lam, h->d_rest, h->d_pos, h->d_lo,
@@ -830,12 +874,18 @@ int enqueue_local_collision(xpbd_handle* h)
                                    h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, false);
             if (e != hipSuccess) return hip_err(h, e, "local contacts_gs launch");
         } else {
-            e = launch_jacobi_local(s, ta, h->d_mult, h->d_pos, h->d_acc, h->nt);
-            if (e == hipSuccess)
-                e = launch_contacts_jacobi(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
-                                           h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, false);
-            if (e == hipSuccess) e = launch_jacobi_apply(s, h->d_pos, h->d_acc, h->nv);
+            if (jacobi_fixed_order(h) && h->d_edx) {
+                e = launch_jacobi_local_det(s, ta, h->d_mult, h->d_pos, h->d_edx, h->nt);
+                if (e == hipSuccess)
+                    e = launch_local_gather(s, ta, h->nv, h->d_vt_off, h->d_vt_pos, h->d_mult, h->d_edx, h->d_acc);
+            } else {
+                e = launch_jacobi_local(s, ta, h->d_mult, h->d_pos, h->d_acc, h->nt);
+            }
             if (e != hipSuccess) return hip_err(h, e, "local jacobi launch");
+            int rc = enqueue_contacts_jacobi(h, false);
+            if (rc) return rc;
+            e = launch_jacobi_apply(s, h->d_pos, h->d_acc, h->nv);
+            if (e != hipSuccess) return hip_err(h, e, "local jacobi apply launch");
         }
     }
     return XPBD_OK;

@@ -107,6 +107,12 @@ struct xpbd_handle {
     int32_t* d_vt_off = nullptr;
     int32_t* d_vt_pos = nullptr;
     int32_t* d_mult = nullptr;
+    // deterministic patch Jacobi: per-face contact corrections and the vertex ->
+    // (face slot * 3 + corner) CSR that gathers them; per-element local-pass sums
+    float* d_cdx = nullptr;
+    int32_t* d_vf_off = nullptr;
+    int32_t* d_vf = nullptr;
+    float* d_edx = nullptr;
     std::vector<void*> allocs;
 
     // stepping
@@ -141,6 +147,8 @@ int enqueue_detect(xpbd_handle* h);
 int enqueue_predict(xpbd_handle* h);
 int enqueue_finish(xpbd_handle* h);
 int enqueue_local_collision(xpbd_handle* h);
+// Jacobi contact pass into the accumulator (fixed order in the deterministic patch mode).
+int enqueue_contacts_jacobi(xpbd_handle* h, bool first);
 bool detect_this_step(const xpbd_handle* h);
 int run_steps(xpbd_handle* h, int32_t nsteps);
 void timed_begin(xpbd_handle* h);

@@ -35,6 +35,18 @@ hipError_t launch_predict(hipStream_t s, float4* pos, const float4* vel, int nv,
 hipError_t launch_predict_first_order(hipStream_t s, float4* pos, const float* dz1, int nv);
 hipError_t launch_finish(hipStream_t s, float4* pos, float4* lo, float4* vel, int nv, float dt,
                          unsigned long long* step_counter);
+// Deterministic Jacobi: contacts into per-face slots, then a per-vertex gather in
+// face-slot order; local passes into per-element slots, then a per-vertex gather
+// in solver-position order.
+hipError_t launch_contacts_jacobi_det(hipStream_t s, const int4* faces, int nf, const int* cpos, const int* clist,
+                                      const double* rest, const float4* pos, const float4* lo, float* cdx,
+                                      const double* cp, const double* cn, const double* cb, double* clam, bool first);
+hipError_t launch_contact_gather(hipStream_t s, int nv, const int* vf_off, const int* vf, const int* flag,
+                                 const float* cdx, float* acc);
+hipError_t launch_jacobi_local_det(hipStream_t s, const TetArrays& ta, const int* mult, const float4* pos, float* edx,
+                                   int nt);
+hipError_t launch_local_gather(hipStream_t s, const TetArrays& ta, int nv, const int* vt_off, const int* vt_pos,
+                               const int* mult, const float* edx, float* acc);
 // lambda = -C / alpha~ of every element at the current positions (learned step start)
 hipError_t launch_lambda_start(hipStream_t s, const TetArrays& ta, const float4* pos, const float4* lo, int nt);
 hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int begin, int end, bool first);

@@ -1043,9 +1043,9 @@ static int enqueue_multiplier_start(xpbd_handle* h)
             e = launch_contacts_gs(s, h->d_faces, h->d_fci_off, h->nfcolors, h->ninst, h->d_cpos, h->d_clist,
                                    h->d_rest, h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, true);
         } else {
-            e = launch_contacts_jacobi(s, h->d_faces, h->nf, h->d_cpos, h->d_clist, h->d_rest, h->d_pos, h->d_lo,
-                                       h->d_acc, h->d_cp, h->d_cn, h->d_cb, h->d_clam, true);
-            if (e == hipSuccess) e = launch_jacobi_apply(s, h->d_pos, h->d_acc, h->nv);
+            int rc = enqueue_contacts_jacobi(h, true);
+            if (rc) return rc;
+            e = launch_jacobi_apply(s, h->d_pos, h->d_acc, h->nv);
         }
     }
     if (e == hipSuccess) e = launch_lambda_start(s, tet_arrays(h), h->d_pos, h->d_lo, h->nt);

@@ -1453,7 +1453,69 @@ __global__ __launch_bounds__(kBlock) void k_contacts_jacobi(const int4* __restri
     }
     clam[j] = lam;
 }
+
+// Fixed-order variant for the deterministic Jacobi mode: contact j leaves its
+// nine corrections (zeros when inactive) in its face slot of cdx instead of
+// adding them to the accumulator; k_contact_gather then adds them per vertex
+// in ascending face-slot order.
+template <bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_contacts_jacobi_det(const int4* __restrict__ faces, int nf,
+                                                               const int* __restrict__ cpos,
+                                                               const int* __restrict__ clist,
+                                                               const double* __restrict__ rest,
+                                                               const float4* __restrict__ pos,
+                                                               const float4* __restrict__ lo, float* __restrict__ cdx,
+                                                               const double* __restrict__ cp,
+                                                               const double* __restrict__ cn,
+                                                               const double* __restrict__ cb, double* __restrict__ clam)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= cpos[nf]) return;
+    const int j = clist[k];
+    const int4 f = faces[j];
+    double x0[3], x1[3], x2[3];
+    world_pos(rest, pos, lo, f.x, true, x0);
+    world_pos(rest, pos, lo, f.y, true, x1);
+    world_pos(rest, pos, lo, f.z, true, x2);
+    const double* xs[3] = { x0, x1, x2 };
+    const float w[3] = { pos[2 * (size_t)f.x].w, pos[2 * (size_t)f.y].w, pos[2 * (size_t)f.z].w };
+    double lam = FIRST ? 0.0 : clam[j];
+    double dx[9];
+    const bool act = contact_dx(xs, w, cp + 3 * (size_t)j, cn + 3 * (size_t)j, cb + 3 * (size_t)j, &lam, dx);
+#pragma unroll
+    for (int i = 0; i < 9; i++) cdx[9 * (size_t)j + i] = act ? (float)dx[i] : 0.f;
+    clam[j] = lam;
+}
 #pragma clang fp contract(fast)
+
+// Per vertex, the contact corrections of its faces (vf_off / vf: face slot * 3
+// + corner, ascending face slot) that hold a contact this step, summed in that
+// order and added to the accumulator.
+__global__ __launch_bounds__(kBlock) void k_contact_gather(int nv, const int* __restrict__ vf_off,
+                                                          const int* __restrict__ vf, const int* __restrict__ flag,
+                                                          const float* __restrict__ cdx, float* __restrict__ acc)
+{
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nv) return;
+    const int b = vf_off[v], e = vf_off[v + 1];
+    if (b == e) return;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    bool any = false;
+    for (int k = b; k < e; k++) {
+        const int jc = vf[k], j = jc / 3, c = jc - 3 * j;
+        if (!flag[j]) continue;
+        const float* d = cdx + 9 * (size_t)j + 3 * c;
+        s0 += d[0];
+        s1 += d[1];
+        s2 += d[2];
+        any = true;
+    }
+    if (any) {
+        acc[4 * (size_t)v] += s0;
+        acc[4 * (size_t)v + 1] += s1;
+        acc[4 * (size_t)v + 2] += s2;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Fused patch Jacobi (fixed-order accumulation), one launch per iteration.
@@ -1727,7 +1789,8 @@ __global__ __launch_bounds__(kBlock) void k_attach_jacobi(const int* __restrict_
     double lam = FIRST ? 0.0 : alam[a];
     attach_dx(x, ap + 3 * (size_t)a, pos[2 * (size_t)v].w, &lam, dx);
     alam[a] = lam;
-    for (int k = 0; k < 3; k++) unsafeAtomicAdd(acc + 4 * (size_t)v + k, (float)dx[k]);
+    // attached vertices are distinct (xpbd_set_attachments checks it): one writer per slot
+    for (int k = 0; k < 3; k++) acc[4 * (size_t)v + k] += (float)dx[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1805,6 +1868,65 @@ __global__ __launch_bounds__(kBlock) void k_gs_local_cluster(TetArrays ta, const
 
 // Jacobi: the m projections of element e all read the pass's starting positions
 // (the multiplier advances between them); their sum goes into acc.
+// Fixed-order variant: each counted element leaves its summed corrections in
+// its solver position's slot of edx; k_local_gather adds them per vertex.
+__global__ __launch_bounds__(kBlock) void k_jacobi_local_det(TetArrays ta, const int* __restrict__ mult,
+                                                            const float4* __restrict__ pos, float* __restrict__ edx,
+                                                            int nt)
+{
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nt) return;
+    const int m = mult[e];
+    if (m == 0) return;
+    const int4 t = ta.tet[e];
+    Gather g;
+    gather4(pos, t, g);
+    float q[9];
+    float2 at;
+    float gm;
+    load_tet(ta, e, q, at, gm);
+    const float2 lv = ta.lam[e];
+    float l[2] = { lv.x, lv.y };
+    float sum[12] = {};
+    for (int r = 0; r < m; r++) {
+        float dx[12];
+        project_tet_split<float>(g.b, g.d, g.w, q, at.x, at.y, gm, l, dx);
+#pragma unroll
+        for (int k = 0; k < 12; k++) sum[k] += dx[k];
+    }
+    ta.lam[e] = make_float2(l[0], l[1]);
+#pragma unroll
+    for (int k = 0; k < 12; k++) edx[12 * (size_t)e + k] = sum[k];
+}
+
+// Per vertex, the local-pass corrections of its counted elements in ascending
+// solver position (vt_off / vt_pos), added to the accumulator.
+__global__ __launch_bounds__(kBlock) void k_local_gather(TetArrays ta, int nv, const int* __restrict__ vt_off,
+                                                        const int* __restrict__ vt_pos, const int* __restrict__ mult,
+                                                        const float* __restrict__ edx, float* __restrict__ acc)
+{
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nv) return;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    bool any = false;
+    for (int k = vt_off[v]; k < vt_off[v + 1]; k++) {
+        const int p = vt_pos[k];
+        if (mult[p] == 0) continue;
+        const int4 t = ta.tet[p];
+        const int c = t.x == v ? 0 : t.y == v ? 1 : t.z == v ? 2 : 3;
+        const float* d = edx + 12 * (size_t)p + 3 * c;
+        s0 += d[0];
+        s1 += d[1];
+        s2 += d[2];
+        any = true;
+    }
+    if (any) {
+        acc[4 * (size_t)v] += s0;
+        acc[4 * (size_t)v + 1] += s1;
+        acc[4 * (size_t)v + 2] += s2;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_jacobi_local(TetArrays ta, const int* __restrict__ mult,
                                                         const float4* __restrict__ pos, float* __restrict__ acc, int nt)
 {
@@ -1912,6 +2034,46 @@ hipError_t launch_finish(hipStream_t s, float4* pos, float4* lo, float4* vel, in
 {
     hipLaunchKernelGGL(k_finish, dim3(nblk(nv > 0 ? nv : 1, kBlock)), dim3(kBlock), 0, s, pos, lo, vel, nv, dt,
                        step_counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_contacts_jacobi_det(hipStream_t s, const int4* faces, int nf, const int* cpos, const int* clist,
+                                      const double* rest, const float4* pos, const float4* lo, float* cdx,
+                                      const double* cp, const double* cn, const double* cb, double* clam, bool first)
+{
+    if (nf <= 0) return hipSuccess;
+    const dim3 g(nblk(nf, kBlock));
+    if (first)
+        hipLaunchKernelGGL(k_contacts_jacobi_det<true>, g, dim3(kBlock), 0, s, faces, nf, cpos, clist, rest, pos, lo,
+                           cdx, cp, cn, cb, clam);
+    else
+        hipLaunchKernelGGL(k_contacts_jacobi_det<false>, g, dim3(kBlock), 0, s, faces, nf, cpos, clist, rest, pos, lo,
+                           cdx, cp, cn, cb, clam);
+    return hipGetLastError();
+}
+
+hipError_t launch_contact_gather(hipStream_t s, int nv, const int* vf_off, const int* vf, const int* flag,
+                                 const float* cdx, float* acc)
+{
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_contact_gather, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, nv, vf_off, vf, flag, cdx, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_local_det(hipStream_t s, const TetArrays& ta, const int* mult, const float4* pos, float* edx,
+                                   int nt)
+{
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jacobi_local_det, dim3(nblk(nt, kBlock)), dim3(kBlock), 0, s, ta, mult, pos, edx, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_local_gather(hipStream_t s, const TetArrays& ta, int nv, const int* vt_off, const int* vt_pos,
+                               const int* mult, const float* edx, float* acc)
+{
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_local_gather, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, ta, nv, vt_off, vt_pos, mult, edx,
+                       acc);
     return hipGetLastError();
 }
 
