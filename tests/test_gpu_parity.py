@@ -669,29 +669,79 @@ def test_clusters_of_eight_elements_match_oracle(variant):
     assert errs.max() <= TOL
 
 
-@pytest.mark.parametrize("mesh_kind", ["hexahedral cells", "8-element clusters"])
-def test_lds_lean_quad_form_gives_the_bits_of_the_full_form(tmp_path, mesh_kind):
-    """Both forms of the quad kernel, each forced through the tuning cache: the
-    lean form sums vertex 3's weights in the quad, in the order the full form
-    stages them, and moves vertex bases between lanes unchanged, so the two give
-    the same bits, and both match the oracle."""
-    mesh = W.cube(8, seed=21) if mesh_kind == "hexahedral cells" else _eight_element_cells(3, 4, 5, seed=23)
-    probe = X.Solver(mesh, dt=1e-3, iters=10, solver="gs", variant="cluster")
+def _forced(mesh_or_meshes, tmp_path, variant, block, xcd, **kw):
+    """A GS handle whose kernel shape is forced through the tuning cache."""
+    probe = X.Solver(mesh_or_meshes, dt=1e-3, solver="gs", variant="cluster", **kw)
     probe.autotune(str(tmp_path / "probe.txt"))
     key = open(tmp_path / "probe.txt").read().split()[0]
+    probe.close()
+    cache = tmp_path / f"tune_{variant}_{block}_{xcd}.txt"
+    cache.write_text(f"{key} {X.VARIANTS[variant]} {block} {xcd} 0\n")
+    gpu = X.Solver(mesh_or_meshes, dt=1e-3, solver="gs", variant="cluster", **kw)
+    gpu.autotune(str(cache))
+    assert gpu.launch_config() == (variant, block if variant == "quad" else 0, bool(xcd))
+    return gpu
+
+
+@pytest.mark.parametrize("mesh_kind", ["hexahedral cells", "8-element clusters"])
+@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("xcd", [0, 1])
+def test_lds_lean_quad_form_gives_the_bits_of_the_full_form(tmp_path, mesh_kind, block, xcd):
+    """Both forms of the quad kernel at every workgroup size, with and without
+    the XCD-contiguous mapping, each forced through the tuning cache: the lean
+    form sums vertex 3's weights in the quad, in the order the full form stages
+    them, and moves vertex bases between lanes unchanged, so the two give the
+    same bits, and both match the oracle."""
+    mesh = W.cube(8, seed=21) if mesh_kind == "hexahedral cells" else _eight_element_cells(3, 4, 5, seed=23)
     out = {}
-    for block in (64, -64):
-        cache = tmp_path / f"tune{block}.txt"
-        cache.write_text(f"{key} {X.VARIANTS['quad']} {block} 1 0\n")
-        gpu = X.Solver(mesh, dt=1e-3, iters=10, solver="gs", variant="cluster")
-        assert gpu.autotune(str(cache)) == ("quad", block)
+    for b in (block, -block):
+        gpu = _forced(mesh, tmp_path, "quad", b, xcd, iters=10)
         o = orc.Sim(mesh, dt=1e-3, g=9.81, iters=10, mode="gs")
         o.set_order(gpu.tet_order())
         errs = run_pair(gpu, o, 30)
-        print(f"{mesh_kind}, quad block {block}: max rel err {errs.max():.3e}")
+        print(f"{mesh_kind}, quad block {b}, xcd {xcd}: max rel err {errs.max():.3e}")
         assert errs.max() <= TOL
-        out[block] = gpu.positions()
-    assert np.array_equal(out[64], out[-64])
+        out[b] = gpu.positions()
+    assert np.array_equal(out[block], out[-block])
+
+
+def test_uneven_batch_on_the_benchmarked_kernel_shape(tmp_path):
+    """The shape the autotuner picks for the 256-cube batch (LDS-lean quad form,
+    256 threads, XCD mapping), forced on an uneven batch with box colliders and
+    contacts every step: every instance against its own oracle step."""
+    meshes, sdfs = _uneven_batch(6, (1e3, 5e4), seed=9, kind=X.SDF_BOX)
+    gpu = _forced(meshes, tmp_path, "quad", -256, 1, iters=10, collision_every=1)
+    for i, (kind, pos, quat, prm) in enumerate(sdfs):
+        gpu.set_sdf(kind, pos, quat, prm, instance=i)
+    t_off = np.concatenate([[0], np.cumsum([m.nt for m in meshes])])
+    f_off = np.concatenate([[0], np.cumsum([len(m.faces) for m in meshes])])
+    order, rank = gpu.tet_order(), gpu.face_rank()
+    oracles = []
+    for i, (m, (kind, pos, quat, prm)) in enumerate(zip(meshes, sdfs)):
+        o = orc.Sim(m, dt=1e-3, iters=10, mode="gs")
+        o.set_order(order[(order >= t_off[i]) & (order < t_off[i + 1])] - t_off[i])
+        o.set_contact_order(rank[f_off[i]:f_off[i + 1]])
+        o.set_sdf(kind, pos, quat, prm)
+        oracles.append(o)
+    v_off = np.concatenate([[0], np.cumsum([m.nv for m in meshes])])
+    worst, most = 0.0, 0
+    for _ in range(40):
+        # each step from the device's state, with the device's contacts
+        x0, v0 = gpu.positions(), gpu.velocities()
+        gpu.step(1)
+        face, cp, cn, cb = gpu.contacts()
+        most = max(most, len(face))
+        x = gpu.positions()
+        for i, o in enumerate(oracles):
+            sel = (face >= f_off[i]) & (face < f_off[i + 1])
+            o.set_state(x0[v_off[i]:v_off[i + 1]], v0[v_off[i]:v_off[i + 1]])
+            o.set_contacts(face[sel] - f_off[i], cp[sel], cn[sel], cb[sel])
+            o.step(1, detect=False)
+            worst = max(worst, rel_err(x[v_off[i]:v_off[i + 1]], o.x()))
+    print(f"batch of {len(meshes)} on ('quad', -256, xcd), per step from a shared state: max rel err {worst:.3e}, "
+          f"contacts up to {most}")
+    assert most > 0
+    assert worst <= TOL
 
 
 def test_batch_step_is_the_step_of_the_batch_handle():
