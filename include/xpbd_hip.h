@@ -336,6 +336,17 @@ int xpbd_mlp_save(xpbd_mlp* m, const char* path);
 int xpbd_mlp_load(xpbd_mlp* m, const char* path);
 /* Forward on n host rows (n x 24) -> out (n x 3). */
 int xpbd_mlp_forward(xpbd_mlp* m, int32_t n, const float* x, float* out);
+/* Forward without device allocation: the same inference as xpbd_mlp_forward
+ * (the same kernel, the same results), with the rows staged through a fixed
+ * file-scope device array of XPBD_MLP_NOALLOC_MAX_ROWS rows (8 MB, held on
+ * every GPU the library runs on until the process ends) instead of a
+ * workspace grown with hipMalloc.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (a serving loop, code under HIP
+ * graph capture).  n > XPBD_MLP_NOALLOC_MAX_ROWS is refused with
+ * XPBD_E_INVALID_ARG.  All calls on one GPU share the array: calls that
+ * overlap in time on one GPU (from several threads or networks) race. */
+#define XPBD_MLP_NOALLOC_MAX_ROWS 65536
+int xpbd_mlp_forward_noalloc(xpbd_mlp* m, int32_t n, const float* x, float* out);
 /* Loss and parameter gradient on n rows (host arrays): the mean over n*3 of
  * (MLP(x) - y / scale_out)^2, the squared error in units of scale_out; grad
  * receives 20,099 floats.  Used by the gradient checks; training minimises

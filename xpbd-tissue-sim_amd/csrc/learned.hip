@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -479,6 +480,7 @@ __global__ void k_zero_delta(float4* __restrict__ pos, int nv)
 // network object
 // ---------------------------------------------------------------------------
 struct xpbd_mlp {
+    int device = 0;           // device current at creation (weights and stream live there)
     hipStream_t stream = nullptr;
     float* d_p = nullptr;     // FP32 master parameters
     float* d_g = nullptr;     // gradient
@@ -624,7 +626,8 @@ extern "C" int xpbd_mlp_create(xpbd_mlp** out)
     *out = nullptr;
     xpbd_mlp* m = new (std::nothrow) xpbd_mlp();
     if (!m) return XPBD_E_NOMEM;
-    hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&m->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     auto A = [&](void** p, size_t bytes) {
         if (e == hipSuccess) e = hipMalloc(p, bytes);
     };
@@ -804,6 +807,79 @@ extern "C" int xpbd_mlp_forward(xpbd_mlp* m, int32_t n, const float* x, float* o
         out[3 * (size_t)i + 2] = o[i].z;
     }
     return XPBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Forward without allocation: the rows are staged through a fixed file-scope
+// device array instead of the network's workspace (which xpbd_mlp_forward grows
+// with hipMalloc).  Each GPU holds its own copy of the array; its address on a
+// device is looked up once (hipGetSymbolAddress with that device current) and
+// kept, since the array's name in host code is not a device address.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kNoAllocRows = XPBD_MLP_NOALLOC_MAX_ROWS;
+// per row: 24 input floats, a float4 network output, 3 packed output floats
+constexpr size_t kScratchFloats = (size_t)kNoAllocRows * (kIn + 4 + kOut);
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// The calling thread's device is made `device` for the lookup and restored.
+hipError_t scratch_on(int device, float** out)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+        void* p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+// out3[3 i + k] = out4[i].k for the first three components
+__global__ void k_pack3(const float4* __restrict__ out4, int n, float* __restrict__ out3)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = out4[i];
+    out3[3 * (size_t)i] = v.x;
+    out3[3 * (size_t)i + 1] = v.y;
+    out3[3 * (size_t)i + 2] = v.z;
+}
+}  // namespace
+
+extern "C" int xpbd_mlp_forward_noalloc(xpbd_mlp* m, int32_t n, const float* x, float* out)
+{
+    if (!m || n < 0 || (n > 0 && (!x || !out))) return XPBD_E_INVALID_ARG;
+    if (n > kNoAllocRows)
+        return merr(m, XPBD_E_INVALID_ARG,
+                    "xpbd_mlp_forward_noalloc: " + std::to_string(n) + " rows exceed the fixed scratch of " +
+                        std::to_string(kNoAllocRows) + " rows (call it in chunks, or use xpbd_mlp_forward)");
+    if (n == 0) return XPBD_OK;
+    float* base = nullptr;
+    hipError_t e = scratch_on(m->device, &base);
+    if (e != hipSuccess) return merr(m, XPBD_E_HIP, std::string("scratch lookup: ") + hipGetErrorString(e));
+    float* dX = base;                                                                   // n x 24
+    float4* d4 = reinterpret_cast<float4*>(base + (size_t)kNoAllocRows * kIn);         // n float4
+    float* d3 = base + (size_t)kNoAllocRows * (kIn + 4);                                // n x 3
+    e = hipMemcpyAsync(dX, x, sizeof(float) * kIn * (size_t)n, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = mlp_infer(m, m->stream, dX, n, d4);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_pack3, dim3(nb(n)), dim3(256), 0, m->stream, d4, n, d3);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d3, sizeof(float) * kOut * (size_t)n, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    return e == hipSuccess ? XPBD_OK : merr(m, XPBD_E_HIP, hipGetErrorString(e));
 }
 
 extern "C" int xpbd_mlp_loss_grad(xpbd_mlp* m, int32_t n, const float* x, const float* y, double* loss, float* grad)

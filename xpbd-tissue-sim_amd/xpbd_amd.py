@@ -86,6 +86,7 @@ _sig("xpbd_get_tet_order", [_vp, _ip])
 _sig("xpbd_get_face_rank", [_vp, _ip])
 _sig("xpbd_get_color_offsets", [_vp, _ip])
 _sig("xpbd_set_learned_start", [_vp, C.c_int32])
+_sig("xpbd_mlp_forward_noalloc", [_vp, C.c_int32, _fp, _fp])
 _sig("xpbd_set_sdf", [_vp, C.c_int32, _dp, _dp])
 _sig("xpbd_set_sdf_velocity", [_vp, _dp])
 _sig("xpbd_detect_contacts", [_vp])
