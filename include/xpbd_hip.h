@@ -217,6 +217,33 @@ int xpbd_set_sdf_velocity(xpbd_handle* h, const double* vel3);
  * clearCollisionConstraints + addStaticCollisionConstraint
  * (XPBDMeshObject.cpp:180-251). */
 int xpbd_detect_contacts(xpbd_handle* h);
+/* Replace the contact set with the caller's n contacts: face index (caller's
+ * face numbering), surface point p (3), normal nrm (3) and barycentrics bary
+ * (3) each, multipliers zero; n = 0 clears it.  Replaces
+ * XPBDMeshObject_::clearCollisionConstraints followed by one
+ * addStaticCollisionConstraint per contact (XPBDMeshObject.cpp:180-251), so a
+ * host-side collision scene (the reference's CollisionScene) can drive the
+ * device solver.  A face carries at most one contact.  The contacts are
+ * projected in every iteration until the next detection (xpbd_detect_contacts,
+ * or a step that detects: collision_every > 0 with a collider set) or the next
+ * call of this function. */
+int xpbd_set_contacts(xpbd_handle* h, int32_t n, const int32_t* faces, const double* p, const double* nrm,
+                      const double* bary);
+
+/* XPBDSolver::solve(projector_references, num_iter, initialize)
+ * (include/solver/xpbd_solver/XPBDSolver.hpp:106-125) between steps: the n
+ * listed elements (caller's indices; an element listed k times is projected k
+ * times per pass, back to back) are projected num_iter times in the solver's
+ * visiting order, each pass followed by every contact when with_contacts;
+ * initialize = 1 starts their multipliers (and the contacts') at zero first
+ * (the projectors' initialize()).  Gauss-Seidel applies each projection at
+ * once; Jacobi sums a pass from the same positions (float atomics).  The
+ * positions change in place; velocities are not updated (the next step's
+ * velocity update sees the moved positions), and pinned vertices are not
+ * reset. */
+int xpbd_solve_elements(xpbd_handle* h, int32_t n, const int32_t* elements, int32_t num_iter, int32_t initialize,
+                        int32_t with_contacts);
+
 /* Copy out the current contact set (at most cap entries), in solver order:
  * face index, surface point p (3), normal n (3), barycentrics (3). */
 int xpbd_get_contacts(xpbd_handle* h, int32_t cap, int32_t* count, int32_t* face, double* p,

@@ -81,6 +81,8 @@ _sig("orc_set_order_levels", [C.c_void_p, _ip, C.c_int, _ip], C.c_int)
 _sig("orc_set_contacts", [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp])
 _sig("orc_momentum_residual", [C.c_void_p, _dp, _dp], C.c_double)
 _sig("orc_step_warm", [C.c_void_p, C.c_int, _dp, C.c_int])
+_sig("orc_solve_elements", [C.c_void_p, C.c_int, _ip, C.c_int, C.c_int, C.c_int])
+_sig("orc_set_lambdas", [C.c_void_p, _dp])
 
 
 def project_contact(x9, w3, cp, cn, bary, lam=0.0):
@@ -315,6 +317,17 @@ class Sim:
         out = np.empty((self.nv, 3)) if vec else None
         n = lib.orc_momentum_residual(self.s, p(_d(xt)), p(out))
         return (n, out) if vec else n
+
+    def set_lambdas(self, lam):
+        """Overwrite the element multipliers (T x 2: lambda_D, lambda_H)."""
+        lib.orc_set_lambdas(self.s, p(_d(lam).reshape(-1)))
+
+    def solve_elements(self, elements, num_iter=1, initialize=False, with_contacts=False):
+        """XPBDSolver::solve(projector_references, num_iter, initialize) on a list
+        of elements (repeats allowed), optionally followed by every contact."""
+        e = np.ascontiguousarray(elements, np.int32)
+        lib.orc_solve_elements(self.s, len(e), p(e, C.c_int32), int(num_iter), int(bool(initialize)),
+                               int(bool(with_contacts)))
 
     def step_corrected(self, corr, iters, detect=False):
         """One step with `corr` (V x 3) added to the predicted positions before `iters` iterations."""

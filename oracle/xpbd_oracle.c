@@ -1086,6 +1086,8 @@ static int cmp_elem(const void* a, const void* b)
     return (ra > rb) - (ra < rb);
 }
 
+static void project_list(orc_sim* s, int* list, size_t n, int iters, int with_contacts);
+
 static void local_collision_iterations(orc_sim* s)
 {
     s->last_local_n = 0;
@@ -1118,6 +1120,19 @@ static void local_collision_iterations(orc_sim* s)
             }
     }
     s->last_local_n = (int)n;
+    project_list(s, list, n, s->local_iters, 1);
+    free(list);
+}
+
+/* Project the element list (repeats allowed) iters times, each pass followed
+ * by every contact when with_contacts, with the multipliers kept: Gauss-Seidel
+ * applies each projection at once (XPBDGaussSeidelSolver.hpp:44-53), Jacobi
+ * buffers all of a pass and sums it afterwards (XPBDJacobiSolver.hpp:119-145).
+ * With a device order set (orc_set_order) the list is sorted into it first,
+ * repeats of an element back to back. */
+static void project_list(orc_sim* s, int* list, size_t n, int iters, int with_contacts)
+{
+    const int nt = s->nt;
     if (s->order_set && n > 1) {
         int* rank = (int*)malloc(sizeof(int) * (size_t)nt);
         for (int i = 0; i < nt; i++) rank[s->order[i]] = i;
@@ -1128,7 +1143,7 @@ static void local_collision_iterations(orc_sim* s)
     double* buf = s->mode == ORC_JACOBI ? (double*)malloc(sizeof(double) * 12 * (n > 0 ? n : 1)) : NULL;
     double* cbuf = s->mode == ORC_JACOBI ? (double*)malloc(sizeof(double) * 9 * (size_t)s->nc) : NULL;
     int* cact = s->mode == ORC_JACOBI ? (int*)malloc(sizeof(int) * (size_t)s->nc) : NULL;
-    for (int it = 0; it < s->local_iters; it++) {
+    for (int it = 0; it < iters; it++) {
         for (size_t u = 0; u < n; u++) {
             const int e = list[u];
             const int* t = s->tets + 4 * e;
@@ -1141,7 +1156,7 @@ static void local_collision_iterations(orc_sim* s)
                 for (int i = 0; i < 4; i++)
                     for (int k = 0; k < 3; k++) s->x[3 * t[i] + k] += dx[3 * i + k];
         }
-        for (int ci = 0; ci < s->nc; ci++) {
+        for (int ci = 0; with_contacts && ci < s->nc; ci++) {
             const int c = buf ? ci : s->corder[ci];
             const int* f = s->faces + 3 * s->c_face[c];
             const double* p[3] = { s->x + 3 * f[0], s->x + 3 * f[1], s->x + 3 * f[2] };
@@ -1160,7 +1175,7 @@ static void local_collision_iterations(orc_sim* s)
                 for (int i = 0; i < 4; i++)
                     for (int k = 0; k < 3; k++) s->x[3 * t[i] + k] += buf[12 * u + 3 * i + k];
             }
-            for (int c = 0; c < s->nc; c++) {
+            for (int c = 0; with_contacts && c < s->nc; c++) {
                 if (!cact[c]) continue;
                 const int* f = s->faces + 3 * s->c_face[c];
                 for (int i = 0; i < 3; i++)
@@ -1168,10 +1183,28 @@ static void local_collision_iterations(orc_sim* s)
             }
         }
     }
-    free(list);
     free(buf);
     free(cbuf);
     free(cact);
+}
+
+/* XPBDSolver::solve(projector_references, num_iter, initialize)
+ * (include/solver/xpbd_solver/XPBDSolver.hpp:106-125) on a list of n elements
+ * (repeats allowed), followed by every contact when with_contacts: with
+ * initialize the listed projectors' multipliers (and the contacts') start at
+ * zero, then num_iter passes (project_list).  Positions change in place; no
+ * velocity update and no pin reset happen here. */
+ORC_API void orc_solve_elements(orc_sim* s, int n, const int* elements, int num_iter, int initialize,
+                                int with_contacts)
+{
+    int* list = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    memcpy(list, elements, sizeof(int) * (size_t)(n > 0 ? n : 0));
+    if (initialize) {
+        for (int u = 0; u < n; u++) s->lam[2 * list[u]] = s->lam[2 * list[u] + 1] = 0;
+        for (int c = 0; with_contacts && c < s->nc; c++) s->c_lam[c] = 0;
+    }
+    project_list(s, list, (size_t)n, num_iter, with_contacts);
+    free(list);
 }
 
 ORC_API void orc_set_local_collision_iters(orc_sim* s, int n) { s->local_iters = n > 0 ? n : 0; }
@@ -1355,6 +1388,7 @@ ORC_API void orc_get_x(const orc_sim* s, double* out) { memcpy(out, s->x, sizeof
 ORC_API void orc_get_xprev(const orc_sim* s, double* out) { memcpy(out, s->xp, sizeof(double) * 3 * s->nv); }
 ORC_API void orc_get_v(const orc_sim* s, double* out) { memcpy(out, s->v, sizeof(double) * 3 * s->nv); }
 ORC_API void orc_get_lambda(const orc_sim* s, double* out) { memcpy(out, s->lam, sizeof(double) * 2 * s->nt); }
+ORC_API void orc_set_lambdas(orc_sim* s, const double* lam) { memcpy(s->lam, lam, sizeof(double) * 2 * s->nt); }
 ORC_API void orc_get_masses(const orc_sim* s, double* out) { memcpy(out, s->m, sizeof(double) * s->nv); }
 ORC_API void orc_get_rest(const orc_sim* s, double* Q, double* vol)
 {

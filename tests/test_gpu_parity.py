@@ -758,3 +758,93 @@ def test_batch_step_is_the_step_of_the_batch_handle():
         runs.append(np.concatenate([s.instance_positions(i) for i in range(len(meshes))]))
         s.close()
     assert np.array_equal(runs[0], runs[1])
+
+
+def _pressed_block(seed=31, depth=0.002):
+    n = 6
+    rng = np.random.default_rng(seed)
+    E, nu = W.heterogeneous(6 * n ** 3, rng, E_range=(1e3, 1e4))
+    mesh = W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.005, faces=True, rng=rng)
+    L = 0.01 * n
+    sph = (X.SDF_SPHERE, (0.47 * L, 0.53 * L, L + 0.02 - depth), (0, 0, 0, 1), (0.02, 0, 0))
+    return mesh, sph
+
+
+@pytest.mark.parametrize("solver,variant", [("gs", "quad"), ("gs", "cluster"), ("gs", "simple"), ("jacobi", "patch")])
+def test_contacts_given_by_the_caller_match_oracle(solver, variant):
+    """xpbd_set_contacts (clearCollisionConstraints + addStaticCollisionConstraint):
+    the oracle detects on its own state, both take that contact set, both step
+    without detecting, the set persisting across steps; then an empty set
+    clears it on both.  Every step starts the oracle from the device's state:
+    the reference's plain-sum Jacobi under sustained contact amplifies rounding
+    by ~10x per 10 steps, so only the step itself is comparable there."""
+    mesh, sph = _pressed_block()
+    gpu = X.Solver(mesh, dt=1e-3, iters=10, solver=solver, variant=variant, collision_every=0)
+    o = orc.Sim(mesh, dt=1e-3, iters=10, mode=solver)
+    if solver == "gs":
+        o.set_order(gpu.tet_order())
+        o.set_contact_order(gpu.face_rank())
+    o.set_sdf(*sph)
+    errs, counts = [], []
+    for k in range(30):
+        if k % 10 == 0:
+            # the caller's collision pass every 10 steps (on the oracle's state, shared with the device)
+            o.set_state(gpu.positions(), gpu.velocities())
+            o.detect()
+            face, cp, cn, cb = o.contacts()
+            gpu.set_contacts(face, cp, cn, cb)
+            got = gpu.contacts()
+            assert sorted(got[0]) == sorted(face)
+            counts.append(len(face))
+        if k == 25:
+            gpu.set_contacts([], [], [], [])
+            o.set_contacts(np.zeros(0, np.int32), np.zeros((0, 3)), np.zeros((0, 3)), np.zeros((0, 3)))
+            assert len(gpu.contacts()[0]) == 0
+        o.set_state(gpu.positions(), gpu.velocities())
+        gpu.step(1)
+        o.step(1, detect=False)
+        errs.append(rel_err(gpu.positions(), o.x()))
+    print(f"given contacts {solver}/{variant}: counts {counts}, max rel err per step {max(errs):.3e}")
+    assert counts[0] > 0
+    assert max(errs) <= TOL
+
+
+@pytest.mark.parametrize("solver,variant", [("gs", "quad"), ("gs", "cluster"), ("gs", "simple"), ("jacobi", "patch")])
+@pytest.mark.parametrize("initialize", [False, True])
+def test_subset_solve_matches_oracle(solver, variant, initialize):
+    """xpbd_solve_elements (XPBDSolver::solve(projector_references, num_iter,
+    initialize)) between steps on a list with repeats, followed by the
+    contacts: positions and multipliers as the oracle's orc_solve_elements."""
+    mesh, sph = _pressed_block(seed=32, depth=0.004)
+    gpu = X.Solver(mesh, dt=1e-3, iters=5, solver=solver, variant=variant, collision_every=1)
+    gpu.set_sdf(*sph)
+    o = orc.Sim(mesh, dt=1e-3, iters=5, mode=solver)
+    if solver == "gs":
+        o.set_order(gpu.tet_order())
+        o.set_contact_order(gpu.face_rank())
+    o.set_sdf(*sph)
+    gpu.step(1)
+    o.step(1, detect=True)
+    o.set_state(gpu.positions(), gpu.velocities())
+    gpu.detect()
+    face, cp, cn, cb = gpu.contacts()
+    o.set_contacts(face, cp, cn, cb)
+    rng = np.random.default_rng(33)
+    listed = np.concatenate([rng.choice(mesh.nt, 60, replace=False), rng.choice(mesh.nt, 20, replace=False)])
+    # the oracle's multipliers start from the device's (the subset solve keeps them unless initialize)
+    lam_dev = gpu.lambdas()
+    o_lam = o.lambdas()
+    gpu.solve_elements(listed, num_iter=3, initialize=initialize, with_contacts=True)
+    o_x0 = o.x()
+    # the oracle starts from the device's multipliers too
+    o.set_lambdas(lam_dev)
+    o.solve_elements(listed, num_iter=3, initialize=initialize, with_contacts=True)
+    err = rel_err(gpu.positions(), o.x())
+    moved = float(np.max(np.abs(o.x() - o_x0)))
+    lg, lo = gpu.lambdas()[listed], o.lambdas()[listed]
+    lerr = float(np.max(np.abs(lg - lo)) / max(np.max(np.abs(lo)), 1e-30))
+    print(f"subset solve {solver}/{variant} initialize={initialize}: moved {moved:.3e} m, rel err {err:.3e}, "
+          f"listed multipliers rel err {lerr:.3e}, {len(face)} contacts")
+    assert moved > 0 and np.any(o_lam != 0) and len(face) > 0
+    assert err <= TOL
+    assert lerr <= 1e-4

@@ -826,3 +826,42 @@ def test_sliver_shape_factor_of_the_tissue_meshes(name, worst):
         assert k.max() > worst
     else:
         assert k.max() < worst
+
+
+def test_subset_solve_is_the_reference_solve_on_a_list():
+    """orc_solve_elements restates XPBDSolver::solve(projector_references,
+    num_iter, initialize): with every element listed once, initialize and
+    num_iter passes after the inertial prediction, it is the step's global
+    solve (orc_step on a mesh without pinned vertices gives the same positions
+    and multipliers, bit for bit); splitting the passes over two calls without
+    initialize changes nothing; listing an element twice projects it twice; an
+    empty list changes nothing."""
+    m = _mesh(3, noise=0.02, seed=12, pin=False)
+    for mode in ("gs", "jacobi"):
+        a = orc.Sim(m, iters=4, mode=mode)
+        a.step(2)
+        x0, v0 = a.x(), a.v()
+        b = orc.Sim(m, iters=4, mode=mode)
+        b.set_state(x0, v0)
+        b.step(1)
+        a.predict()
+        xp = a.x()
+        a.solve_elements(np.arange(m.nt), num_iter=4, initialize=True)
+        assert np.array_equal(a.x(), b.x()) and np.array_equal(a.lambdas(), b.lambdas())
+        c = orc.Sim(m, iters=4, mode=mode)
+        c.set_state(xp, None)
+        c.solve_elements(np.arange(m.nt), num_iter=1, initialize=True)
+        c.solve_elements(np.arange(m.nt), num_iter=3, initialize=False)
+        assert np.array_equal(a.x(), c.x()) and np.array_equal(a.lambdas(), c.lambdas())
+        # element 0 listed twice: two projections of it per pass
+        d1 = orc.Sim(m, iters=1, mode=mode)
+        d1.set_state(xp, None)
+        d1.solve_elements([0, 0], num_iter=1, initialize=True)
+        d2 = orc.Sim(m, iters=1, mode=mode)
+        d2.set_state(xp, None)
+        d2.solve_elements([0], num_iter=1, initialize=True)
+        assert not np.array_equal(d1.lambdas()[0], d2.lambdas()[0])
+        e = orc.Sim(m, iters=4, mode=mode)
+        e.set_state(xp, None)
+        e.solve_elements(np.zeros(0, np.int32), num_iter=3, initialize=True)
+        assert np.array_equal(e.x(), xp)

@@ -73,7 +73,8 @@ TetArrays tet_arrays(xpbd_handle* h)
     return ta;
 }
 
-bool contacts_enabled(const xpbd_handle* h) { return h->nf > 0 && h->sdf_active; }
+// contacts are projected when a collider detects them or the caller gave them
+bool contacts_enabled(const xpbd_handle* h) { return h->nf > 0 && (h->sdf_active || h->contacts_given); }
 
 PatchArrays patch_arrays(xpbd_handle* h)
 {
@@ -962,7 +963,7 @@ int collect_timing(xpbd_handle* h)
 bool detect_this_step(const xpbd_handle* h)
 {
     const int k = h->params.collision_every;
-    return contacts_enabled(h) && k > 0 && (h->host_step % k) == 0;
+    return h->nf > 0 && h->sdf_active && k > 0 && (h->host_step % k) == 0;
 }
 
 int run_steps(xpbd_handle* h, int32_t nsteps)
@@ -1386,10 +1387,113 @@ extern "C" int xpbd_set_sdf_velocity(xpbd_handle* h, const double* vel3)
 extern "C" int xpbd_detect_contacts(xpbd_handle* h)
 {
     if (!h) return XPBD_E_INVALID_ARG;
-    if (!contacts_enabled(h)) return XPBD_OK;
+    if (!(h->nf > 0 && h->sdf_active)) return XPBD_OK;
     int rc = enqueue_detect(h);
     if (rc) return rc;
     return xpbd_synchronize(h);
+}
+
+// XPBDMeshObject_::clearCollisionConstraints + addStaticCollisionConstraint
+// (XPBDMeshObject.cpp:180-251): the contact set replaced by the caller's.
+extern "C" int xpbd_set_contacts(xpbd_handle* h, int32_t n, const int32_t* faces, const double* p,
+                                 const double* nrm, const double* bary)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (n < 0 || (n > 0 && (!faces || !p || !nrm || !bary)))
+        return set_err(h, XPBD_E_INVALID_ARG, "set_contacts: n >= 0 and all arrays required");
+    if (n > 0 && h->nf == 0) return set_err(h, XPBD_E_STATE, "set_contacts: the object has no surface faces");
+    if (n > 0 && (!finite_all(p, 3 * (size_t)n) || !finite_all(nrm, 3 * (size_t)n) || !finite_all(bary, 3 * (size_t)n)))
+        return set_err(h, XPBD_E_NONFINITE, "set_contacts: non-finite input");
+    const int nf = h->nf;
+    std::vector<int32_t> slot(nf), flag(std::max(nf, 1), 0);
+    for (int j = 0; j < nf; j++) slot[h->fperm[j]] = j;
+    std::vector<double> cp(3 * (size_t)std::max(nf, 1), 0.0), cn(cp), cb(cp);
+    for (int c = 0; c < n; c++) {
+        const int f = faces[c];
+        if (f < 0 || f >= nf) return set_err(h, XPBD_E_INVALID_ARG, "set_contacts: face index out of range");
+        const int j = slot[f];
+        if (flag[j]) return set_err(h, XPBD_E_INVALID_ARG, "set_contacts: a face may carry one contact");
+        flag[j] = 1;
+        for (int k = 0; k < 3; k++) {
+            cp[3 * (size_t)j + k] = p[3 * (size_t)c + k];
+            cn[3 * (size_t)j + k] = nrm[3 * (size_t)c + k];
+            cb[3 * (size_t)j + k] = bary[3 * (size_t)c + k];
+        }
+    }
+    if (nf > 0) {
+        hipError_t e = hipMemcpyAsync(h->d_flag, flag.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h->d_cp, cp.data(), sizeof(double) * 3 * nf, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h->d_cn, cn.data(), sizeof(double) * 3 * nf, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h->d_cb, cb.data(), sizeof(double) * 3 * nf, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = launch_fill_zero(h->stream, h->d_clam, sizeof(double) * nf);
+        if (e == hipSuccess) e = launch_compact(h->stream, h->d_flag, nf, h->d_cpos, h->d_clist, h->d_bsum);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return hip_err(h, e, "set_contacts");
+    }
+    const bool given = n > 0;
+    if (given != h->contacts_given) {
+        h->contacts_given = given;
+        invalidate_graphs(h);  // the captured steps depend on whether contacts are projected
+    }
+    return XPBD_OK;
+}
+
+// XPBDSolver::solve(projector_references, num_iter, initialize)
+// (XPBDSolver.hpp:106-125) on a list of elements, optionally followed by
+// every contact, between steps.
+extern "C" int xpbd_solve_elements(xpbd_handle* h, int32_t n, const int32_t* elements, int32_t num_iter,
+                                   int32_t initialize, int32_t with_contacts)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (n < 0 || num_iter < 0 || (n > 0 && !elements))
+        return set_err(h, XPBD_E_INVALID_ARG, "solve_elements: n >= 0, num_iter >= 0 and an element list required");
+    const int nt = h->nt;
+    std::vector<int32_t> rank(nt), mult(std::max(nt, 1), 0);
+    for (int i = 0; i < nt; i++) rank[h->order[i]] = i;
+    for (int u = 0; u < n; u++) {
+        const int e = elements[u];
+        if (e < 0 || e >= nt) return set_err(h, XPBD_E_INVALID_ARG, "solve_elements: element index out of range");
+        mult[rank[e]]++;
+    }
+    hipStream_t s = h->stream;
+    hipError_t e = hipSuccess;
+    if (!h->d_mult) {
+        if (hipMalloc(reinterpret_cast<void**>(&h->d_mult), sizeof(int32_t) * (size_t)std::max(nt, 1)) != hipSuccess)
+            return set_err(h, XPBD_E_NOMEM, "solve_elements: count buffer allocation failed");
+        h->allocs.push_back(h->d_mult);
+    }
+    e = hipMemcpyAsync(h->d_mult, mult.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice, s);
+    const TetArrays ta = tet_arrays(h);
+    if (e == hipSuccess && initialize) e = launch_reset_listed(s, ta.lam, h->d_mult, nt);
+    if (e != hipSuccess) return hip_err(h, e, "solve_elements set-up");
+    const bool con = with_contacts && contacts_enabled(h);
+    const bool gs = h->params.solver == XPBD_SOLVER_GS_COLORED;
+    const bool cluster = h->variant == XPBD_VARIANT_CLUSTER || h->variant == XPBD_VARIANT_CLUSTER_QUAD;
+    for (int it = 0; it < num_iter; it++) {
+        const bool first = initialize && it == 0;  // contacts restart from zero multipliers
+        if (gs) {
+            for (int c = 0; c < h->ncolors && e == hipSuccess; c++)
+                e = cluster ? launch_gs_local_cluster(s, ta, h->d_cstart, h->d_mult, h->d_pos, h->color_off[c],
+                                                      h->color_off[c + 1])
+                            : launch_gs_local_color(s, ta, h->d_mult, h->d_pos, h->color_off[c], h->color_off[c + 1]);
+            if (e == hipSuccess && con)
+                e = launch_contacts_gs(s, h->d_faces, h->d_fci_off, h->nfcolors, h->ninst, h->d_cpos, h->d_clist,
+                                       h->d_rest, h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
+            if (e != hipSuccess) return hip_err(h, e, "solve_elements launch");
+        } else {
+            e = launch_jacobi_local(s, ta, h->d_mult, h->d_pos, h->d_acc, nt);
+            if (e != hipSuccess) return hip_err(h, e, "solve_elements launch");
+            if (con) {
+                int rc = enqueue_contacts_jacobi(h, first);
+                if (rc) return rc;
+            }
+            e = launch_jacobi_apply(s, h->d_pos, h->d_acc, h->nv);
+            if (e != hipSuccess) return hip_err(h, e, "solve_elements apply launch");
+        }
+    }
+    e = launch_commit(s, h->d_pos, h->d_lo, h->nv);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "solve_elements");
 }
 
 extern "C" int xpbd_get_contacts(xpbd_handle* h, int32_t cap, int32_t* count, int32_t* face, double* p, double* n,

@@ -50,6 +50,7 @@ struct xpbd_handle {
     std::vector<int32_t> fci_off;
     std::vector<xpbd::SdfDev> sdf_host;
     bool sdf_active = false;
+    bool contacts_given = false;  // a contact set from xpbd_set_contacts is in use
 
     // device state
     float4* d_pos = nullptr;     // 2 per vertex: (base displacement, inverse mass), (in-step motion, 0)

@@ -47,6 +47,10 @@ hipError_t launch_jacobi_local_det(hipStream_t s, const TetArrays& ta, const int
                                    int nt);
 hipError_t launch_local_gather(hipStream_t s, const TetArrays& ta, int nv, const int* vt_off, const int* vt_pos,
                                const int* mult, const float* edx, float* acc);
+// in-step motion folded into the base, no velocity update (positions moved between steps)
+hipError_t launch_commit(hipStream_t s, float4* pos, float4* lo, int nv);
+// multipliers of the elements with mult > 0 set to zero
+hipError_t launch_reset_listed(hipStream_t s, float2* lam, const int* mult, int nt);
 // lambda = -C / alpha~ of every element at the current positions (learned step start)
 hipError_t launch_lambda_start(hipStream_t s, const TetArrays& ta, const float4* pos, const float4* lo, int nt);
 hipError_t launch_gs_color(hipStream_t s, const TetArrays& ta, float4* pos, int begin, int end, bool first);

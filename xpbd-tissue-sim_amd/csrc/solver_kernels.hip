@@ -125,6 +125,32 @@ __global__ __launch_bounds__(kBlock) void k_finish(float4* __restrict__ pos, flo
     lo[i] = l;
 }
 
+// The in-step motion folded into the base (compensated, as k_finish does),
+// without the velocity update, the pin reset or the step counter: positions
+// moved between steps (a subset solve) stay where they are.
+__global__ __launch_bounds__(kBlock) void k_commit(float4* __restrict__ pos, float4* __restrict__ lo, int nv)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nv) return;
+    float4 b = pos[2 * (size_t)i];
+    const float4 d = pos[2 * (size_t)i + 1];
+    float4 l = lo[i];
+    two_sum(b.x, d.x + l.x, b.x, l.x);
+    two_sum(b.y, d.y + l.y, b.y, l.y);
+    two_sum(b.z, d.z + l.z, b.z, l.z);
+    pos[2 * (size_t)i] = b;
+    pos[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    lo[i] = l;
+}
+
+// Multipliers of the listed elements (mult > 0) back to zero: a projector's
+// initialize() (XPBDSolver.hpp:108-116).
+__global__ __launch_bounds__(kBlock) void k_reset_listed(float2* __restrict__ lam, const int* __restrict__ mult, int nt)
+{
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e < nt && mult[e] > 0) lam[e] = make_float2(0.f, 0.f);
+}
+
 // ---------------------------------------------------------------------------
 // Colour-ordered Gauss-Seidel: one launch per colour; tets of one colour share
 // no vertex, so each thread reads and writes its four vertices directly.
@@ -2074,6 +2100,20 @@ hipError_t launch_local_gather(hipStream_t s, const TetArrays& ta, int nv, const
     if (nv <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_local_gather, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, ta, nv, vt_off, vt_pos, mult, edx,
                        acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit(hipStream_t s, float4* pos, float4* lo, int nv)
+{
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_commit, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos, lo, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_listed(hipStream_t s, float2* lam, const int* mult, int nt)
+{
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reset_listed, dim3(nblk(nt, kBlock)), dim3(kBlock), 0, s, lam, mult, nt);
     return hipGetLastError();
 }
 

@@ -87,6 +87,8 @@ _sig("xpbd_get_face_rank", [_vp, _ip])
 _sig("xpbd_get_color_offsets", [_vp, _ip])
 _sig("xpbd_set_learned_start", [_vp, C.c_int32])
 _sig("xpbd_mlp_forward_noalloc", [_vp, C.c_int32, _fp, _fp])
+_sig("xpbd_set_contacts", [_vp, C.c_int32, _ip, _dp, _dp, _dp])
+_sig("xpbd_solve_elements", [_vp, C.c_int32, _ip, C.c_int32, C.c_int32, C.c_int32])
 _sig("xpbd_set_sdf", [_vp, C.c_int32, _dp, _dp])
 _sig("xpbd_set_sdf_velocity", [_vp, _dp])
 _sig("xpbd_detect_contacts", [_vp])
@@ -417,6 +419,23 @@ class Solver:
                                       ptr(n, C.c_double), ptr(b, C.c_double)))
         k = cnt.value
         return face[:k], p[:k], n[:k], b[:k]
+
+    def set_contacts(self, faces, p, n, bary):
+        """Replace the contact set (clearCollisionConstraints +
+        addStaticCollisionConstraint per contact); empty arrays clear it."""
+        f = np.ascontiguousarray(faces, np.int32).reshape(-1)
+        pp = np.ascontiguousarray(p, np.float64).reshape(-1)
+        nn = np.ascontiguousarray(n, np.float64).reshape(-1)
+        bb = np.ascontiguousarray(bary, np.float64).reshape(-1)
+        self._c(lib.xpbd_set_contacts(self.h, len(f), ptr(f, C.c_int32), ptr(pp, C.c_double), ptr(nn, C.c_double),
+                                      ptr(bb, C.c_double)))
+
+    def solve_elements(self, elements, num_iter=1, initialize=False, with_contacts=False):
+        """XPBDSolver::solve(projector_references, num_iter, initialize) on a list
+        of elements (repeats allowed), optionally followed by every contact."""
+        e = np.ascontiguousarray(elements, np.int32).reshape(-1)
+        self._c(lib.xpbd_solve_elements(self.h, len(e), ptr(e, C.c_int32), int(num_iter), int(bool(initialize)),
+                                        int(bool(with_contacts))))
 
     def set_kernel_timing(self, on: bool):
         self._c(lib.xpbd_set_kernel_timing(self.h, int(on)))
