@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--learned-steps", type=int, default=50, help="steps of the learned-vs-plain comparison (0: off)")
-    ap.add_argument("--learned-iters", type=int, default=1)
+    ap.add_argument("--learned-iters", type=int, default=3, help="XPBD iterations of a learned step")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every step eagerly (kernel timing mode) instead of replaying HIP graphs")
     ap.add_argument("--tune-cache", default=None,
@@ -120,10 +120,10 @@ def traffic_for(workload, solver, launch):
         return None, None, "no counter profile for this workload"
     with open(tfile) as f:
         tj = json.load(f)
-    rec = tj.get("launch_config")
-    if rec is None or list(rec) != list(launch):
-        return None, tj.get("kernel"), f"counter profile recorded for launch config {rec}, not {list(launch)}"
-    return tj.get("bytes_per_iteration"), tj.get("kernel"), None
+    for ent in tj.get("by_launch_config", []):
+        if list(ent["launch_config"]) == list(launch):
+            return ent["bytes_per_iteration"], tj.get("kernel"), None
+    return None, tj.get("kernel"), f"no counter profile recorded for launch config {list(launch)}"
 
 
 def measure(s, cfg, workload, steps, warmup, timing_steps, no_graph=False, tune_cache=None, variant="auto"):
@@ -254,10 +254,16 @@ def learned_vs_plain(a, net_path):
     a.learned_iters iterations per step:
       with_fallback  network correction, multiplier start (the default);
       no_correction  the same step with a network that outputs zero;
-      positions_start  network correction, multipliers from zero.
-    Each run reports its trajectory error against the plain run after the
-    last step, as max |x - x_plain| / max |x_plain| and in units of the plain
-    run's largest displacement from rest."""
+      positions_start  network correction, multipliers from zero;
+      positions_start_no_fallback  the same, never falling back (its residual
+                       is reported, not judged);
+    and plain_same_iterations is the plain solver with a.learned_iters
+    iterations (no network, no fallback), for scale.  Each run reports its
+    trajectory error against the plain run after the last step, as
+    max |x - x_plain| / max |x_plain| and in units of the plain run's largest
+    displacement from rest.  The learned runs replay captured steps; the
+    network's inference time comes from a separate run of a few steps in
+    kernel timing mode."""
     import numpy as np
     import train_warmstart as TW
     import xpbd_amd as X
@@ -280,34 +286,55 @@ def learned_vs_plain(a, net_path):
            "steps": a.learned_steps, "weights": os.path.relpath(net_path, ROOT),
            "plain_steps_per_s_with_residual_readback": a.learned_steps / t_plain,
            "plain_residual_median": float(np.median(r_plain))}
-    for mode, network, start in (("with_fallback", net, "multipliers"), ("no_correction", zero, "multipliers"),
-                                 ("positions_start", net, "positions")):
+    for mode, network, start, judged in (("with_fallback", net, "multipliers", True),
+                                         ("no_correction", zero, "multipliers", True),
+                                         ("positions_start", net, "positions", True),
+                                         ("positions_start_no_fallback", net, "positions", False)):
         s, _, _, _ = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
         s.attach(network)
         s.set_learned_start(start)
-        s.set_kernel_timing(True)
         ratios, fb, t = [], 0, 0.0
         for n in range(a.learned_steps):
             t0 = time.perf_counter()
-            f, r = s.step_learned(a.learned_iters, r_plain[n], 1.1)
+            f, r = s.step_learned(a.learned_iters, r_plain[n] if judged else 0.0, 1.1)
             t += time.perf_counter() - t0
             fb += int(f)
             ratios.append(r / max(r_plain[n], 1e-300))
-        st = s.learned_stats()
         x = s.positions()
         ratios = np.array(ratios)
-        ms_inf = st["mlp_ms"] / max(st["mlp_calls"], 1)
         dx = float(np.max(np.abs(x - x_plain)))
         out[mode] = {"start": start, "steps_per_s": a.learned_steps / t, "fallback_rate": fb / a.learned_steps,
                      "residual_ratio_median": float(np.median(ratios)), "residual_ratio_max": float(ratios.max()),
                      "rel_pos_err_vs_plain_end": dx / float(np.max(np.abs(x_plain))),
-                     "traj_err_vs_plain_end_over_displacement": dx / disp,
-                     "mlp_ms_per_inference": ms_inf}
+                     "traj_err_vs_plain_end_over_displacement": dx / disp}
         if mode == "with_fallback":
+            # inference time: a few more steps, launched eagerly with HIP events around the network
+            s.set_kernel_timing(True)
+            for _ in range(5):
+                s.step_learned(a.learned_iters, 0.0, 1.1)
+            st = s.learned_stats()
+            ms_inf = st["mlp_ms"] / max(st["mlp_calls"], 1)
             flop = MLP_FLOP_PER_VERTEX * s.nv
+            out["mlp_ms_per_inference"] = ms_inf
             out["mlp_tflops"] = flop / (ms_inf * 1e-3) / 1e12
             out["mlp_mfma_util"] = out["mlp_tflops"] / MFMA_BF16_PEAK_TFLOPS
         s.close()
+    # the plain solver at the learned step's iteration count, for scale
+    s, _, _, _ = build_workload(a.workload, a.solver, a.learned_iters, a.variant, 0 if a.nondeterministic else 1)
+    s.set_residual_policy(True)
+    ratios, t = [], 0.0
+    for n in range(a.learned_steps):
+        t0 = time.perf_counter()
+        s.step(1)
+        ratios.append(s.last_step_residual() / max(r_plain[n], 1e-300))
+        t += time.perf_counter() - t0
+    dx = float(np.max(np.abs(s.positions() - x_plain)))
+    out["plain_same_iterations"] = {"iterations": a.learned_iters, "steps_per_s": a.learned_steps / t,
+                                    "residual_ratio_median": float(np.median(ratios)),
+                                    "residual_ratio_max": float(np.max(ratios)),
+                                    "rel_pos_err_vs_plain_end": dx / float(np.max(np.abs(x_plain))),
+                                    "traj_err_vs_plain_end_over_displacement": dx / disp}
+    s.close()
     out["learned_over_plain_at_equal_residual"] = (out["with_fallback"]["steps_per_s"] /
                                                    out["plain_steps_per_s_with_residual_readback"])
     return out

@@ -209,6 +209,44 @@ def test_fallback_reproduces_the_plain_step_exactly(solver, variant):
     assert st["steps"] == 5 and st["fallbacks"] == 5
 
 
+@pytest.mark.parametrize("solver,variant", [("gs", "quad"), ("jacobi", "patch")])
+def test_captured_learned_step_equals_the_eager_one(solver, variant):
+    """The learned step replays a captured graph; in kernel timing mode it
+    launches eagerly.  Both give the same bits, step after step, through
+    fallbacks forced on every third step, detection steps, and a change of the
+    network's output scale after the graphs were captured."""
+    import train_warmstart as TW
+    if solver == "gs":
+        mesh, sph = _held_out_block(seed=79, n=6)
+    else:  # the reference's plain-sum Jacobi is stable on soft material only
+        mesh = W.cube(6, seed=10, E=1e3)
+        b = W.block_sphere(6, r=0.02)
+        sph = (b[0], (0.03, 0.03, 0.06 + 0.02 - 0.0005), b[2], b[3], b[4])
+    graph, eager = (X.Solver(mesh, dt=1e-3, iters=6, solver=solver, variant=variant) for _ in range(2))
+    net = TW.load(TW.DEFAULT_WEIGHTS)
+    for s in (graph, eager):
+        s.set_sdf(*sph[:4])
+        s.set_sdf_velocity(sph[4])
+        s.set_residual_policy(True)
+        s.attach(net)
+    eager.set_kernel_timing(True)
+    z = np.load(TW.DEFAULT_WEIGHTS, allow_pickle=False)
+    fbs = 0
+    for k in range(12):
+        if k == 6:
+            net.set_scaling(z["mean"], z["std"], 0.5 * float(z["scale"]))
+        ref = 1e-30 if k % 3 == 2 else 1e30
+        fg, rg = graph.step_learned(3, ref_residual=ref, ratio=1.1)
+        fe, re = eager.step_learned(3, ref_residual=ref, ratio=1.1)
+        assert fg == fe == (k % 3 == 2), k
+        assert rg == re, k
+        assert np.array_equal(graph.positions(), eager.positions()), k
+        assert np.array_equal(graph.velocities(), eager.velocities()), k
+        fbs += fg
+    assert graph.learned_stats()["fallbacks"] == fbs == 4
+    assert eager.learned_stats()["mlp_calls"] == 12
+
+
 def test_learned_step_requires_a_network():
     mesh = W.cube(4, seed=1)
     gpu = X.Solver(mesh, dt=1e-3, iters=5, solver="gs")

@@ -57,6 +57,7 @@ static void release_graphs(xpbd_handle* h)
         if (h->graph_exec[i]) hipGraphExecDestroy(h->graph_exec[i]);
         h->graph_exec[i] = nullptr;
     }
+    h->graph_gen++;
 }
 
 void invalidate_graphs(xpbd_handle* h) { release_graphs(h); }

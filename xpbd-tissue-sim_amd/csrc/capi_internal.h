@@ -121,6 +121,7 @@ struct xpbd_handle {
     int residual_policy = 0;     // 1: record ||r|| at the end of every solve (reference EVERY_SUBSTEP)
     double* d_step_res = nullptr;
     hipGraphExec_t graph_exec[2] = { nullptr, nullptr };  // [0] plain step, [1] step with detection
+    uint64_t graph_gen = 0;      // bumped whenever captured steps go stale (other graphs compare against it)
 
     // kernel timing
     bool timing = false;

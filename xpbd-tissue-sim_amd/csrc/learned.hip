@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -479,6 +480,12 @@ __global__ void k_zero_delta(float4* __restrict__ pos, int nv)
 // ---------------------------------------------------------------------------
 // network object
 // ---------------------------------------------------------------------------
+static uint64_t next_mlp_version()
+{
+    static std::atomic<uint64_t> counter{ 0 };
+    return ++counter;
+}
+
 struct xpbd_mlp {
     int device = 0;           // device current at creation (weights and stream live there)
     hipStream_t stream = nullptr;
@@ -493,6 +500,10 @@ struct xpbd_mlp {
     float* d_istd = nullptr;
     float* d_split = nullptr; // split-K partial products
     float scale = 1.f;
+    // Identifies the network and its output scale (the scale is a launch
+    // argument of the inference, so graphs that captured it go stale when it
+    // changes); never reused within the process.
+    uint64_t version = 0;
     float h_mean[24] = {}, h_std[24] = {};  // host copies of the scaling (for xpbd_mlp_save)
     bool has_scaling = false;
     int64_t adam_t = 0;
@@ -626,6 +637,7 @@ extern "C" int xpbd_mlp_create(xpbd_mlp** out)
     *out = nullptr;
     xpbd_mlp* m = new (std::nothrow) xpbd_mlp();
     if (!m) return XPBD_E_NOMEM;
+    m->version = next_mlp_version();
     hipError_t e = hipGetDevice(&m->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     auto A = [&](void** p, size_t bytes) {
@@ -720,6 +732,7 @@ extern "C" int xpbd_mlp_set_scaling(xpbd_mlp* m, const float* mean_in, const flo
     if (e == hipSuccess) e = hipMemcpyAsync(m->d_istd, istd, sizeof(float) * kIn, hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     m->scale = scale_out;
+    m->version = next_mlp_version();
     if (e == hipSuccess) {
         std::memcpy(m->h_mean, mean_in, sizeof(float) * kIn);
         std::memcpy(m->h_std, std_in, sizeof(float) * kIn);
@@ -979,6 +992,15 @@ struct Learned {
     float4* d_corr = nullptr;    // network output
     float4* d_save = nullptr;    // snapshot for the fallback: pos (2 nv), lo (nv), vel (nv)
     unsigned long long* d_step_save = nullptr;
+    double* d_res = nullptr;     // squared residual of the learned solve
+    // Captured learned steps, one per (network version, handle graph
+    // generation, detection, iterations, start mode).
+    struct Graph {
+        uint64_t net = 0, gen = 0;
+        int det = 0, iters = 0, start = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
     // inference timing (kernel timing mode of the handle)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double mlp_ms = 0;
@@ -991,8 +1013,10 @@ struct Learned {
 static void free_learned(Learned* L)
 {
     if (!L) return;
+    for (auto& g : L->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     void* bufs[] = { L->d_nb_off, L->d_nb, L->d_vmat, L->d_pinfrac, L->d_feat, L->d_pred,
-                     L->d_cprev,  L->d_corr, L->d_save, L->d_step_save };
+                     L->d_cprev,  L->d_corr, L->d_save, L->d_step_save, L->d_res };
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (L->ev0) (void)hipEventDestroy(L->ev0);
@@ -1061,6 +1085,7 @@ static int ensure_learned(xpbd_handle* h)
     A(&L->d_corr, sizeof(float4) * nv);
     A(&L->d_save, sizeof(float4) * 4 * (size_t)nv);
     A(&L->d_step_save, sizeof(unsigned long long));
+    A(&L->d_res, sizeof(double));
     if (e == hipSuccess) e = hipMemcpy(L->d_nb_off, off.data(), sizeof(int) * (nv + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess && !nbr.empty())
         e = hipMemcpy(L->d_nb, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice);
@@ -1200,21 +1225,22 @@ extern "C" int xpbd_collect_training(xpbd_handle* h, int32_t nsteps, int32_t cap
     return XPBD_OK;
 }
 
-extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_residual, double fallback_ratio,
-                                 int32_t* fell_back, double* residual)
+namespace xpbd {
+// The learned step up to its end, before the fallback decision: detection (on
+// the plain solver's cadence), a snapshot of the start-of-step state, the
+// prediction, features, network correction, the multiplier start, iters
+// iterations, the squared residual into L->d_res, then the rest of the step
+// (EVERY_SUBSTEP residual, local collision passes, the correction history and
+// the velocity update).  A fallback restores the snapshot and redoes the
+// whole step as the plain solver does, so finishing the step before the
+// decision changes nothing on either branch.
+static int enqueue_learned_attempt(xpbd_handle* h, int32_t iters, bool det, bool timed)
 {
-    if (!h) return XPBD_E_INVALID_ARG;
-    if (iters < 0 || !std::isfinite(ref_residual) || !std::isfinite(fallback_ratio))
-        return set_err(h, XPBD_E_INVALID_ARG, "step_learned: iters >= 0 and finite thresholds required");
-    if (!h->learned || !h->learned->mlp) return set_err(h, XPBD_E_STATE, "no network attached");
     Learned* L = h->learned;
-    xpbd_mlp* m = L->mlp;
     const int nv = h->nv;
     hipStream_t s = h->stream;
     int rc;
-    const bool det = detect_this_step(h);
     if (det && (rc = enqueue_detect(h))) return rc;
-    // snapshot for the fallback (state at the start of the step)
     hipError_t e = hipMemcpyAsync(L->d_save, h->d_pos, sizeof(float4) * 2 * (size_t)nv, hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(L->d_save + 2 * (size_t)nv, h->d_lo, sizeof(float4) * nv, hipMemcpyDeviceToDevice, s);
@@ -1224,17 +1250,81 @@ extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_resid
         e = hipMemcpyAsync(L->d_step_save, h->d_step, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_err(h, e, "learned snapshot");
     if ((rc = enqueue_predict(h)) || (rc = enqueue_copy_pred(h)) || (rc = enqueue_features(h))) return rc;
-    if (h->timing) (void)hipEventRecord(L->ev0, s);
-    e = mlp_infer(m, s, L->d_feat, nv, L->d_corr);
-    if (h->timing) (void)hipEventRecord(L->ev1, s);
+    if (timed) (void)hipEventRecord(L->ev0, s);
+    e = mlp_infer(L->mlp, s, L->d_feat, nv, L->d_corr);
+    if (timed) (void)hipEventRecord(L->ev1, s);
     if (e != hipSuccess) return hip_err(h, e, "mlp launch");
     hipLaunchKernelGGL(k_apply_corr, dim3(nb(nv)), dim3(256), 0, s, h->d_pos, h->d_vel, L->d_corr, nv);
     if ((e = hipGetLastError()) != hipSuccess) return hip_err(h, e, "apply launch");
     const bool warm = L->start == XPBD_LEARNED_START_MULTIPLIERS;
     if (warm && (rc = enqueue_multiplier_start(h))) return rc;
     if ((rc = enqueue_iterations(h, iters, false, !warm))) return rc;
-    double r = 0;
-    if ((rc = device_residual(h, &r, nullptr))) return rc;
+    e = launch_residual(s, tet_arrays(h), h->d_pos, h->d_lo, true, h->nt, h->d_part, h->red_blocks, nullptr);
+    if (e == hipSuccess) e = launch_sum_partials(s, h->d_part, h->red_blocks, L->d_res);
+    if (e != hipSuccess) return hip_err(h, e, "learned residual");
+    if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+    if ((rc = enqueue_local_collision(h))) return rc;
+    if ((rc = enqueue_store_cprev(h)) || (rc = enqueue_finish(h))) return rc;
+    return XPBD_OK;
+}
+
+// Launch the attempt from a captured graph (built on first use of its shape).
+static int launch_learned_attempt(xpbd_handle* h, int32_t iters, bool det)
+{
+    Learned* L = h->learned;
+    const uint64_t net = L->mlp->version;
+    hipGraphExec_t exec = nullptr;
+    for (size_t i = 0; i < L->graphs.size();) {
+        Learned::Graph& g = L->graphs[i];
+        if (g.gen != h->graph_gen) {  // the handle's captured steps went stale: so did this one
+            (void)hipGraphExecDestroy(g.exec);
+            L->graphs[i] = L->graphs.back();
+            L->graphs.pop_back();
+            continue;
+        }
+        if (g.net == net && g.det == (int)det && g.iters == iters && g.start == L->start) exec = g.exec;
+        i++;
+    }
+    if (!exec) {
+        hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_err(h, e, "hipStreamBeginCapture");
+        int rc = enqueue_learned_attempt(h, iters, det, false);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(h->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_err(h, e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_err(h, e, "hipGraphInstantiate");
+        L->graphs.push_back({ net, h->graph_gen, (int)det, iters, L->start, exec });
+    }
+    hipError_t e = hipGraphLaunch(exec, h->stream);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "hipGraphLaunch");
+}
+}  // namespace xpbd
+
+extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_residual, double fallback_ratio,
+                                 int32_t* fell_back, double* residual)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (iters < 0 || !std::isfinite(ref_residual) || !std::isfinite(fallback_ratio))
+        return set_err(h, XPBD_E_INVALID_ARG, "step_learned: iters >= 0 and finite thresholds required");
+    if (!h->learned || !h->learned->mlp) return set_err(h, XPBD_E_STATE, "no network attached");
+    Learned* L = h->learned;
+    const int nv = h->nv;
+    hipStream_t s = h->stream;
+    const bool det = detect_this_step(h);
+    // kernel timing mode launches eagerly and times the inference with events
+    int rc = h->timing ? enqueue_learned_attempt(h, iters, det, true) : launch_learned_attempt(h, iters, det);
+    if (rc) return rc;
+    double r2 = 0;
+    hipError_t e = hipMemcpyAsync(&r2, L->d_res, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_err(h, e, "learned step");
+    double r = std::sqrt(r2);
     if (h->timing) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, L->ev0, L->ev1);
@@ -1255,13 +1345,13 @@ extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_resid
         if ((rc = enqueue_predict(h)) || (rc = enqueue_copy_pred(h))) return rc;
         if ((rc = enqueue_iterations(h, h->params.iters, false))) return rc;
         if ((rc = device_residual(h, &r, nullptr))) return rc;
+        if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+        if ((rc = enqueue_local_collision(h))) return rc;
+        if ((rc = enqueue_store_cprev(h)) || (rc = enqueue_finish(h))) return rc;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_err(h, e, "learned fallback");
         L->fallbacks++;
     }
-    if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
-    if ((rc = enqueue_local_collision(h))) return rc;
-    if ((rc = enqueue_store_cprev(h)) || (rc = enqueue_finish(h))) return rc;
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_err(h, e, "learned step");
     h->host_step++;
     L->steps++;
     if (fell_back) *fell_back = fb ? 1 : 0;

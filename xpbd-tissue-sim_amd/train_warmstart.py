@@ -9,12 +9,16 @@ seeded family of meshes:
   * heterogeneous blocks (E log-uniform in [1e3, 5e4] Pa, nu in [0.45, 0.49],
     base pinned) of 8..14 cells per side pressed by a moving sphere, like the
     1M-element tissue block.
-Every step contributes one row per free vertex: the 24 features after the
-inertial prediction and the solver's correction x_solved - x_predicted.
-Collection and training (BF16 MFMA forward/backward, Adam) both run through
-the HIP library.
+Every step yields one row per free vertex: the 24 features after the
+inertial prediction and the solver's correction x_solved - x_predicted.  A
+seeded fraction of each step's rows is kept (row subsampling), so that many
+steps of many meshes fit one training set.  Collection and training (BF16
+MFMA forward/backward, Adam) both run through the HIP library.
 
-    python train_warmstart.py --out weights/warmstart.npz [--steps 100 --meshes 8 --epochs 20]
+    python train_warmstart.py --out weights/warmstart.npz \
+        [--beams 200 --blocks 40 --steps 250 --keep 0.0625 --epochs 20]
+
+(200 beams x 250 steps = 50k steps of the beam family, plus 40 blocks x 250.)
 """
 from __future__ import annotations
 
@@ -52,22 +56,41 @@ def family_member(kind: str, seed: int):
     return mesh, (X.SDF_SPHERE, pose, (0.0, 0.0, 0.0, 1.0), (r, 0.0, 0.0), vel)
 
 
-def collect(kind: str, seed: int, steps: int, iters: int = 20):
+def collect(kind: str, seed: int, steps: int, iters: int = 20, keep: float = 1.0, chunk: int = 50):
+    """Rows of `steps` plain steps of one family member; a seeded fraction
+    `keep` of each chunk's rows is kept."""
     mesh, (k, pos, quat, prm, vel) = family_member(kind, seed)
     s = X.Solver(mesh, dt=1e-3, iters=iters, solver="gs")
     s.set_sdf(k, pos, quat, prm)
     s.set_sdf_velocity(vel)
     s.attach(X.Network())
-    return s.collect_training(steps)
-
-
-def build_dataset(nmeshes: int, steps: int, seed0: int = 1000):
+    rng = np.random.default_rng(seed + 7919)
     xs, ys = [], []
-    for i in range(nmeshes):
-        for kind in ("beam", "block"):
-            x, y = collect(kind, seed0 + 2 * i + (kind == "block"), steps)
-            xs.append(x)
-            ys.append(y)
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        x, y = s.collect_training(n)
+        if keep < 1.0:
+            sel = rng.random(len(x)) < keep
+            x, y = x[sel], y[sel]
+        xs.append(x)
+        ys.append(y)
+        done += n
+    s.close()
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def build_dataset(beams: int, blocks: int, steps: int, keep: float, seed0: int = 1000):
+    xs, ys = [], []
+    members = [("beam", seed0 + 2 * i) for i in range(beams)] + [("block", seed0 + 2 * i + 1) for i in range(blocks)]
+    t0 = time.time()
+    for i, (kind, seed) in enumerate(members):
+        x, y = collect(kind, seed, steps, keep=keep)
+        xs.append(x)
+        ys.append(y)
+        if (i + 1) % 20 == 0 or i + 1 == len(members):
+            print(f"collected {i + 1}/{len(members)} meshes, {sum(len(v) for v in xs)} rows, "
+                  f"{time.time() - t0:.1f} s", flush=True)
     return np.concatenate(xs), np.concatenate(ys)
 
 
@@ -93,16 +116,18 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
                                                  "warmstart.npz"))
-    ap.add_argument("--meshes", type=int, default=8, help="meshes per kind (beam, block)")
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--beams", type=int, default=200, help="beam-family meshes")
+    ap.add_argument("--blocks", type=int, default=40, help="block-family meshes")
+    ap.add_argument("--steps", type=int, default=250, help="steps per mesh")
+    ap.add_argument("--keep", type=float, default=0.0625, help="fraction of each step's rows kept")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     a = ap.parse_args()
     t0 = time.time()
-    x, y = build_dataset(a.meshes, a.steps)
+    x, y = build_dataset(a.beams, a.blocks, a.steps, a.keep)
     t1 = time.time()
-    xe, ye = collect("block", 99, a.steps)
+    xe, ye = collect("block", 99, 100)
     net = X.Network()
     mean, std, scale = net.fit_scaling(x, y)
     loss = net.train(x, y, batch=a.batch, epochs=a.epochs, lr=a.lr, seed=7)
@@ -110,7 +135,9 @@ def main():
     pred = net.forward(xe)
     err = float(np.linalg.norm(pred - ye) / np.linalg.norm(ye))
     err_prev = float(np.linalg.norm(xe[:, 15:18] - ye) / np.linalg.norm(ye))
-    meta = dict(rows=int(len(x)), meshes_per_kind=a.meshes, steps=a.steps, epochs=a.epochs, batch=a.batch, lr=a.lr,
+    meta = dict(rows=int(len(x)), beams=a.beams, blocks=a.blocks, steps_per_mesh=a.steps,
+                beam_steps=a.beams * a.steps, block_steps=a.blocks * a.steps, keep=a.keep,
+                epochs=a.epochs, batch=a.batch, lr=a.lr,
                 final_batch_loss=loss, heldout_rel_err=err, heldout_prev_correction_rel_err=err_prev,
                 collect_s=t1 - t0, train_s=t2 - t1)
     save(a.out, net, mean, std, scale, meta)
