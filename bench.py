@@ -267,7 +267,17 @@ def learned_vs_plain(a, net_path):
     import numpy as np
     import train_warmstart as TW
     import xpbd_amd as X
-    s_plain, meshes, cfg, _ = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
+    det = 0 if a.nondeterministic else 1
+
+    def make(iters):
+        """A solver of the workload on the launch shape the other legs use (shared tuning cache)."""
+        s_, meshes_, cfg_, _ = build_workload(a.workload, a.solver, iters, a.variant, det)
+        if a.variant == "auto":
+            s_.autotune(a.tune_cache)
+        return s_, meshes_, cfg_
+
+    s_plain, meshes, cfg = make(a.iters)
+    launch_plain = list(s_plain.launch_config())
     rest = np.concatenate([m.x0 for m in meshes])
     s_plain.set_residual_policy(True)
     r_plain, t_plain = [], 0.0
@@ -283,6 +293,7 @@ def learned_vs_plain(a, net_path):
     zero = X.Network(np.zeros(X.MLP_PARAMS, np.float32))
     zero.set_scaling(np.zeros(24), np.ones(24), 1.0)
     out = {"iterations": a.learned_iters, "plain_iterations": cfg["iterations"], "fallback_ratio": 1.1,
+           "launch_config": launch_plain,
            "steps": a.learned_steps, "weights": os.path.relpath(net_path, ROOT),
            "plain_steps_per_s_with_residual_readback": a.learned_steps / t_plain,
            "plain_residual_median": float(np.median(r_plain))}
@@ -290,7 +301,7 @@ def learned_vs_plain(a, net_path):
                                          ("no_correction", zero, "multipliers", True),
                                          ("positions_start", net, "positions", True),
                                          ("positions_start_no_fallback", net, "positions", False)):
-        s, _, _, _ = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
+        s, _, _ = make(a.iters)
         s.attach(network)
         s.set_learned_start(start)
         ratios, fb, t = [], 0, 0.0
@@ -320,7 +331,7 @@ def learned_vs_plain(a, net_path):
             out["mlp_mfma_util"] = out["mlp_tflops"] / MFMA_BF16_PEAK_TFLOPS
         s.close()
     # the plain solver at the learned step's iteration count, for scale
-    s, _, _, _ = build_workload(a.workload, a.solver, a.learned_iters, a.variant, 0 if a.nondeterministic else 1)
+    s, _, _ = make(a.learned_iters)
     s.set_residual_policy(True)
     ratios, t = [], 0.0
     for n in range(a.learned_steps):
@@ -347,6 +358,10 @@ def main():
                                    "described in DESIGN.md but not built)"}))
         sys.exit(2)
     import xpbd_amd as X  # noqa: F401  (fails loudly if the HIP library is missing)
+    if a.tune_cache is None:
+        # one tuning per workload shape, shared by every leg of this run
+        import tempfile
+        a.tune_cache = os.path.join(tempfile.mkdtemp(prefix="xpbd_tune_"), "tune.txt")
     s, meshes, cfg, sdf = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
     cfg["variant"] = a.variant
     cfg["deterministic"] = not a.nondeterministic

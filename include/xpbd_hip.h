@@ -237,7 +237,8 @@ int xpbd_set_contacts(xpbd_handle* h, int32_t n, const int32_t* faces, const dou
  * visiting order, each pass followed by every contact when with_contacts;
  * initialize = 1 starts their multipliers (and the contacts') at zero first
  * (the projectors' initialize()).  Gauss-Seidel applies each projection at
- * once; Jacobi sums a pass from the same positions (float atomics).  The
+ * once; Jacobi sums a pass from the same positions, per vertex in a fixed
+ * order for the deterministic PATCH variant, with float atomics otherwise.  The
  * positions change in place; velocities are not updated (the next step's
  * velocity update sees the moved positions), and pinned vertices are not
  * reset. */

@@ -848,3 +848,32 @@ def test_subset_solve_matches_oracle(solver, variant, initialize):
     assert moved > 0 and np.any(o_lam != 0) and len(face) > 0
     assert err <= TOL
     assert lerr <= 1e-4
+
+
+def test_deterministic_jacobi_subset_solve_is_bitwise_reproducible():
+    """The deterministic patch Jacobi subset solve sums each vertex's
+    corrections in a fixed order: two handles give the same bits, whether
+    the vertex-element lists it gathers through were built at set-up (local
+    collision passes configured) or on first use."""
+    mesh, sph = _pressed_block(seed=32, depth=0.004)
+    rng = np.random.default_rng(34)
+    listed = np.concatenate([rng.choice(mesh.nt, 120, replace=False), rng.choice(mesh.nt, 40, replace=False)])
+    ref = X.Solver(mesh, dt=1e-3, iters=5, solver="jacobi", variant="patch", collision_every=1)
+    ref.set_sdf(*sph)
+    ref.step(1)
+    xs, vs = ref.positions(), ref.velocities()
+    ref.close()
+    out = []
+    for local in (0, 0, 2):
+        gpu = X.Solver(mesh, dt=1e-3, iters=5, solver="jacobi", variant="patch", collision_every=1,
+                       local_collision_iters=local)
+        gpu.set_sdf(*sph)
+        gpu.set_state(xs, vs)        # the same state for every handle (local passes would change a step)
+        gpu.detect()
+        x0 = gpu.positions()
+        gpu.solve_elements(listed, num_iter=4, initialize=True, with_contacts=True)
+        out.append((gpu.positions(), gpu.lambdas()))
+        assert np.max(np.abs(out[-1][0] - x0)) > 0
+        gpu.close()
+    for x, lam in out[1:]:
+        assert np.array_equal(x, out[0][0]) and np.array_equal(lam, out[0][1])

@@ -865,3 +865,82 @@ def test_subset_solve_is_the_reference_solve_on_a_list():
         e.set_state(xp, None)
         e.solve_elements(np.zeros(0, np.int32), num_iter=3, initialize=True)
         assert np.array_equal(e.x(), xp)
+
+
+# ---------------------------------------------------------------- learned-step starts
+def test_momentum_residual_of_one_projection_is_the_gradient_change():
+    """After one Gauss-Seidel projection of a single free element from lambda = 0
+    the XPBD update gives m (x - xt) = grad C(xt)^T lambda exactly, so the
+    momentum residual R = m (x - xt) - grad C(x)^T lambda must equal
+    (grad C(xt) - grad C(x))^T lambda: checked against the autograd gradients
+    of torch_ref at xt and at x (a wrong sign, a missing mass or a gradient
+    taken at the wrong point fails it)."""
+    import torch_ref
+    import xpbd_amd as X
+    x0 = np.array([[0.0, 0.0, 0.0], [0.01, 0.0, 0.0], [0.0, 0.01, 0.0], [0.0, 0.0, 0.01]])
+    v0 = np.array([[0.0, 0.0, 0.0], [0.8, 0.0, 0.0], [0.0, -0.5, 0.0], [0.3, 0.2, 0.6]])
+    mesh = X.Mesh(x0=x0, tets=np.array([[0, 1, 2, 3]], np.int32), E=np.array([3e3]), nu=np.array([0.45]),
+                  rho=np.array([1000.0]), v0=v0)
+    s = orc.Sim(mesh, dt=1e-3, g=0.0, iters=1, mode="gs")
+    xt = x0 + 1e-3 * v0
+    s.step(1)
+    x, lam = s.x(), s.lambdas()
+    _, _, gam = s.compliance()
+    Q = torch_ref.rest(x0)[0].numpy()
+    _, _, gD0, gH0 = torch_ref.grads(xt.reshape(12), Q, gam[0])
+    _, _, gD1, gH1 = torch_ref.grads(x.reshape(12), Q, gam[0])
+    R_exp = ((gD0 - gD1) * lam[0, 0] + (gH0 - gH1) * lam[0, 1]).reshape(4, 3)
+    n, R = s.momentum_residual(xt, vec=True)
+    inertia = np.abs(s.masses()[:, None] * (x - xt)).max()
+    assert np.abs(R_exp).max() > 1e-4 * inertia      # the gradients did change
+    np.testing.assert_allclose(R, R_exp, rtol=1e-6, atol=1e-9 * np.abs(R_exp).max())
+    assert n == pytest.approx(np.linalg.norm(R_exp), rel=1e-6)
+
+
+def test_multiplier_start_zeroes_the_residual_and_leaves_the_iterations_nothing_to_do():
+    """The learned step's multiplier start (orc_step_warm: lambda0 = -C / alpha~
+    at the start positions) makes every element's residual C + alpha~ lambda
+    vanish before the first iteration, so without contacts the iterations move
+    nothing: the residual after 0 and after 5 iterations is at rounding level
+    and the positions agree to rounding.  The momentum balance it starts from
+    is not the XPBD step's: from a block stretched 2% along its height its
+    momentum residual is over 10x the plain 20-iteration step's (44x here; the
+    elastic impulse of the state goes unanswered by any motion)."""
+    m = _mesh(4, E=3e3, nu=0.45)
+    x = m.x0 * np.array([1.0, 1.0, 1.02])         # the pinned base sits at z = 0
+    v = np.zeros_like(x)
+    sims = {k: orc.Sim(m, iters=20, mode="gs") for k in ("warm0", "warm5", "plain")}
+    for s in sims.values():
+        s.set_state(x, v)
+    xt = x.copy()
+    xt[:, 2] -= 9.81e-6
+    sims["warm0"].step_warm(None, 0)
+    sims["warm5"].step_warm(None, 5)
+    sims["plain"].step(1)
+    scale = np.sqrt(2 * m.nt) * np.sqrt(3)        # |C_D| = sqrt 3 at rest
+    assert sims["warm0"].last_step_residual() < 1e-12 * scale
+    assert sims["warm5"].last_step_residual() < 1e-12 * scale
+    np.testing.assert_allclose(sims["warm5"].x(), sims["warm0"].x(), rtol=0,
+                               atol=1e-12 * np.abs(m.x0).max())
+    assert sims["warm0"].momentum_residual(xt) > 10 * sims["plain"].momentum_residual(xt)
+
+
+def test_dual_start_without_multipliers_is_the_plain_step_and_rest_multipliers_balance():
+    """orc_step_dual_warm with no multipliers is orc_step bit for bit; with the
+    rest multipliers lambda = -C(X) / alpha~ (deviatoric and hydrostatic
+    balance each other at F = I) a mesh at rest without gravity starts exactly
+    where it is and stays there with zero residual."""
+    m = _mesh(3)
+    a, b = orc.Sim(m, iters=5, mode="gs"), orc.Sim(m, iters=5, mode="gs")
+    for _ in range(3):
+        a.step(1)
+        b.step_dual_warm(None, 5)
+        assert np.array_equal(a.x(), b.x()) and np.array_equal(a.v(), b.v())
+    s = orc.Sim(m, g=0.0, iters=5, mode="gs")
+    s.set_state(m.x0, np.zeros_like(m.x0))       # the mesh carries a seeded velocity
+    aD, aH, _ = s.compliance()
+    _, r = s.residual(vec=True)                   # C(X) at lambda = 0
+    lam_rest = np.stack([-r[:, 0] * 1e-6 / aD, -r[:, 1] * 1e-6 / aH], axis=1)   # alpha~ = alpha / dt^2
+    s.step_dual_warm(lam_rest, 5)
+    np.testing.assert_allclose(s.x(), m.x0, rtol=0, atol=1e-15)
+    assert s.last_step_residual() < 1e-12

@@ -702,6 +702,37 @@ static bool jacobi_fixed_order(const xpbd_handle* h)
            h->params.deterministic != 0 && h->d_vf_off != nullptr;
 }
 
+// The vertex -> solver positions CSR of the elements attached to each vertex
+// and the per-element correction slots, which the fixed-order Jacobi local
+// pass gathers from; built at set-up when local collision passes are
+// configured, else on first use.
+static int ensure_vertex_elements(xpbd_handle* h)
+{
+    const int nv = h->nv, nt = h->nt;
+    if (!h->d_vt_off) {
+        std::vector<int32_t> vt_off((size_t)nv + 1, 0), vt_pos(4 * (size_t)nt);
+        for (int e = 0; e < nt; e++)
+            for (int j = 0; j < 4; j++) vt_off[h->tets_host[4 * (size_t)e + j] + 1]++;
+        for (int v = 0; v < nv; v++) vt_off[v + 1] += vt_off[v];
+        std::vector<int32_t> fill(vt_off.begin(), vt_off.end() - 1);
+        for (int i = 0; i < nt; i++) {
+            const int e = h->order[i];
+            for (int j = 0; j < 4; j++) vt_pos[fill[h->tets_host[4 * (size_t)e + j]]++] = i;
+        }
+        int32_t *off = nullptr, *pos = nullptr;
+        int rc = dalloc(h, &off, vt_off.size());
+        if (!rc) rc = dalloc(h, &pos, vt_pos.size());
+        if (rc) return rc;
+        hipError_t e = hipMemcpy(off, vt_off.data(), sizeof(int32_t) * vt_off.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(pos, vt_pos.data(), sizeof(int32_t) * vt_pos.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_err(h, e, "vertex-element lists");
+        h->d_vt_off = off;
+        h->d_vt_pos = pos;
+    }
+    if (!h->d_edx) return dalloc(h, &h->d_edx, 12 * (size_t)nt);
+    return XPBD_OK;
+}
+
 int enqueue_contacts_jacobi(xpbd_handle* h, bool first)
 {
     hipError_t e;
@@ -1470,6 +1501,13 @@ extern "C" int xpbd_solve_elements(xpbd_handle* h, int32_t n, const int32_t* ele
     const bool con = with_contacts && contacts_enabled(h);
     const bool gs = h->params.solver == XPBD_SOLVER_GS_COLORED;
     const bool cluster = h->variant == XPBD_VARIANT_CLUSTER || h->variant == XPBD_VARIANT_CLUSTER_QUAD;
+    // deterministic patch Jacobi: each element's pass correction to its own
+    // slot, summed per vertex in a fixed order (as the local collision passes)
+    const bool fixed = !gs && h->variant == XPBD_VARIANT_PATCH && h->params.deterministic != 0;
+    if (fixed) {
+        int rc = ensure_vertex_elements(h);
+        if (rc) return rc;
+    }
     for (int it = 0; it < num_iter; it++) {
         const bool first = initialize && it == 0;  // contacts restart from zero multipliers
         if (gs) {
@@ -1482,7 +1520,13 @@ extern "C" int xpbd_solve_elements(xpbd_handle* h, int32_t n, const int32_t* ele
                                        h->d_rest, h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first);
             if (e != hipSuccess) return hip_err(h, e, "solve_elements launch");
         } else {
-            e = launch_jacobi_local(s, ta, h->d_mult, h->d_pos, h->d_acc, nt);
+            if (fixed) {
+                e = launch_jacobi_local_det(s, ta, h->d_mult, h->d_pos, h->d_edx, nt);
+                if (e == hipSuccess)
+                    e = launch_local_gather(s, ta, h->nv, h->d_vt_off, h->d_vt_pos, h->d_mult, h->d_edx, h->d_acc);
+            } else {
+                e = launch_jacobi_local(s, ta, h->d_mult, h->d_pos, h->d_acc, nt);
+            }
             if (e != hipSuccess) return hip_err(h, e, "solve_elements launch");
             if (con) {
                 int rc = enqueue_contacts_jacobi(h, first);
