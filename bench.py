@@ -165,9 +165,10 @@ def measure(s, cfg, workload, steps, warmup, timing_steps, no_graph=False, tune_
             "tet_projections_per_s": steps_per_s * iters * T, "steps": steps, "roofline": roof}
 
 
-def _oracle_step_loop(o, sdf, cfg, nsteps):
+def _oracle_step_loop(o, sdf, cfg, nsteps, step0=0):
+    """nsteps oracle steps; the kinematic collider is where the device has it at step step0 + n."""
     t0 = time.perf_counter()
-    for n in range(nsteps):
+    for n in range(step0, step0 + nsteps):
         if sdf is not None:
             kind, pos, quat, prm, vel = sdf
             o.set_sdf(kind, [pos[k] + vel[k] * n * cfg["dt"] for k in range(3)], quat, prm)
@@ -203,24 +204,32 @@ def cpu_baseline_batch(meshes, cfg, sdfs, nsteps, stride=16):
     return out
 
 
-def cpu_baseline(meshes, cfg, sdf, nsteps):
-    """FP64 oracle, one thread, on the first mesh of the workload (bounded sample)."""
+def _start_text(start):
+    return "from rest" if start is None else f"from the GPU run's state after its {start[2]} steps"
+
+
+def cpu_baseline(meshes, cfg, sdf, nsteps, start=None):
+    """FP64 oracle, one thread, on the first mesh of the workload (bounded
+    sample), from `start` = (positions, velocities, step count) of the GPU run
+    when given (the collider in contact, as in the timed steps), else from rest."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     if isinstance(sdf, list):
         return cpu_baseline_batch(meshes, cfg, sdf, nsteps)
     mesh = meshes[0]
     o = orc.Sim(mesh, dt=cfg["dt"], iters=cfg["iterations"], mode=cfg["solver"])
-    el = _oracle_step_loop(o, sdf, cfg, nsteps)
+    if start is not None:
+        o.set_state(start[0], start[1])
+    el = _oracle_step_loop(o, sdf, cfg, nsteps, 0 if start is None else start[2])
     out = {"value": nsteps / el, "unit": "steps/s", "cores": 1, "kind": "port",
-           "sample": f"{nsteps} steps of the same workload from rest ({mesh.nt} tets, {cfg['iterations']} "
+           "sample": f"{nsteps} steps of the same workload {_start_text(start)} ({mesh.nt} tets, {cfg['iterations']} "
                      f"{cfg['solver']} iterations) on one host core, FP64 oracle",
            "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el}
     out.update(host_info())
     return out
 
 
-def cpu_baseline_parallel(meshes, cfg, sdf, nsteps):
+def cpu_baseline_parallel(meshes, cfg, sdf, nsteps, start=None):
     """FP64 oracle on all host cores (OpenMP): colour-ordered Gauss-Seidel over
     clusters of consecutive elements (one hex cell each, 8 colours on the block;
     an order of this build, not the reference's element order), or Jacobi with
@@ -232,11 +241,13 @@ def cpu_baseline_parallel(meshes, cfg, sdf, nsteps):
     o = orc.Sim(mesh, dt=cfg["dt"], iters=cfg["iterations"], mode=cfg["solver"])
     ncol = o.color_elements(cluster=True) if cfg["solver"] == "gs" else 0
     o.set_threads(threads)
-    el = _oracle_step_loop(o, sdf, cfg, nsteps)
+    if start is not None:
+        o.set_state(start[0], start[1])
+    el = _oracle_step_loop(o, sdf, cfg, nsteps, 0 if start is None else start[2])
     how = (f"colour-ordered GS, {ncol} colours of element clusters" if cfg["solver"] == "gs"
            else "Jacobi, projections in parallel")
     out = {"value": nsteps / el, "unit": "steps/s", "cores": threads, "kind": "port",
-           "sample": f"{nsteps} steps of the same workload from rest ({mesh.nt} tets, {cfg['iterations']} "
+           "sample": f"{nsteps} steps of the same workload {_start_text(start)} ({mesh.nt} tets, {cfg['iterations']} "
                      f"{cfg['solver']} iterations), FP64 oracle, {how}, {threads} OpenMP threads",
            "tet_proj_per_s": nsteps * cfg["iterations"] * mesh.nt / el}
     out.update(host_info())
@@ -367,6 +378,8 @@ def main():
     cfg["deterministic"] = not a.nondeterministic
     m = measure(s, cfg, a.workload, a.steps, a.warmup, a.timing_steps, a.no_graph, a.tune_cache, a.variant)
     res = s.residual()
+    # the CPU baselines resume from where the GPU run ended (one object; the batch samples from rest)
+    start = None if a.workload == "batch" else (s.positions(), s.velocities(), a.warmup + a.steps + a.timing_steps)
     s.close()
     out = {
         "metric": METRIC,
@@ -397,10 +410,10 @@ def main():
         out["learned"] = learned_vs_plain(a, a.weights)
     if not a.no_cpu:
         n = a.cpu_steps if a.cpu_steps is not None else {"block": 10, "batch": 40}.get(a.workload, 20)
-        out["cpu_baseline"] = cpu_baseline(meshes, cfg, sdf, n)
+        out["cpu_baseline"] = cpu_baseline(meshes, cfg, sdf, n, start)
         if a.workload != "batch":
             # second reported baseline: the same oracle on all host cores
-            out["cpu_baseline_parallel"] = cpu_baseline_parallel(meshes, cfg, sdf, 4 * n)
+            out["cpu_baseline_parallel"] = cpu_baseline_parallel(meshes, cfg, sdf, 4 * n, start)
     print(json.dumps(out))
 
 
