@@ -100,6 +100,13 @@ PatchArrays patch_arrays(xpbd_handle* h)
 // ---------------------------------------------------------------------------
 // set-up
 // ---------------------------------------------------------------------------
+// Waits for the stream when it goes out of scope: host vectors that feed
+// asynchronous copies stay alive until the copies have run, on every return path.
+struct StreamDrain {
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
 static bool finite_all(const double* p, size_t n)
 {
     for (size_t i = 0; i < n; i++)
@@ -1440,6 +1447,7 @@ extern "C" int xpbd_set_contacts(xpbd_handle* h, int32_t n, const int32_t* faces
     std::vector<int32_t> slot(nf), flag(std::max(nf, 1), 0);
     for (int j = 0; j < nf; j++) slot[h->fperm[j]] = j;
     std::vector<double> cp(3 * (size_t)std::max(nf, 1), 0.0), cn(cp), cb(cp);
+    const StreamDrain drain{ h->stream };  // declared after the vectors: runs before they are freed
     for (int c = 0; c < n; c++) {
         const int f = faces[c];
         if (f < 0 || f >= nf) return set_err(h, XPBD_E_INVALID_ARG, "set_contacts: face index out of range");
@@ -1489,6 +1497,7 @@ extern "C" int xpbd_solve_elements(xpbd_handle* h, int32_t n, const int32_t* ele
     }
     hipStream_t s = h->stream;
     hipError_t e = hipSuccess;
+    const StreamDrain drain{ s };  // mult feeds an asynchronous copy
     if (!h->d_mult) {
         if (hipMalloc(reinterpret_cast<void**>(&h->d_mult), sizeof(int32_t) * (size_t)std::max(nt, 1)) != hipSuccess)
             return set_err(h, XPBD_E_NOMEM, "solve_elements: count buffer allocation failed");

@@ -1299,6 +1299,16 @@ static int launch_learned_attempt(xpbd_handle* h, int32_t iters, bool det)
         e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return hip_err(h, e, "hipGraphInstantiate");
+        // graphs of other network versions are unlikely to be replayed again
+        for (size_t i = 0; i < L->graphs.size();) {
+            if (L->graphs[i].net != net) {
+                (void)hipGraphExecDestroy(L->graphs[i].exec);
+                L->graphs[i] = L->graphs.back();
+                L->graphs.pop_back();
+            } else {
+                i++;
+            }
+        }
         L->graphs.push_back({ net, h->graph_gen, (int)det, iters, L->start, exec });
     }
     hipError_t e = hipGraphLaunch(exec, h->stream);
