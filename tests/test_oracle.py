@@ -808,7 +808,8 @@ def test_reference_jacobi_diverges_on_unstructured_meshes():
 
 
 @pytest.mark.parametrize("name,worst", [("Prostate.msh", 1e3), ("tissue_block1x1x0.1_4subdiv.msh", 500),
-                                        ("neuroma_segmentation_decimated_2000.msh", 500), ("2x1x1_block16.msh", 500)])
+                                        ("neuroma_segmentation_decimated_2000.msh", 500), ("2x1x1_block16.msh", 500),
+                                        ("tracheal_tumor.msh", 500), ("Trachea_processed.msh", 500)])
 def test_sliver_shape_factor_of_the_tissue_meshes(name, worst):
     """How far an element amplifies displacement rounding into the strain:
     |Q|_F times its longest edge (about 2 for a regular tet, unbounded for a
