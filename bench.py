@@ -371,8 +371,12 @@ def main():
     import xpbd_amd as X  # noqa: F401  (fails loudly if the HIP library is missing)
     if a.tune_cache is None:
         # one tuning per workload shape, shared by every leg of this run
+        import atexit
+        import shutil
         import tempfile
-        a.tune_cache = os.path.join(tempfile.mkdtemp(prefix="xpbd_tune_"), "tune.txt")
+        tmp = tempfile.mkdtemp(prefix="xpbd_tune_")
+        atexit.register(shutil.rmtree, tmp, True)
+        a.tune_cache = os.path.join(tmp, "tune.txt")
     s, meshes, cfg, sdf = build_workload(a.workload, a.solver, a.iters, a.variant, 0 if a.nondeterministic else 1)
     cfg["variant"] = a.variant
     cfg["deterministic"] = not a.nondeterministic
