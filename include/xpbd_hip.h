@@ -91,8 +91,8 @@ typedef struct xpbd_params {
     int32_t iters;            /* solver iterations per step */
     int32_t solver;           /* XPBD_SOLVER_* */
     int32_t deterministic;    /* Jacobi with the PATCH variant: 1 = every correction (elements,
-                               * contacts, attachments, local passes) summed in an order fixed
-                               * at set-up, bitwise reproducible; 0 = LDS / global float
+                               * contacts, attachments, local passes, subset solves) summed in
+                               * an order fixed at set-up, bitwise reproducible; 0 = LDS / global float
                                * atomics.  The SIMPLE variant always uses float atomics. */
     int32_t collision_every;  /* re-detect contacts every k steps; 0 = never */
     int32_t variant;          /* XPBD_VARIANT_* */
@@ -400,7 +400,12 @@ int xpbd_attach_mlp(xpbd_handle* h, xpbd_mlp* m);
  * the end of the solve) exceeds fallback_ratio * ref_residual, the step is
  * redone from the saved start-of-step state with the handle's full iteration
  * count, which reproduces the plain step exactly; *fell_back reports it and
- * *residual the final ||r||_2.  ref_residual <= 0 disables the fallback. */
+ * *residual the final ||r||_2.  ref_residual <= 0 disables the fallback.
+ * Everything up to the fallback decision (and the rest of the step, which a
+ * fallback redoes) replays one captured graph per network, iteration count,
+ * start mode and detection; in kernel timing mode the step launches eagerly
+ * and times the network with events (xpbd_learned_stats).  The same bits
+ * either way. */
 int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_residual, double fallback_ratio,
                       int32_t* fell_back, double* residual);
 
