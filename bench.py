@@ -11,8 +11,9 @@ replayed from a HIP graph.
 
 Extra keys of the same line, each timed the same way with its own roofline:
 `beam_gs` and `beam_jacobi` (200x20x20-cell beam, 480,000 tets, 10
-iterations) and `batch` (256 cubes of 4..24 cells per axis, 10 GS iterations,
-a box collider each).
+iterations), `batch` (256 cubes of 4..24 cells per axis, 10 GS iterations,
+a box collider each) and `block_large` (the tissue block at 120^3 cells,
+10,368,000 tets, 20 GS iterations, the same sphere).
 
 `roofline` covers the element-projection kernel: algorithmic bytes per solver
 iteration (80 B/tet + 28 B/vertex) divided by the summed projection-kernel
@@ -46,7 +47,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="block", choices=["block", "beam", "cube", "batch"])
+    ap.add_argument("--workload", default="block", choices=["block", "block_large", "beam", "cube", "batch"])
     ap.add_argument("--solver", default=None, choices=["gs", "jacobi"])
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--variant", default="auto", choices=["auto", "simple", "cluster", "patch", "quad"])
@@ -91,6 +92,17 @@ def build_workload(name, solver, iters, variant="auto", det=1):
         cfg = {"workload": "tissue block 56^3 cells x6 tets, heterogeneous E/nu, sphere contact",
                "tets": mesh.nt, "vertices": mesh.nv, "solver": solver or "gs", "iterations": iters or 20,
                "dt": 1e-3}
+        return s, [mesh], cfg, (kind, pos, quat, prm, vel)
+    if name == "block_large":
+        n = 120
+        mesh = W.block(n)
+        s = X.Solver(mesh, dt=1e-3, iters=iters or 20, solver=solver or "gs", collision_every=1, **kw)
+        kind, pos, quat, prm, vel = W.block_sphere(n)
+        s.set_sdf(kind, pos, quat, prm)
+        s.set_sdf_velocity(vel)
+        cfg = {"workload": "tissue block 120^3 cells x6 tets (ten times the 56^3 block), heterogeneous E/nu, "
+                           "sphere contact", "tets": mesh.nt, "vertices": mesh.nv, "solver": solver or "gs",
+               "iterations": iters or 20, "dt": 1e-3}
         return s, [mesh], cfg, (kind, pos, quat, prm, vel)
     if name == "beam":
         mesh = W.beam(200, 20, 20)
@@ -404,7 +416,8 @@ def main():
         "roofline": m["roofline"],
     }
     if a.extra_steps > 0 and a.workload == "block":
-        for key, wl, solver in (("beam_gs", "beam", "gs"), ("beam_jacobi", "beam", "jacobi"), ("batch", "batch", "gs")):
+        for key, wl, solver in (("beam_gs", "beam", "gs"), ("beam_jacobi", "beam", "jacobi"), ("batch", "batch", "gs"),
+                                ("block_large", "block_large", "gs")):
             se, _, ce, _ = build_workload(wl, solver, None, "auto", 1)
             me = measure(se, ce, wl, a.extra_steps, 10, a.timing_steps, tune_cache=a.tune_cache)
             se.close()

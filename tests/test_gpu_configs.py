@@ -126,6 +126,28 @@ def test_c3_block_sphere_per_step_from_shared_state():
     assert max(devs) <= 1e-12
 
 
+def test_large_block_ten_times_c3_per_step_from_shared_state():
+    """Far past the configurations: a 120^3-cell block (10,368,000 tets,
+    1,771,561 vertices, ten times C3) with C3's materials, the sphere pressed
+    1 cm into its top face, 20 GS iterations; three steps, each from the
+    shared state, against the oracle, with detection compared."""
+    n = 120
+    mesh = W.block(n)
+    gpu = X.Solver(mesh, dt=DT, iters=20, solver="gs", collision_every=1)
+    kind, pos, quat, prm, vel = W.block_sphere(n)
+    sdf = (kind, (pos[0], pos[1], pos[2] - 0.01), quat, prm, vel)
+    gpu.set_sdf(*sdf[:4])
+    gpu.set_sdf_velocity(vel)
+    o = _oracle(gpu, mesh, 20, "gs")
+    errs, derrs, ncon, devs = _shared_state_run(gpu, o, mesh, 3, sdf=sdf)
+    print(f"large block ({mesh.nt} tets, {mesh.nv} vertices, {gpu.launch_config()}), {len(errs)} steps: rel err max "
+          f"{errs.max():.3e}, displacement-relative max {derrs.max():.3e}; contacts per step up to {max(ncon)}, "
+          f"detection equal on every step, largest deviation {max(devs):.3e}")
+    assert min(ncon) > 0
+    assert errs.max() <= TOL
+    assert max(devs) <= 1e-12
+
+
 @pytest.mark.parametrize("depth", [0.002, 0.01, 0.025])
 def test_c3_sphere_over_grid_detection_equals_oracle(depth):
     """The C3 sphere centred over the block's grid of top faces (where
