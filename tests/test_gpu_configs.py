@@ -148,6 +148,41 @@ def test_large_block_ten_times_c3_per_step_from_shared_state():
     assert max(devs) <= 1e-12
 
 
+@pytest.mark.parametrize("solver,E_range,b", [("gs", (1e3, 5e4), 2e5), ("gs", (1e3, 5e3), 2e4),
+                                              ("jacobi", (1e3, 3e3), 2e6)])
+def test_first_order_with_contacts_per_step_from_shared_state(solver, E_range, b):
+    """First-order XPBD with sphere contacts and two local collision passes,
+    per step from the shared state, contacts handed over and detection
+    compared, 30 steps of a 12^3 heterogeneous block pressed by a sphere.
+    Whole trajectories cannot be compared here, and neither can single steps
+    at low damping: there the FP64 oracle alone amplifies a state perturbation
+    1e4x to 3e8x in one step (b = 200..2e4 on E up to 5e4, b = 2e3 on E up to
+    5e3, Jacobi up to b = 2e5).  These damping multipliers are where one step
+    amplifies it ~3.5x, which the gate needs."""
+    n = 12
+    rng = np.random.default_rng(41)
+    E, nu = W.heterogeneous(6 * n ** 3, rng, E_range=E_range)
+    mesh = W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.0, faces=True, rng=rng)
+    L = 0.01 * n
+    # the sphere keeps pressing: a first-order block yields at once, so a slow probe loses contact
+    sdf = (X.SDF_SPHERE, (0.47 * L, 0.53 * L, L + 0.03 - 0.002), (0.0, 0.0, 0.0, 1.0), (0.03, 0.0, 0.0),
+           (0.0, 0.0, -0.3))
+    gpu = X.Solver(mesh, dt=DT, iters=10, solver=solver, collision_every=1, first_order=True, damping_multiplier=b,
+                   local_collision_iters=2)
+    gpu.set_sdf(*sdf[:4])
+    gpu.set_sdf_velocity(sdf[4])
+    o = _oracle(gpu, mesh, 10, solver)
+    o.set_first_order(b, False)
+    o.set_local_collision_iters(2)
+    errs, derrs, ncon, devs = _shared_state_run(gpu, o, mesh, 30, sdf=sdf)
+    touching = sum(k > 0 for k in ncon)
+    print(f"first order {solver}, E {E_range}, b = {b:g}, contacts and 2 local passes: {len(errs)} steps, rel err "
+          f"max {errs.max():.3e}, displacement-relative max {derrs.max():.3e}; contacts on {touching} steps, up to "
+          f"{max(ncon)}, detection equal on every step")
+    assert len(errs) == 30 and touching >= 10
+    assert errs.max() <= TOL
+
+
 @pytest.mark.parametrize("depth", [0.002, 0.01, 0.025])
 def test_c3_sphere_over_grid_detection_equals_oracle(depth):
     """The C3 sphere centred over the block's grid of top faces (where
