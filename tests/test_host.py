@@ -145,8 +145,28 @@ def test_create_rejects_bad_arguments_before_touching_the_device():
     assert rc in (-4, -2)       # zero volume (or no device here)
 
 
+def test_empty_inputs_are_rejected_before_touching_the_device():
+    """A mesh without elements or vertices, and a batch of no meshes, are
+    argument errors, not empty solvers."""
+    m = _small()
+    empty = X.Mesh(x0=np.zeros((0, 3)), tets=np.zeros((0, 4), np.int32), E=np.zeros(0), nu=np.zeros(0),
+                   rho=np.zeros(0))
+    assert _create(empty)[0] == -1
+    no_tets = _small()
+    no_tets.tets = np.zeros((0, 4), np.int32)
+    no_tets.E, no_tets.nu, no_tets.rho = np.zeros(0), np.zeros(0), np.zeros(0)
+    assert _create(no_tets)[0] == -1
+    p = X.make_params()
+    h = C.c_void_p()
+    assert X.lib.xpbd_batch_create(0, C.byref(_desc(m)), C.byref(p), C.byref(h)) == -1
+    assert b"n > 0" in X.lib.xpbd_last_error(None)
+
+
 def test_null_handles_are_rejected():
     assert X.lib.xpbd_step(None, 1) == -1
+    assert X.lib.xpbd_set_contacts(None, 0, None, None, None, None) == -1
+    assert X.lib.xpbd_solve_elements(None, 0, None, 1, 1, 0) == -1
+    assert X.lib.xpbd_step_learned(None, 1, 0.0, 1.1, None, None) == -1
     assert X.lib.xpbd_destroy(None) == 0
     assert X.lib.xpbd_last_error(None) is not None
 
