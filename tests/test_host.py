@@ -197,6 +197,37 @@ def test_reference_tissue_meshes_read(name, nv, nt, nf, tmp_path):
     assert sum(tuple(sorted(tri)) in on_surface for tri in f.tolist()) >= 0.99 * len(f)
 
 
+def test_truncated_and_corrupt_msh_files_fail_cleanly(tmp_path):
+    """A file cut anywhere either reads as a valid prefix (every element index
+    in range) or raises; counts that claim more than the header declares, or
+    more than the file holds, raise instead of allocating or looping on them."""
+    src = open(os.path.join(GOLDEN, "msh", "cube4.msh"), "rb").read()
+    rng = np.random.default_rng(5)
+    p = tmp_path / "cut.msh"
+    for cut in sorted(set(rng.integers(0, len(src), 40).tolist()) | {len(src) // 2, len(src) - 1}):
+        p.write_bytes(src[:cut])
+        try:
+            x, t, _ = X.read_msh(str(p))
+        except X.XpbdError:
+            continue
+        assert t.size == 0 or (t.min() >= 0 and t.max() < len(x))
+    text = src.decode()
+    lines = text.split("\n")
+    k = lines.index("$Nodes")
+    header = lines[k + 1].split()
+    for bad in (f"{header[0]} 99999999999999 {header[2]} {header[3]}",    # more nodes than the file holds
+                f"{header[0]} -5 {header[2]} {header[3]}"):
+        q = lines[:k + 1] + [bad] + lines[k + 2:]
+        p.write_text("\n".join(q))
+        with pytest.raises(X.XpbdError):
+            X.read_msh(str(p))
+    blk = lines[k + 2].split()                    # first node block: dim, tag, parametric, count
+    q = lines[:k + 2] + [" ".join(blk[:3] + ["1000000000000"])] + lines[k + 3:]
+    p.write_text("\n".join(q))
+    with pytest.raises(X.XpbdError):
+        X.read_msh(str(p))
+
+
 def test_msh_without_nodes_section_is_refused(tmp_path):
     """A file with a $MeshFormat header but no $Nodes / $Elements sections (as
     resource/tissue/liver.msh, whose body is a bare node and element list) is

@@ -394,18 +394,24 @@ int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int3
             have_format = true;
         } else if (line.rfind("$Nodes", 0) == 0) {
             have_nodes = true;
-            long long nblocks, nnodes, mintag, maxtag;
+            long long nblocks = -1, nnodes = -1, mintag, maxtag;
             in >> nblocks >> nnodes >> mintag >> maxtag;
             if (!in || nblocks < 0 || nnodes < 0) { err = "malformed $Nodes header"; return -6; }
-            x.reserve(3 * (size_t)nnodes);
+            // the counts are the file's claim: reserve no more than a modest amount up front
+            x.reserve(3 * (size_t)std::min<long long>(nnodes, 1 << 20));
+            long long left = nnodes;  // a block may not hold more nodes than the header declares
             for (long long b = 0; b < nblocks; b++) {
-                int edim, etag, param; long long n;
+                int edim = 0, etag = 0, param = 0; long long n = -1;
                 in >> edim >> etag >> param >> n;
+                if (!in || n < 0 || n > left || edim < 0 || edim > 3) { err = "malformed $Nodes block header"; return -6; }
+                left -= n;
                 std::vector<long long> tags(n);
                 for (long long i = 0; i < n; i++) in >> tags[i];
+                if (!in) { err = "malformed $Nodes block"; return -6; }
                 for (long long i = 0; i < n; i++) {
                     double p[3];
                     in >> p[0] >> p[1] >> p[2];
+                    if (!in) { err = "malformed $Nodes block"; return -6; }
                     if (param) {  // skip parametric coordinates
                         for (int d = 0; d < edim; d++) { double u; in >> u; }
                     }
@@ -414,14 +420,18 @@ int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int3
                 }
                 if (!in) { err = "malformed $Nodes block"; return -6; }
             }
+            if (left != 0) { err = "$Nodes blocks do not add up to the header's node count"; return -6; }
         } else if (line.rfind("$Elements", 0) == 0) {
             have_elements = true;
-            long long nblocks, nel, mintag, maxtag;
+            long long nblocks = -1, nel = -1, mintag, maxtag;
             in >> nblocks >> nel >> mintag >> maxtag;
             if (!in || nblocks < 0 || nel < 0) { err = "malformed $Elements header"; return -6; }
+            long long left = nel;  // a block may not hold more elements than the header declares
             for (long long b = 0; b < nblocks; b++) {
-                int edim, etag, etype; long long n;
+                int edim = 0, etag = 0, etype = 0; long long n = -1;
                 in >> edim >> etag >> etype >> n;
+                if (!in || n < 0 || n > left) { err = "malformed $Elements block header"; return -6; }
+                left -= n;
                 const int nn = etype == 4 ? 4 : etype == 2 ? 3 : etype == 1 ? 2 : etype == 15 ? 1 :
                                etype == 11 ? 10 : etype == 9 ? 6 : etype == 5 ? 8 : -1;
                 if (nn < 0) { err = "unsupported element type " + std::to_string(etype); return -3; }
@@ -429,6 +439,7 @@ int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int3
                     long long etagv; in >> etagv;
                     long long nt[10];
                     for (int j = 0; j < nn; j++) in >> nt[j];
+                    if (!in) { err = "malformed $Elements block"; return -6; }
                     if (etype == 4 || etype == 2) {
                         auto& dst = etype == 4 ? tets : tris;
                         for (int j = 0; j < nn; j++) {
@@ -440,6 +451,7 @@ int read_msh41(const std::string& path, std::vector<double>& x, std::vector<int3
                 }
                 if (!in) { err = "malformed $Elements block"; return -6; }
             }
+            if (left != 0) { err = "$Elements blocks do not add up to the header's element count"; return -6; }
         }
     }
     if (!have_format) { err = "missing $MeshFormat"; return -5; }
