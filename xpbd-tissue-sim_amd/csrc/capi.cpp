@@ -54,7 +54,7 @@ static int dalloc(xpbd_handle* h, T** p, size_t n)
 static void release_graphs(xpbd_handle* h)
 {
     for (int i = 0; i < 2; i++) {
-        if (h->graph_exec[i]) hipGraphExecDestroy(h->graph_exec[i]);
+        if (h->graph_exec[i]) (void)hipGraphExecDestroy(h->graph_exec[i]);
         h->graph_exec[i] = nullptr;
     }
     h->graph_gen++;
@@ -647,14 +647,18 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     }
 #undef UP
     // zero-initialised device state (fill kernels, then wait)
-    launch_fill_zero(h->stream, h->d_acc, sizeof(float) * 4 * (size_t)nv);
-    launch_fill_zero(h->stream, h->d_lo, sizeof(float4) * (size_t)nv);
-    launch_fill_zero(h->stream, h->d_step_res, sizeof(double));
-    launch_fill_zero(h->stream, h->d_lam, sizeof(float2) * nt);
-    launch_fill_zero(h->stream, h->d_step, sizeof(unsigned long long));
-    launch_fill_zero(h->stream, h->d_cpos, sizeof(int32_t) * ((size_t)nf + 1));
-    launch_fill_zero(h->stream, h->d_clam, sizeof(double) * (nf > 0 ? nf : 1));
-    he = hipStreamSynchronize(h->stream);
+    hipError_t fe = hipSuccess;
+    auto zero = [&](void* p, size_t bytes) {
+        if (fe == hipSuccess) fe = launch_fill_zero(h->stream, p, bytes);
+    };
+    zero(h->d_acc, sizeof(float) * 4 * (size_t)nv);
+    zero(h->d_lo, sizeof(float4) * (size_t)nv);
+    zero(h->d_step_res, sizeof(double));
+    zero(h->d_lam, sizeof(float2) * nt);
+    zero(h->d_step, sizeof(unsigned long long));
+    zero(h->d_cpos, sizeof(int32_t) * ((size_t)nf + 1));
+    zero(h->d_clam, sizeof(double) * (nf > 0 ? nf : 1));
+    he = fe != hipSuccess ? fe : hipStreamSynchronize(h->stream);
     if (he != hipSuccess) {
         int r = hip_err(nullptr, he, "initialisation");
         xpbd_destroy(h);
@@ -678,15 +682,15 @@ extern "C" int xpbd_batch_create(int32_t n, const xpbd_mesh_desc* meshes, const 
 extern "C" int xpbd_destroy(xpbd_handle* h)
 {
     if (!h) return XPBD_OK;
-    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
     release_graphs(h);
     for (auto& ev : h->events) {
-        hipEventDestroy(ev.first);
-        hipEventDestroy(ev.second);
+        if (ev.first) (void)hipEventDestroy(ev.first);
+        if (ev.second) (void)hipEventDestroy(ev.second);
     }
-    for (void* p : h->allocs) hipFree(p);
+    for (void* p : h->allocs) (void)hipFree(p);
     destroy_learned(h);
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return XPBD_OK;
 }
@@ -968,20 +972,22 @@ static int enqueue_step(xpbd_handle* h, bool detect, bool timed)
     return enqueue_finish(h);
 }
 
+// Kernel timing spans.  An event that could not be created stays null and its
+// span is left out of the statistics.
 void timed_begin(xpbd_handle* h)
 {
     if (h->ev_used == h->events.size()) {
-        hipEvent_t a, b;
-        hipEventCreate(&a);
-        hipEventCreate(&b);
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess) a = nullptr;
+        if (hipEventCreate(&b) != hipSuccess) b = nullptr;
         h->events.push_back({ a, b });
     }
-    hipEventRecord(h->events[h->ev_used].first, h->stream);
+    if (h->events[h->ev_used].first) (void)hipEventRecord(h->events[h->ev_used].first, h->stream);
 }
 
 void timed_end(xpbd_handle* h)
 {
-    hipEventRecord(h->events[h->ev_used].second, h->stream);
+    if (h->events[h->ev_used].second) (void)hipEventRecord(h->events[h->ev_used].second, h->stream);
     h->ev_used++;
 }
 
@@ -989,12 +995,16 @@ int collect_timing(xpbd_handle* h)
 {
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_err(h, e, "step");
+    int64_t spans = 0;
     for (size_t i = 0; i < h->ev_used; i++) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, h->events[i].first, h->events[i].second);
+        if (!h->events[i].first || !h->events[i].second ||
+            hipEventElapsedTime(&ms, h->events[i].first, h->events[i].second) != hipSuccess)
+            continue;
         h->timed_ms += ms;
+        spans++;
     }
-    h->timed_launches += (int64_t)h->ev_used;
+    h->timed_launches += spans;
     h->ev_used = 0;
     return XPBD_OK;
 }
@@ -1022,12 +1032,12 @@ int run_steps(xpbd_handle* h, int32_t nsteps)
                 hipGraph_t g = nullptr;
                 e = hipStreamEndCapture(h->stream, &g);
                 if (rc) {
-                    if (g) hipGraphDestroy(g);
+                    if (g) (void)hipGraphDestroy(g);
                     return rc;
                 }
                 if (e != hipSuccess) return hip_err(h, e, "hipStreamEndCapture");
                 e = hipGraphInstantiate(&h->graph_exec[gi], g, nullptr, nullptr, 0);
-                hipGraphDestroy(g);
+                (void)hipGraphDestroy(g);
                 if (e != hipSuccess) return hip_err(h, e, "hipGraphInstantiate");
             }
             hipError_t e = hipGraphLaunch(h->graph_exec[gi], h->stream);
