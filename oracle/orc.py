@@ -80,6 +80,8 @@ _sig("orc_get_colors", [C.c_void_p, _ip, _ip, _ip], C.c_int)
 _sig("orc_set_order_levels", [C.c_void_p, _ip, C.c_int, _ip], C.c_int)
 _sig("orc_set_contacts", [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp])
 _sig("orc_momentum_residual", [C.c_void_p, _dp, _dp], C.c_double)
+_sig("orc_set_primary_residual", [C.c_void_p, C.c_int], None)
+_sig("orc_last_primary_residual", [C.c_void_p, _dp], C.c_double)
 _sig("orc_step_warm", [C.c_void_p, C.c_int, _dp, C.c_int])
 _sig("orc_solve_elements", [C.c_void_p, C.c_int, _ip, C.c_int, C.c_int, C.c_int])
 _sig("orc_set_lambdas", [C.c_void_p, _dp])
@@ -312,10 +314,23 @@ class Sim:
         lib.orc_step_warm(self.s, 1 if detect else 0, p(c), iters)
 
     def momentum_residual(self, xt, vec=False):
-        """||m (x - xt) - sum grad C^T lambda|| over free vertices after a step
-        whose inertial prediction was xt (V x 3)."""
+        """||(x - xt) / w - sum grad C^T lambda|| (elements and contacts) over
+        the vertices that are neither pinned nor attached, for the inertial
+        prediction xt (V x 3)."""
         out = np.empty((self.nv, 3)) if vec else None
         n = lib.orc_momentum_residual(self.s, p(_d(xt)), p(out))
+        return (n, out) if vec else n
+
+    def set_primary_residual(self, on=True):
+        """Record the primary residual at the end of every solve (as the
+        residual policy EVERY_SUBSTEP does)."""
+        lib.orc_set_primary_residual(self.s, int(bool(on)))
+
+    def last_primary_residual(self, vec=False):
+        """Norm (and, with vec, the V x 3 vector) of the primary residual
+        recorded at the end of the last solve."""
+        out = np.empty((self.nv, 3)) if vec else None
+        n = lib.orc_last_primary_residual(self.s, p(out))
         return (n, out) if vec else n
 
     def set_lambdas(self, lam):

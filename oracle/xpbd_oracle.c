@@ -444,6 +444,10 @@ typedef struct orc_sim {
     int* face_rank;        /* optional visiting rank of faces for contact projection */
     int* corder;           /* contact visiting order (indices into the contact list) */
     double last_res;       /* ||r||_2 at the end of the last solve (before pin reset) */
+    int want_pres;         /* record the primary residual at the end of every solve */
+    double* xt;            /* the step's inertial prediction (when want_pres) */
+    double* pres;          /* primary residual at the end of the last solve (3*nv, when want_pres) */
+    double last_pres;      /* its norm */
     double* jbuf;          /* Jacobi coordinate-update buffer (values) */
     int* jidx;             /* Jacobi coordinate-update buffer (coordinate index) */
     double ts;             /* compliance time scale: dt^2 (second order) or dt (first order) */
@@ -561,6 +565,12 @@ ORC_API void orc_set_first_order(orc_sim* s, double damping_multiplier, int adju
 
 ORC_API void orc_get_inverse_inertia(const orc_sim* s, double* out) { memcpy(out, s->w, sizeof(double) * s->nv); }
 
+/* the inertial prediction of the step, kept for the primary residual */
+static void keep_prediction(orc_sim* s)
+{
+    if (s->want_pres) memcpy(s->xt, s->x, sizeof(double) * 3 * (size_t)s->nv);
+}
+
 /* inertial move of one step, both orders (XPBDMeshObject.cpp:430, 446-470) */
 static void move_inertially(orc_sim* s)
 {
@@ -569,12 +579,13 @@ static void move_inertially(orc_sim* s)
     memcpy(s->xp, s->x, sizeof(double) * 3 * nv);
     if (s->first_order) {
         for (int i = 0; i < nv; i++) s->x[3 * i + 2] += s->dz1[i];
-        return;
+    } else {
+        for (int i = 0; i < nv; i++)
+            for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
+        const double dz = -s->g * dt * dt;
+        for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
     }
-    for (int i = 0; i < nv; i++)
-        for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
-    const double dz = -s->g * dt * dt;
-    for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
+    keep_prediction(s);
 }
 
 ORC_API void orc_destroy(orc_sim* s)
@@ -587,6 +598,7 @@ ORC_API void orc_destroy(orc_sim* s)
     free(s->face_rank); free(s->corder); free(s->dz1); free(s->E); free(s->nu); free(s->color_off); free(s->unit_off);
     free(s->att_v); free(s->att_p); free(s->att_lam);
     free(s->v2e_off); free(s->v2e);
+    free(s->xt); free(s->pres);
     free(s);
 }
 
@@ -1242,6 +1254,8 @@ ORC_API int orc_local_list_size(const orc_sim* s)
  *   XPBDMeshObject.cpp:507-535  v = (x - x_prev) / dt                       */
 ORC_API double orc_residual(const orc_sim* s, double* vec);
 
+static void record_primary(orc_sim* s);
+
 ORC_API void orc_step(orc_sim* s, int detect)
 {
     if (detect) orc_detect(s);
@@ -1255,7 +1269,7 @@ ORC_API void orc_step(orc_sim* s, int detect)
     /* EVERY_SUBSTEP residual is taken here, at the end of the global solve,
      * before the local collision iterations and the pin reset (XPBDSolver.hpp:91-96,
      * XPBDMeshObject.cpp:473-503) */
-    s->last_res = orc_residual(s, NULL);
+    s->last_res = orc_residual(s, NULL); record_primary(s);
     local_collision_iterations(s);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])
@@ -1277,12 +1291,13 @@ ORC_API void orc_step_corrected(orc_sim* s, int detect, const double* corr, int 
         for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
     const double dz = -s->g * dt * dt;
     for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
+    keep_prediction(s);
     if (corr)
         for (int i = 0; i < 3 * nv; i++) s->x[i] += corr[i];
     memset(s->lam, 0, sizeof(double) * 2 * (size_t)s->nt);
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
     for (int it = 0; it < iters; it++) iterate(s);
-    s->last_res = orc_residual(s, NULL);
+    s->last_res = orc_residual(s, NULL); record_primary(s);
     local_collision_iterations(s);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])
@@ -1307,6 +1322,7 @@ ORC_API void orc_step_dual_warm(orc_sim* s, int detect, const double* lam0, int 
         for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
     const double dz = -s->g * dt * dt;
     for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
+    keep_prediction(s);
     if (lam0) {
         memcpy(s->lam, lam0, sizeof(double) * 2 * (size_t)s->nt);
         double* dx = (double*)calloc(3 * (size_t)nv, sizeof(double));
@@ -1327,7 +1343,7 @@ ORC_API void orc_step_dual_warm(orc_sim* s, int detect, const double* lam0, int 
     }
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
     for (int it = 0; it < iters; it++) iterate(s);
-    s->last_res = orc_residual(s, NULL);
+    s->last_res = orc_residual(s, NULL); record_primary(s);
     local_collision_iterations(s);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])
@@ -1355,6 +1371,7 @@ ORC_API void orc_step_learned(orc_sim* s, int detect, const double* corr, int it
         for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
     const double dz = -s->g * dt * dt;
     for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
+    keep_prediction(s);
     double* dx = (double*)calloc(3 * (size_t)nv, sizeof(double));
     for (int e = 0; e < s->nt; e++) {
         const int* t = s->tets + 4 * e;
@@ -1373,7 +1390,7 @@ ORC_API void orc_step_learned(orc_sim* s, int detect, const double* corr, int it
     free(dx);
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
     for (int it = 0; it < iters; it++) iterate(s);
-    s->last_res = orc_residual(s, NULL);
+    s->last_res = orc_residual(s, NULL); record_primary(s);
     local_collision_iterations(s);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])
@@ -1423,13 +1440,23 @@ ORC_API double orc_residual(const orc_sim* s, double* vec)
     return sqrt(acc);
 }
 
-/* Momentum balance of the last solve (implicit Euler): for every free vertex i,
- *     R_i = m_i (x_i - xt_i) - sum_e grad_i C_e(x)^T lambda_e   (Dev and Hyd),
- * with xt the inertial prediction of the step (3*nv, given) and lambda the
+/* Primary residual (Equation (8) of the XPBD paper; the reference declares it
+ * as XPBDSolver::primaryResidual, XPBDSolver.hpp:127-130, and leaves
+ * _calculatePrimaryResidual empty, :322): for every free vertex i,
+ *     R_i = (x_i - xt_i) / w_i - sum_c grad_i C_c(x)^T lambda_c
+ * over the constraints c that hold multipliers: every element's deviatoric and
+ * hydrostatic constraint, every contact (grad = barycentric * normal,
+ * StaticDeformableCollisionConstraint.cpp:20-55).  xt is the
+ * step's inertial prediction (3*nv, given), w the inverse inertia the
+ * projections use (1/m, or the first-order damping's), lambda the
  * multipliers left by the solve (alpha~ lambda is the constraint force times
  * dt^2, XPBDSolver.hpp:66-97 with CombinedNeohookeanConstraintProjector.cpp:98-179).
- * Pinned vertices are left out (their constraint force is the support's).
- * Returns ||R||_2; vec (3*nv, optional) receives R (zero on pinned vertices).
+ * Pinned and attached vertices are left out: a pin, and an attachment with
+ * its zero compliance (AttachmentConstraint.hpp:33-88), hold the vertex where
+ * it must be, so their force is whatever the balance needs (the gradient
+ * (x - p) / |x - p| of a met attachment has no direction to test either).
+ * Returns ||R||_2; vec (3*nv, optional) receives R (zero on the vertices left
+ * out).
  * Gauss-Seidel XPBD keeps R = 0 only up to the change of the gradients along
  * the iterates; a start that sets lambda without moving x by M^-1 grad C^T
  * lambda violates it by the difference. */
@@ -1438,7 +1465,7 @@ ORC_API double orc_momentum_residual(const orc_sim* s, const double* xt, double*
     const int nv = s->nv;
     double* R = (double*)calloc(3 * (size_t)nv, sizeof(double));
     for (int i = 0; i < nv; i++)
-        for (int k = 0; k < 3; k++) R[3 * i + k] = s->m[i] * (s->x[3 * i + k] - xt[3 * i + k]);
+        for (int k = 0; k < 3; k++) R[3 * i + k] = (s->x[3 * i + k] - xt[3 * i + k]) / s->w[i];
     for (int e = 0; e < s->nt; e++) {
         const int* t = s->tets + 4 * e;
         double x12[12], C[2], g[24];
@@ -1449,6 +1476,13 @@ ORC_API double orc_momentum_residual(const orc_sim* s, const double* xt, double*
             for (int k = 0; k < 3; k++)
                 R[3 * t[j] + k] -= g[3 * j + k] * s->lam[2 * e] + g[12 + 3 * j + k] * s->lam[2 * e + 1];
     }
+    for (int c = 0; c < s->nc; c++) {
+        const int* f = s->faces + 3 * s->c_face[c];
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) R[3 * f[i] + k] -= s->c_bary[3 * c + i] * s->c_n[3 * c + k] * s->c_lam[c];
+    }
+    for (int a = 0; a < s->na; a++)
+        for (int k = 0; k < 3; k++) R[3 * s->att_v[a] + k] = 0;
     double acc = 0;
     for (int i = 0; i < nv; i++)
         for (int k = 0; k < 3; k++) {
@@ -1458,6 +1492,29 @@ ORC_API double orc_momentum_residual(const orc_sim* s, const double* xt, double*
     if (vec) memcpy(vec, R, sizeof(double) * 3 * (size_t)nv);
     free(R);
     return sqrt(acc);
+}
+
+/* Record the primary residual where the reference's EVERY_SUBSTEP policy
+ * evaluates its residuals: at the end of the global solve, before the local
+ * collision iterations and the pin reset (XPBDSolver.hpp:91-96). */
+static void record_primary(orc_sim* s)
+{
+    if (s->want_pres) s->last_pres = orc_momentum_residual(s, s->xt, s->pres);
+}
+
+ORC_API void orc_set_primary_residual(orc_sim* s, int on)
+{
+    s->want_pres = on != 0;
+    if (s->want_pres && !s->xt) {
+        s->xt = (double*)calloc(3 * (size_t)s->nv, sizeof(double));
+        s->pres = (double*)calloc(3 * (size_t)s->nv, sizeof(double));
+    }
+}
+
+ORC_API double orc_last_primary_residual(const orc_sim* s, double* vec)
+{
+    if (vec && s->pres) memcpy(vec, s->pres, sizeof(double) * 3 * (size_t)s->nv);
+    return s->last_pres;
 }
 
 /* One contact pass from the current positions, as in iterate(): Gauss-Seidel
@@ -1518,7 +1575,7 @@ ORC_API void orc_step_warm(orc_sim* s, int detect, const double* corr, int iters
     }
     for (int a = 0; a < s->na; a++) s->att_lam[a] = 0;
     for (int it = 0; it < iters; it++) iterate(s);
-    s->last_res = orc_residual(s, NULL);
+    s->last_res = orc_residual(s, NULL); record_primary(s);
     local_collision_iterations(s);
     for (int i = 0; i < nv; i++)
         if (s->pin[i])

@@ -947,6 +947,70 @@ def test_dual_start_without_multipliers_is_the_plain_step_and_rest_multipliers_b
     assert s.last_step_residual() < 1e-12
 
 
+def _falling_block_on_contacts(E, mode):
+    """Unpinned 3x3x3-cell block (seeded velocities) with its bottom faces
+    given contacts 0.2 mm above them (normal +z, centroid barycentrics) and
+    one top vertex attached 0.3 mm off; the primary residual is recorded."""
+    import workloads as W
+    m = W.grid_object(3, 3, 3, 0.01, E=E, pin_axis=None, faces=True, seed=11)
+    s = orc.Sim(m, iters=6, mode=mode)
+    s.set_primary_residual(True)
+    z0 = m.x0[:, 2].min()
+    face = np.where(np.all(m.x0[m.faces][:, :, 2] < z0 + 1e-9, axis=1))[0].astype(np.int32)
+    cp = m.x0[m.faces[face]].mean(axis=1) + np.array([0.0, 0.0, 2e-4])
+    cn = np.tile([0.0, 0.0, 1.0], (len(face), 1))
+    s.set_contacts(face, cp, cn, np.full((len(face), 3), 1.0 / 3.0))
+    top = int(np.argmax(m.x0[:, 2] + m.x0[:, 0]))
+    s.set_attachments([top], m.x0[[top]] + np.array([3e-4, 0.0, 0.0]))
+    xt = m.x0 + 1e-3 * m.v0
+    xt[:, 2] -= 9.81e-6
+    return m, s, face, top, xt
+
+
+@pytest.mark.parametrize("mode", ["gs", "jacobi"])
+def test_primary_residual_contact_impulses_balance_the_motion_of_a_compliant_block(mode):
+    """With elements compliant enough to carry no force (E = 1e-6 Pa) every
+    correction comes from the contacts (whose gradient barycentric * normal
+    does not change with x) and the zero-compliance attachment.  Each
+    correction is dx = w grad C dlambda, so the motion they cause is exactly
+    w * grad C^T lambda and the primary residual vanishes to rounding
+    (leaving the contact term out leaves (x - xt) / w; a flipped sign doubles
+    it).  The attached vertex is left out; the residual is recorded at the
+    end of the solve: without pins or local passes it is the one evaluated on
+    the step's end state."""
+    m, s, face, top, xt = _falling_block_on_contacts(1e-6, mode)
+    s.step(1)
+    n, R = s.last_primary_residual(vec=True)
+    w = s.inverse_inertia()
+    inertia = (s.x() - xt) / w[:, None]
+    bottom = np.unique(m.faces[face])
+    assert inertia[bottom, 2].min() > 0.02 * np.abs(inertia).max()           # lifted by the contacts
+    assert np.abs(inertia[top]).max() > 0.02 * np.abs(inertia).max()       # moved by the attachment
+    assert np.all(R[top] == 0)
+    assert n < 1e-9 * np.linalg.norm(inertia)
+    assert n == pytest.approx(np.linalg.norm(R), rel=1e-12)
+    n2, R2 = s.momentum_residual(xt, vec=True)
+    np.testing.assert_allclose(R2, R, rtol=0, atol=1e-9 * np.abs(inertia).max())
+
+
+def test_primary_residual_stiff_block_and_pins():
+    """At tissue stiffness the elements carry the load and the Gauss-Seidel
+    residual is the change of their gradients along the iterates: small
+    against the inertial term but not zero; pinned vertices read zero, and
+    the recording is switched off by default (zero norm)."""
+    m, s, face, top, xt = _falling_block_on_contacts(3e3, "gs")
+    s.step(1)
+    n, R = s.last_primary_residual(vec=True)
+    inertia = np.linalg.norm((s.x() - xt) / s.inverse_inertia()[:, None])
+    assert 0 < n < 0.2 * inertia
+    p = orc.Sim(_mesh(3), iters=6, mode="gs")
+    assert p.last_primary_residual() == 0.0
+    p.set_primary_residual(True)
+    p.step(2)
+    n, R = p.last_primary_residual(vec=True)
+    assert n > 0 and np.all(R[_mesh(3).pinned.astype(bool)] == 0)
+
+
 def test_even_the_exact_multipliers_save_few_iterations():
     """An upper bound on any warm start that keeps the momentum balance: from
     the exact multipliers of the plain 20-iteration solve, with positions
