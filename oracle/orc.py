@@ -82,6 +82,7 @@ _sig("orc_set_contacts", [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp])
 _sig("orc_momentum_residual", [C.c_void_p, _dp, _dp], C.c_double)
 _sig("orc_set_primary_residual", [C.c_void_p, C.c_int], None)
 _sig("orc_last_primary_residual", [C.c_void_p, _dp], C.c_double)
+_sig("orc_last_prediction", [C.c_void_p, _dp])
 _sig("orc_step_warm", [C.c_void_p, C.c_int, _dp, C.c_int])
 _sig("orc_solve_elements", [C.c_void_p, C.c_int, _ip, C.c_int, C.c_int, C.c_int])
 _sig("orc_set_lambdas", [C.c_void_p, _dp])
@@ -332,6 +333,13 @@ class Sim:
         out = np.empty((self.nv, 3)) if vec else None
         n = lib.orc_last_primary_residual(self.s, p(out))
         return (n, out) if vec else n
+
+    def last_prediction(self):
+        """The last step's inertial prediction xt (V x 3), kept while the
+        primary residual is recorded."""
+        out = np.zeros((self.nv, 3))
+        lib.orc_last_prediction(self.s, p(out))
+        return out
 
     def set_lambdas(self, lam):
         """Overwrite the element multipliers (T x 2: lambda_D, lambda_H)."""

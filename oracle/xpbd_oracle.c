@@ -1511,6 +1511,12 @@ ORC_API void orc_set_primary_residual(orc_sim* s, int on)
     }
 }
 
+/* the inertial prediction kept for the primary residual (3*nv) */
+ORC_API void orc_last_prediction(const orc_sim* s, double* out)
+{
+    if (s->xt) memcpy(out, s->xt, sizeof(double) * 3 * (size_t)s->nv);
+}
+
 ORC_API double orc_last_primary_residual(const orc_sim* s, double* vec)
 {
     if (vec && s->pres) memcpy(vec, s->pres, sizeof(double) * 3 * (size_t)s->nv);

@@ -980,6 +980,7 @@ def test_primary_residual_contact_impulses_balance_the_motion_of_a_compliant_blo
     the step's end state."""
     m, s, face, top, xt = _falling_block_on_contacts(1e-6, mode)
     s.step(1)
+    np.testing.assert_allclose(s.last_prediction(), xt, rtol=0, atol=1e-15)
     n, R = s.last_primary_residual(vec=True)
     w = s.inverse_inertia()
     inertia = (s.x() - xt) / w[:, None]

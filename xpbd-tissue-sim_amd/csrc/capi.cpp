@@ -658,6 +658,7 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     zero(h->d_step, sizeof(unsigned long long));
     zero(h->d_cpos, sizeof(int32_t) * ((size_t)nf + 1));
     zero(h->d_clam, sizeof(double) * (nf > 0 ? nf : 1));
+    zero(h->d_flag, sizeof(int32_t) * (nf > 0 ? nf : 1));   // no face in contact before the first detection
     he = fe != hipSuccess ? fe : hipStreamSynchronize(h->stream);
     if (he != hipSuccess) {
         int r = hip_err(nullptr, he, "initialisation");
@@ -710,14 +711,14 @@ namespace xpbd {
 static bool jacobi_fixed_order(const xpbd_handle* h)
 {
     return h->params.solver == XPBD_SOLVER_JACOBI && h->variant == XPBD_VARIANT_PATCH &&
-           h->params.deterministic != 0 && h->d_vf_off != nullptr;
+           h->params.deterministic != 0 && h->d_cdx != nullptr;
 }
 
 // The vertex -> solver positions CSR of the elements attached to each vertex
-// and the per-element correction slots, which the fixed-order Jacobi local
-// pass gathers from; built at set-up when local collision passes are
-// configured, else on first use.
-static int ensure_vertex_elements(xpbd_handle* h)
+// (the fixed-order Jacobi local pass and the primary residual gather over it)
+// and, with slots, the per-element correction slots of the local pass; built
+// at set-up when local collision passes are configured, else on first use.
+static int ensure_vertex_elements(xpbd_handle* h, bool slots)
 {
     const int nv = h->nv, nt = h->nt;
     if (!h->d_vt_off) {
@@ -740,7 +741,37 @@ static int ensure_vertex_elements(xpbd_handle* h)
         h->d_vt_off = off;
         h->d_vt_pos = pos;
     }
-    if (!h->d_edx) return dalloc(h, &h->d_edx, 12 * (size_t)nt);
+    if (slots && !h->d_edx) return dalloc(h, &h->d_edx, 12 * (size_t)nt);
+    return XPBD_OK;
+}
+
+// The vertex -> (face slot * 3 + corner) CSR over the contact faces, in
+// ascending face slot; built at set-up for Jacobi, else on first use (from
+// the device's face list).
+static int ensure_vertex_faces(xpbd_handle* h)
+{
+    const int nv = h->nv, nf = h->nf;
+    if (h->d_vf_off || nf <= 0) return XPBD_OK;
+    std::vector<int4> hf(nf);
+    hipError_t e = hipMemcpy(hf.data(), h->d_faces, sizeof(int4) * nf, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_err(h, e, "vertex-face lists");
+    std::vector<int32_t> vf_off((size_t)nv + 1, 0), vf(3 * (size_t)nf);
+    auto corner = [&](int j, int c) { return c == 0 ? hf[j].x : c == 1 ? hf[j].y : hf[j].z; };
+    for (int j = 0; j < nf; j++)
+        for (int c = 0; c < 3; c++) vf_off[corner(j, c) + 1]++;
+    for (int v = 0; v < nv; v++) vf_off[v + 1] += vf_off[v];
+    std::vector<int32_t> fill(vf_off.begin(), vf_off.end() - 1);
+    for (int j = 0; j < nf; j++)
+        for (int c = 0; c < 3; c++) vf[fill[corner(j, c)]++] = 3 * j + c;
+    int32_t *off = nullptr, *lst = nullptr;
+    int rc = dalloc(h, &off, vf_off.size());
+    if (!rc) rc = dalloc(h, &lst, vf.size());
+    if (rc) return rc;
+    e = hipMemcpy(off, vf_off.data(), sizeof(int32_t) * vf_off.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(lst, vf.data(), sizeof(int32_t) * vf.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_err(h, e, "vertex-face lists");
+    h->d_vf_off = off;
+    h->d_vf = lst;
     return XPBD_OK;
 }
 
@@ -951,6 +982,7 @@ int enqueue_predict(xpbd_handle* h)
     hipError_t e = h->params.first_order
                        ? launch_predict_first_order(h->stream, h->d_pos, h->d_dz1, h->nv)
                        : launch_predict(h->stream, h->d_pos, h->d_vel, h->nv, (float)dt, (float)(-h->params.g * dt * dt));
+    if (e == hipSuccess && h->residual_policy) e = launch_keep_prediction(h->stream, h->d_pos, h->d_pred, h->nv);
     return e == hipSuccess ? XPBD_OK : hip_err(h, e, "predict launch");
 }
 
@@ -1231,14 +1263,66 @@ int device_residual(xpbd_handle* h, double* norm, double* vec_solver_order_dev)
     *norm = std::sqrt(s);
     return XPBD_OK;
 }
-// ||r||^2 at the end of the solve, before the pin reset (where the reference's
-// EVERY_SUBSTEP policy evaluates it, XPBDSolver.hpp:91-96), into d_step_res.
+// ||r||^2 and the primary residual at the end of the solve, before the local
+// collision passes and the pin reset (where the reference's EVERY_SUBSTEP
+// policy evaluates both, XPBDSolver.hpp:91-96), into d_step_res, d_pres and
+// d_step_pres.
 int enqueue_step_residual(xpbd_handle* h)
 {
     hipError_t e = launch_residual(h->stream, tet_arrays(h), h->d_pos, h->d_lo, true, h->nt, h->d_part,
                                    h->red_blocks, nullptr);
     if (e == hipSuccess) e = launch_sum_partials(h->stream, h->d_part, h->red_blocks, h->d_step_res);
+    if (e == hipSuccess) {
+        PrimaryArgs A;
+        A.pos = h->d_pos;
+        A.lo = h->d_lo;
+        A.vel = h->d_vel;
+        A.pred = h->d_pred;
+        A.vt_off = h->d_vt_off;
+        A.vt_pos = h->d_vt_pos;
+        A.vf_off = h->nf > 0 ? h->d_vf_off : nullptr;
+        A.vf = h->d_vf;
+        A.flag = h->d_flag;
+        A.cn = h->d_cn;
+        A.cb = h->d_cb;
+        A.clam = h->d_clam;
+        A.att_v = h->d_att_v;
+        A.natt = h->natt;
+        A.nv = h->nv;
+        A.R = h->d_pres;
+        A.part = h->d_part;
+        A.nblocks = h->red_blocks;
+        e = launch_primary_residual(h->stream, tet_arrays(h), A, h->d_step_pres);
+    }
     return e == hipSuccess ? XPBD_OK : hip_err(h, e, "step residual");
+}
+
+// Buffers of the primary residual: the vertex lists it gathers over, the kept
+// prediction, the residual vector and its norm (zero until the first step).
+static int ensure_primary_residual(xpbd_handle* h)
+{
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_err(h, e, "residual policy");
+    int rc = ensure_vertex_elements(h, false);
+    if (!rc) rc = ensure_vertex_faces(h);
+    if (rc) return rc;
+    if (!h->d_pred) {
+        float4* pred = nullptr;
+        double *pres = nullptr, *norm = nullptr;
+        rc = dalloc(h, &pred, (size_t)h->nv);
+        if (!rc) rc = dalloc(h, &pres, 3 * (size_t)h->nv);
+        if (!rc) rc = dalloc(h, &norm, 1);
+        if (rc) return rc;
+        e = launch_fill_zero(h->stream, pred, sizeof(float4) * (size_t)h->nv);
+        if (e == hipSuccess) e = launch_fill_zero(h->stream, pres, sizeof(double) * 3 * (size_t)h->nv);
+        if (e == hipSuccess) e = launch_fill_zero(h->stream, norm, sizeof(double));
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return hip_err(h, e, "primary residual buffers");
+        h->d_pred = pred;
+        h->d_pres = pres;
+        h->d_step_pres = norm;
+    }
+    return XPBD_OK;
 }
 }  // namespace xpbd
 
@@ -1247,9 +1331,28 @@ extern "C" int xpbd_set_residual_policy(xpbd_handle* h, int32_t policy)
     if (!h) return XPBD_E_INVALID_ARG;
     if (policy != 0 && policy != 1) return set_err(h, XPBD_E_INVALID_ARG, "policy must be 0 or 1");
     if (policy != h->residual_policy) {
+        if (policy) {
+            const int rc = ensure_primary_residual(h);
+            if (rc) return rc;
+        }
         h->residual_policy = policy;
         invalidate_graphs(h);
     }
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_last_step_primary_residual(xpbd_handle* h, double* norm_out, double* vec_out)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (!norm_out) return set_err(h, XPBD_E_INVALID_ARG, "norm_out is NULL");
+    if (!h->residual_policy) return set_err(h, XPBD_E_STATE, "residual policy is off");
+    double s = 0;
+    hipError_t e = hipMemcpyAsync(&s, h->d_step_pres, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && vec_out)
+        e = hipMemcpyAsync(vec_out, h->d_pres, sizeof(double) * 3 * (size_t)h->nv, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_err(h, e, "last_step_primary_residual");
+    *norm_out = std::sqrt(s);
     return XPBD_OK;
 }
 
@@ -1524,7 +1627,7 @@ extern "C" int xpbd_solve_elements(xpbd_handle* h, int32_t n, const int32_t* ele
     // slot, summed per vertex in a fixed order (as the local collision passes)
     const bool fixed = !gs && h->variant == XPBD_VARIANT_PATCH && h->params.deterministic != 0;
     if (fixed) {
-        int rc = ensure_vertex_elements(h);
+        int rc = ensure_vertex_elements(h, true);
         if (rc) return rc;
     }
     for (int it = 0; it < num_iter; it++) {

@@ -120,6 +120,11 @@ struct xpbd_handle {
     int64_t host_step = 0;
     int residual_policy = 0;     // 1: record ||r|| at the end of every solve (reference EVERY_SUBSTEP)
     double* d_step_res = nullptr;
+    // primary residual under the policy: the prediction's in-step motion, the
+    // per-vertex residual (3 nv) and its squared norm
+    float4* d_pred = nullptr;
+    double* d_pres = nullptr;
+    double* d_step_pres = nullptr;
     hipGraphExec_t graph_exec[2] = { nullptr, nullptr };  // [0] plain step, [1] step with detection
     uint64_t graph_gen = 0;      // bumped whenever captured steps go stale (other graphs compare against it)
 

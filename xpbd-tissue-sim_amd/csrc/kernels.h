@@ -138,6 +138,35 @@ hipError_t launch_energy(hipStream_t s, const TetArrays& ta, const float4* pos, 
                          double* part, int nblocks);
 hipError_t launch_sum_partials(hipStream_t s, const double* part, int n, double* out);
 
+// Primary residual (momentum balance) of a solve, per vertex in FP64:
+//   R_v = (x_v - xt_v) / w_v - sum_e grad_v C_e^T lambda_e - sum_c b_vc n_c lambda_c
+// over the elements and active contacts around v; zero on pinned and attached
+// vertices.  pred[v] is the in-step motion the inertial prediction set.
+struct PrimaryArgs {
+    const float4* pos;
+    const float4* lo;
+    const float4* vel;    // .w != 0 marks pins
+    const float4* pred;
+    const int* vt_off;    // vertex -> solver positions of its elements
+    const int* vt_pos;
+    const int* vf_off;    // vertex -> face slot * 3 + corner (null without faces)
+    const int* vf;
+    const int* flag;      // per face slot: contact active
+    const double* cn;
+    const double* cb;
+    const double* clam;
+    const int* att_v;
+    int natt;
+    int nv;
+    double* R;            // 3 * nv
+    double* part;         // nblocks partial sums of R^2
+    int nblocks;
+};
+// keep the inertial prediction: pred[v] = in-step motion of v
+hipError_t launch_keep_prediction(hipStream_t s, const float4* pos, float4* pred, int nv);
+// R into A.R and ||R||^2 into *out
+hipError_t launch_primary_residual(hipStream_t s, const TetArrays& ta, const PrimaryArgs& A, double* out);
+
 // Contacts.  faces: (a, b, c, instance) in colour-sorted order.
 hipError_t launch_detect(hipStream_t s, const int4* faces, int nf, const double* rest, const float4* pos,
                          const float4* lo, const SdfDev* sdf, const unsigned long long* step, double dt, int* flag,

@@ -1032,6 +1032,100 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Primary residual: the momentum balance of Equation (8) of the XPBD paper,
+//   R_v = (x_v - xt_v) / w_v - sum grad_v C^T lambda,
+// which the reference declares (XPBDSolver.hpp:127-130) and leaves empty
+// (XPBDSolver.hpp:322).  The sum runs over both constraints of every element
+// around v and every active contact on a face around v (gradient barycentric *
+// normal, StaticDeformableCollisionConstraint.cpp:20-55).  x - xt is the
+// in-step motion minus the motion the inertial prediction set, so it is formed
+// exactly.  Pinned vertices read zero (their force is the support's), and so
+// do attached ones: a zero-compliance attachment holds its vertex wherever the
+// balance needs.  One thread per vertex walks its elements and faces in the
+// fixed order of its lists, so the result is reproducible.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_keep_prediction(const float4* __restrict__ pos, float4* __restrict__ pred,
+                                                           int nv)
+{
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < nv) pred[v] = pos[2 * (size_t)v + 1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_primary_residual(TetArrays ta, PrimaryArgs A)
+{
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= A.nv) return;
+    const float4 b = A.pos[2 * (size_t)v];
+    double r[3] = { 0.0, 0.0, 0.0 };
+    if (A.vel[v].w == 0.f && b.w > 0.f) {
+        const float4 d = A.pos[2 * (size_t)v + 1];
+        const float4 p = A.pred[v];
+        const double w = (double)b.w;
+        r[0] = ((double)d.x - (double)p.x) / w;
+        r[1] = ((double)d.y - (double)p.y) / w;
+        r[2] = ((double)d.z - (double)p.z) / w;
+        for (int k = A.vt_off[v]; k < A.vt_off[v + 1]; k++) {
+            const int e = A.vt_pos[k];
+            const int4 t = ta.tet[e];
+            double u0[3], u1[3], u2[3], u3[3];
+            disp_f64(A.pos, A.lo, t.x, true, u0);
+            disp_f64(A.pos, A.lo, t.y, true, u1);
+            disp_f64(A.pos, A.lo, t.z, true, u2);
+            disp_f64(A.pos, A.lo, t.w, true, u3);
+            double q[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) q[i] = ta.Q[(size_t)i * ta.qstride + e];
+            double F[9], CD, CH, J, gD[12], gH[12];
+            deformation_gradient<double>(u0, u1, u2, u3, q, F);
+            element_constraints<double>(F, (double)ta.gamma[e], &CD, &CH, &J);
+            constraint_gradients<double>(F, q, CD, J, gD, gH);
+            const float2 l = ta.lam[e];
+            const int c = t.x == v ? 0 : t.y == v ? 1 : t.z == v ? 2 : 3;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const double gd = c == 0 ? gD[i] : c == 1 ? gD[3 + i] : c == 2 ? gD[6 + i] : gD[9 + i];
+                const double gh = c == 0 ? gH[i] : c == 1 ? gH[3 + i] : c == 2 ? gH[6 + i] : gH[9 + i];
+                r[i] -= gd * (double)l.x + gh * (double)l.y;
+            }
+        }
+        if (A.vf_off) {
+            for (int k = A.vf_off[v]; k < A.vf_off[v + 1]; k++) {
+                const int jc = A.vf[k], j = jc / 3, c = jc - 3 * j;
+                if (!A.flag[j]) continue;
+                const double bc = A.cb[3 * (size_t)j + c], lam = A.clam[j];
+#pragma unroll
+                for (int i = 0; i < 3; i++) r[i] -= bc * A.cn[3 * (size_t)j + i] * lam;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) A.R[3 * (size_t)v + i] = r[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_attached(const int* __restrict__ att_v, int natt, int nv,
+                                                         double* __restrict__ R)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= natt) return;
+    const int v = att_v[a];
+    if (v < 0 || v >= nv) return;
+    R[3 * (size_t)v] = 0.0;
+    R[3 * (size_t)v + 1] = 0.0;
+    R[3 * (size_t)v + 2] = 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_squares(const double* __restrict__ x, size_t n,
+                                                       double* __restrict__ part)
+{
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        acc += x[i] * x[i];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------
 // Signed distance fields and contact detection in FP64:
 // SphereSDF.cpp:16-26, BoxSDF.cpp:19-84, GeometryUtils.cpp:6-24,92-102,
 // CollisionScene.cpp:171-214 (face loop) and 253-283 (Frank-Wolfe).
@@ -2307,6 +2401,25 @@ hipError_t launch_energy(hipStream_t s, const TetArrays& ta, const float4* pos, 
 hipError_t launch_sum_partials(hipStream_t s, const double* part, int n, double* out)
 {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, part, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_keep_prediction(hipStream_t s, const float4* pos, float4* pred, int nv)
+{
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_keep_prediction, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos, pred, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_primary_residual(hipStream_t s, const TetArrays& ta, const PrimaryArgs& A, double* out)
+{
+    if (A.nv <= 0 || A.nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_primary_residual, dim3(nblk(A.nv, kBlock)), dim3(kBlock), 0, s, ta, A);
+    if (A.natt > 0)
+        hipLaunchKernelGGL(k_zero_attached, dim3(nblk(A.natt, kBlock)), dim3(kBlock), 0, s, A.att_v, A.natt, A.nv,
+                           A.R);
+    hipLaunchKernelGGL(k_sum_squares, dim3(A.nblocks), dim3(kBlock), 0, s, A.R, 3 * (size_t)A.nv, A.part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, A.part, A.nblocks, out);
     return hipGetLastError();
 }
 

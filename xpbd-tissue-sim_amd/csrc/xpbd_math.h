@@ -113,19 +113,13 @@ XPBD_HD void deformation_gradient_split(const R* b, const R* d, const R* Q, R* F
                          + D[6 + r] * Q[3 * c + 2];
 }
 
-// One combined deviatoric + hydrostatic projection.
-//   u[12]   vertex displacements (vertex-major)
-//   w[4]    inverse masses
-//   Q[9]    rest-shape inverse, column-major
-//   atD/atH compliance divided by dt^2 (alpha tilde)
-//   lam[2]  multipliers, updated in place
-//   dx[12]  position corrections for the four vertices
+// Gradients of C_D and C_H at the four vertices (vertex-major xyz) from F and
+// the C_D and J that element_constraints returned for it:
+//   grad_i C_D = (F Q^T)_i / C_D,   grad_i C_H = dC_H/dJ (cof(F) Q^T)_i   (i < 3),
+// and vertex 3 the negative sum of the other three.
 template <typename R>
-XPBD_HD void project_tet_F(const R* F, const R* w, const R* Q, R atD, R atH, R gamma,
-                           R* lam, R* dx)
+XPBD_HD void constraint_gradients(const R* F, const R* Q, R CD, R J, R* gD, R* gH)
 {
-    R CD, CH, J;
-    element_constraints<R>(F, gamma, &CD, &CH, &J);
     const R inv_C = xrcp<R>(CD);
     const R a = J - R(1);
     const R fac = (J >= R(1)) ? xrcp<R>(J) : R(1) - a + a * a;
@@ -142,7 +136,6 @@ XPBD_HD void project_tet_F(const R* F, const R* w, const R* Q, R atD, R atH, R g
     Fc[7] = F[2] * F[3] - F[0] * F[5];
     Fc[8] = F[0] * F[4] - F[1] * F[3];
 
-    R gD[12], gH[12];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -155,6 +148,23 @@ XPBD_HD void project_tet_F(const R* F, const R* w, const R* Q, R atD, R atH, R g
         gD[9 + r] = -gD[r] - gD[3 + r] - gD[6 + r];
         gH[9 + r] = -gH[r] - gH[3 + r] - gH[6 + r];
     }
+}
+
+// One combined deviatoric + hydrostatic projection.
+//   u[12]   vertex displacements (vertex-major)
+//   w[4]    inverse masses
+//   Q[9]    rest-shape inverse, column-major
+//   atD/atH compliance divided by dt^2 (alpha tilde)
+//   lam[2]  multipliers, updated in place
+//   dx[12]  position corrections for the four vertices
+template <typename R>
+XPBD_HD void project_tet_F(const R* F, const R* w, const R* Q, R atD, R atH, R gamma,
+                           R* lam, R* dx)
+{
+    R CD, CH, J;
+    element_constraints<R>(F, gamma, &CD, &CH, &J);
+    R gD[12], gH[12];
+    constraint_gradients<R>(F, Q, CD, J, gD, gH);
 
     R s00 = R(0), s01 = R(0), s11 = R(0);
 #pragma unroll
