@@ -284,7 +284,10 @@ def learned_vs_plain(a, net_path):
     iterations (no network, no fallback), for scale.  Each run reports its
     trajectory error against the plain run after the last step, as
     max |x - x_plain| / max |x_plain| and in units of the plain run's largest
-    displacement from rest.  The learned runs replay captured steps; the
+    displacement from rest.  primary_residual repeats the plain run and three
+    of the others untimed with the primary residual (the momentum balance
+    (x - xt) / w - sum grad C^T lambda) recorded every step, and reports its
+    norm against the plain run's, step by step.  The learned runs replay captured steps; the
     network's inference time comes from a separate run of a few steps in
     kernel timing mode."""
     import numpy as np
@@ -371,6 +374,34 @@ def learned_vs_plain(a, net_path):
     s.close()
     out["learned_over_plain_at_equal_residual"] = (out["with_fallback"]["steps_per_s"] /
                                                    out["plain_steps_per_s_with_residual_readback"])
+
+    def primary_norms(iters, network=None, start=None, judged=False):
+        """Primary residual norm (momentum balance) of every step of a run,
+        recorded on the device in an untimed repeat of the run."""
+        s_, _, _ = make(iters)
+        s_.set_primary_residual(True)
+        if network is not None:
+            s_.attach(network)
+            s_.set_learned_start(start)
+        p = []
+        for n in range(a.learned_steps):
+            if network is None:
+                s_.step(1)
+            else:
+                s_.step_learned(a.learned_iters, r_plain[n] if judged else 0.0, 1.1)
+            p.append(s_.last_step_primary_residual())
+        s_.close()
+        return np.array(p)
+
+    p_plain = primary_norms(a.iters)
+    pr = {"plain_median": float(np.median(p_plain))}
+    for key, args in (("with_fallback", (a.iters, net, "multipliers", True)),
+                      ("positions_start_no_fallback", (a.iters, net, "positions", False)),
+                      ("plain_same_iterations", (a.learned_iters,))):
+        q = primary_norms(*args) / np.maximum(p_plain, 1e-300)
+        pr[key + "_over_plain_median"] = float(np.median(q))
+        pr[key + "_over_plain_max"] = float(np.max(q))
+    out["primary_residual"] = pr
     return out
 
 

@@ -176,15 +176,18 @@ int xpbd_constraint_residual(xpbd_handle* h, double* norm_out, double* vec_out);
  * 0 = never (default), 1 = every step: ||r||_2 is evaluated on the device at the
  * end of every solve, before the pin reset, where XPBDSolver::solve computes it
  * for EVERY_SUBSTEP (XPBDSolver.hpp:91-96).  xpbd_last_step_residual returns
- * the value of the last step (XPBD_E_STATE when the policy is off).  Turning
- * the policy on allocates the primary residual's buffers (4 floats and 3
- * doubles per vertex, plus the vertex -> element and vertex -> face lists). */
+ * the value of the last step (XPBD_E_STATE when the policy is off). */
 int xpbd_set_residual_policy(xpbd_handle* h, int32_t policy);
 int xpbd_last_step_residual(xpbd_handle* h, double* out);
 
-/* Primary residual of the last step's solve (Equation (8) of the XPBD paper),
- * recorded under the residual policy at the same point as the constraint
- * residual: per vertex
+/* Primary residual of every step's solve (Equation (8) of the XPBD paper),
+ * on = 1 / off = 0 (default), recorded at the point of the constraint
+ * residual.  The reference's EVERY_SUBSTEP policy asks for both
+ * (XPBDSolver.hpp:91-96); here it has its own switch because it costs an FP64
+ * pass over every vertex's elements, which the constraint residual that the
+ * learned step's fallback reads does not need.  Switching it on allocates 4
+ * floats and 3 doubles per vertex plus the vertex -> element and vertex ->
+ * face lists.  xpbd_last_step_primary_residual returns, per vertex
  *     R_v = (x_v - xt_v) / w_v - sum grad_v C^T lambda
  * over both constraints of every element and every active contact around v,
  * with xt the step's inertial prediction and w the inverse mass (first order:
@@ -192,8 +195,9 @@ int xpbd_last_step_residual(xpbd_handle* h, double* out);
  * *norm_out and, if vec_out != NULL, the 3*V entries (caller's vertex order).
  * Replaces XPBDSolver::_calculatePrimaryResidual + primaryResidual()
  * (XPBDSolver.hpp:127-130; the reference's _calculatePrimaryResidual, :322,
- * is empty, so its vector stays zero).  XPBD_E_STATE when the policy is off;
- * zero before the first step. */
+ * is empty, so its vector stays zero).  XPBD_E_STATE when it is off; zero
+ * before the first step. */
+int xpbd_set_primary_residual(xpbd_handle* h, int32_t on);
 int xpbd_last_step_primary_residual(xpbd_handle* h, double* norm_out, double* vec_out);
 
 /* Total strain energy sum(C^2 / alpha) over all deviatoric and hydrostatic

@@ -38,14 +38,19 @@ def test_beam_simulation_matches_oracle(tmp_path):
                   pinned=pinned)
     o = orc.Sim(mesh, dt=1e-3, g=9.81, iters=10, mode="gs")
     o.set_order(order)
+    o.set_primary_residual(True)
     o.step(100)
     err = rel_err(x, o.x())
     lines = log.strip().splitlines()
-    print(f"beam via xpbd_sim: rel err {err:.2e}; log header {lines[0]!r}, last {lines[-1]!r}")
+    print(f"beam via xpbd_sim: rel err {err:.2e}; log header {lines[0]!r}, last {lines[-1]!r}, "
+          f"oracle primary residual {o.last_primary_residual():.6e}")
     assert err <= 1e-5
-    assert len(lines) == 101 and lines[0].split()[1:] == ["Beam_Z-deflection[m]", "Beam_Constraint-Residual"]
+    assert len(lines) == 101 and lines[0].split()[1:] == ["Beam_Z-deflection[m]", "Beam_Constraint-Residual",
+                                                          "Beam_Primary-Residual"]
     defl = [float(l.split()[1]) for l in lines[1:]]
     assert defl[-1] > 0 and np.all(np.isfinite(defl))
+    # the last step's primary residual, after 100 steps of trajectory
+    assert float(lines[-1].split()[3]) == pytest.approx(o.last_primary_residual(), rel=1e-2)
 
 
 def test_block_with_sphere_on_reference_collision_cadence(tmp_path):

@@ -2,7 +2,8 @@
 on the device against the FP64 oracle's, one step at a time from a shared
 state.
 
-Both record, at the end of the global solve (before local collision passes
+Both record (switched on with xpbd_set_primary_residual on the device), at
+the end of the global solve (before local collision passes
 and the pin reset, where the reference's EVERY_SUBSTEP policy evaluates its
 residuals, XPBDSolver.hpp:91-96),
     R_v = (x_v - xt_v) / w_v - sum grad_v C^T lambda
@@ -54,7 +55,7 @@ def _pair(mesh, solver, variant, iters, det=1, first_order=None):
     if solver == "gs":
         o.set_order(gpu.tet_order())
         o.set_contact_order(gpu.face_rank())
-    gpu.set_residual_policy(True)
+    gpu.set_primary_residual(True)
     o.set_primary_residual(True)
     return gpu, o
 
@@ -139,17 +140,19 @@ def test_primary_residual_first_order_matches_oracle():
     assert np.all(np.abs(ratios - 1) <= 1e-2)
 
 
-def test_primary_residual_policy_off_and_reproducible():
-    """Off: XPBD_E_STATE.  On: zero before the first step; two handles of the
+def test_primary_residual_switch_and_reproducible():
+    """Off: XPBD_E_STATE.  On: zero before the first step; independent of the
+    residual policy (the constraint residual stays off); two handles of the
     same configuration record the same bits (a fixed-order gather)."""
     mesh, sph = _block()
     gpu = X.Solver(mesh, dt=DT, iters=10, solver="gs", collision_every=1)
     with pytest.raises(X.XpbdError):
         gpu.last_step_primary_residual()
-    gpu.set_residual_policy(True)
+    gpu.set_primary_residual(True)
     assert gpu.last_step_primary_residual() == 0.0
     _collider(gpu, sph)
     twin = X.Solver(mesh, dt=DT, iters=10, solver="gs", collision_every=1)
+    twin.set_primary_residual(True)
     twin.set_residual_policy(True)
     _collider(twin, sph)
     for _ in range(5):
@@ -158,6 +161,9 @@ def test_primary_residual_policy_off_and_reproducible():
         a, ra = gpu.last_step_primary_residual(vec=True)
         b, rb = twin.last_step_primary_residual(vec=True)
         assert a > 0 and a == b and np.array_equal(ra, rb)
-    gpu.set_residual_policy(False)
+    with pytest.raises(X.XpbdError):
+        gpu.last_step_residual()
+    assert twin.last_step_residual() > 0
+    gpu.set_primary_residual(False)
     with pytest.raises(X.XpbdError):
         gpu.last_step_primary_residual()

@@ -982,7 +982,7 @@ int enqueue_predict(xpbd_handle* h)
     hipError_t e = h->params.first_order
                        ? launch_predict_first_order(h->stream, h->d_pos, h->d_dz1, h->nv)
                        : launch_predict(h->stream, h->d_pos, h->d_vel, h->nv, (float)dt, (float)(-h->params.g * dt * dt));
-    if (e == hipSuccess && h->residual_policy) e = launch_keep_prediction(h->stream, h->d_pos, h->d_pred, h->nv);
+    if (e == hipSuccess && h->primary_residual) e = launch_keep_prediction(h->stream, h->d_pos, h->d_pred, h->nv);
     return e == hipSuccess ? XPBD_OK : hip_err(h, e, "predict launch");
 }
 
@@ -999,7 +999,7 @@ static int enqueue_step(xpbd_handle* h, bool detect, bool timed)
     if (detect && contacts_enabled(h) && (rc = enqueue_detect(h))) return rc;
     if ((rc = enqueue_predict(h))) return rc;
     if ((rc = enqueue_iterations(h, h->params.iters, timed))) return rc;
-    if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+    if (records_residuals(h) && (rc = enqueue_step_residual(h))) return rc;
     if ((rc = enqueue_local_collision(h))) return rc;
     return enqueue_finish(h);
 }
@@ -1263,16 +1263,19 @@ int device_residual(xpbd_handle* h, double* norm, double* vec_solver_order_dev)
     *norm = std::sqrt(s);
     return XPBD_OK;
 }
-// ||r||^2 and the primary residual at the end of the solve, before the local
-// collision passes and the pin reset (where the reference's EVERY_SUBSTEP
-// policy evaluates both, XPBDSolver.hpp:91-96), into d_step_res, d_pres and
-// d_step_pres.
+// ||r||^2 (residual policy) and the primary residual (when on) at the end of
+// the solve, before the local collision passes and the pin reset (where the
+// reference's EVERY_SUBSTEP policy evaluates both, XPBDSolver.hpp:91-96), into
+// d_step_res, d_pres and d_step_pres.
 int enqueue_step_residual(xpbd_handle* h)
 {
-    hipError_t e = launch_residual(h->stream, tet_arrays(h), h->d_pos, h->d_lo, true, h->nt, h->d_part,
-                                   h->red_blocks, nullptr);
-    if (e == hipSuccess) e = launch_sum_partials(h->stream, h->d_part, h->red_blocks, h->d_step_res);
-    if (e == hipSuccess) {
+    hipError_t e = hipSuccess;
+    if (h->residual_policy) {
+        e = launch_residual(h->stream, tet_arrays(h), h->d_pos, h->d_lo, true, h->nt, h->d_part, h->red_blocks,
+                            nullptr);
+        if (e == hipSuccess) e = launch_sum_partials(h->stream, h->d_part, h->red_blocks, h->d_step_res);
+    }
+    if (e == hipSuccess && h->primary_residual) {
         PrimaryArgs A;
         A.pos = h->d_pos;
         A.lo = h->d_lo;
@@ -1331,11 +1334,22 @@ extern "C" int xpbd_set_residual_policy(xpbd_handle* h, int32_t policy)
     if (!h) return XPBD_E_INVALID_ARG;
     if (policy != 0 && policy != 1) return set_err(h, XPBD_E_INVALID_ARG, "policy must be 0 or 1");
     if (policy != h->residual_policy) {
-        if (policy) {
+        h->residual_policy = policy;
+        invalidate_graphs(h);
+    }
+    return XPBD_OK;
+}
+
+extern "C" int xpbd_set_primary_residual(xpbd_handle* h, int32_t on)
+{
+    if (!h) return XPBD_E_INVALID_ARG;
+    if (on != 0 && on != 1) return set_err(h, XPBD_E_INVALID_ARG, "on must be 0 or 1");
+    if (on != h->primary_residual) {
+        if (on) {
             const int rc = ensure_primary_residual(h);
             if (rc) return rc;
         }
-        h->residual_policy = policy;
+        h->primary_residual = on;
         invalidate_graphs(h);
     }
     return XPBD_OK;
@@ -1345,7 +1359,7 @@ extern "C" int xpbd_last_step_primary_residual(xpbd_handle* h, double* norm_out,
 {
     if (!h) return XPBD_E_INVALID_ARG;
     if (!norm_out) return set_err(h, XPBD_E_INVALID_ARG, "norm_out is NULL");
-    if (!h->residual_policy) return set_err(h, XPBD_E_STATE, "residual policy is off");
+    if (!h->primary_residual) return set_err(h, XPBD_E_STATE, "primary residual is off");
     double s = 0;
     hipError_t e = hipMemcpyAsync(&s, h->d_step_pres, sizeof(double), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess && vec_out)

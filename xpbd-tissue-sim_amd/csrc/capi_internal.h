@@ -120,8 +120,9 @@ struct xpbd_handle {
     int64_t host_step = 0;
     int residual_policy = 0;     // 1: record ||r|| at the end of every solve (reference EVERY_SUBSTEP)
     double* d_step_res = nullptr;
-    // primary residual under the policy: the prediction's in-step motion, the
-    // per-vertex residual (3 nv) and its squared norm
+    // primary residual, recorded at the same point when on: the prediction's
+    // in-step motion, the per-vertex residual (3 nv) and its squared norm
+    int primary_residual = 0;
     float4* d_pred = nullptr;
     double* d_pres = nullptr;
     double* d_step_pres = nullptr;
@@ -163,7 +164,10 @@ void timed_end(xpbd_handle* h);
 int collect_timing(xpbd_handle* h);
 int download_f4(xpbd_handle* h, std::vector<float4>& hu, const float4* src, size_t n);
 int device_residual(xpbd_handle* h, double* norm, double* vec_solver_order_dev);
+// residuals recorded at the end of every solve (constraint residual under the
+// residual policy, primary residual when switched on)
 int enqueue_step_residual(xpbd_handle* h);
+inline bool records_residuals(const xpbd_handle* h) { return h->residual_policy || h->primary_residual; }
 void destroy_learned(xpbd_handle* h);
 // Clears the previous-step correction the learned warm start reads (state was replaced).
 int learned_reset_history(xpbd_handle* h);

@@ -1205,7 +1205,7 @@ extern "C" int xpbd_collect_training(xpbd_handle* h, int32_t nsteps, int32_t cap
                                       h->stream);
         if (e != hipSuccess) return hip_err(h, e, "collect features");
         if ((rc = enqueue_iterations(h, h->params.iters, false))) return rc;
-        if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+        if (records_residuals(h) && (rc = enqueue_step_residual(h))) return rc;
         if ((rc = enqueue_local_collision(h))) return rc;
         if ((rc = enqueue_store_cprev(h)) || (rc = enqueue_finish(h))) return rc;
         e = hipMemcpyAsync(corr.data(), L->d_cprev, sizeof(float4) * nv, hipMemcpyDeviceToHost, h->stream);
@@ -1262,7 +1262,7 @@ static int enqueue_learned_attempt(xpbd_handle* h, int32_t iters, bool det, bool
     e = launch_residual(s, tet_arrays(h), h->d_pos, h->d_lo, true, h->nt, h->d_part, h->red_blocks, nullptr);
     if (e == hipSuccess) e = launch_sum_partials(s, h->d_part, h->red_blocks, L->d_res);
     if (e != hipSuccess) return hip_err(h, e, "learned residual");
-    if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+    if (records_residuals(h) && (rc = enqueue_step_residual(h))) return rc;
     if ((rc = enqueue_local_collision(h))) return rc;
     if ((rc = enqueue_store_cprev(h)) || (rc = enqueue_finish(h))) return rc;
     return XPBD_OK;
@@ -1355,7 +1355,7 @@ extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_resid
         if ((rc = enqueue_predict(h)) || (rc = enqueue_copy_pred(h))) return rc;
         if ((rc = enqueue_iterations(h, h->params.iters, false))) return rc;
         if ((rc = device_residual(h, &r, nullptr))) return rc;
-        if (h->residual_policy && (rc = enqueue_step_residual(h))) return rc;
+        if (records_residuals(h) && (rc = enqueue_step_residual(h))) return rc;
         if ((rc = enqueue_local_collision(h))) return rc;
         if ((rc = enqueue_store_cprev(h)) || (rc = enqueue_finish(h))) return rc;
         e = hipStreamSynchronize(s);

@@ -227,7 +227,11 @@ void XPBDGPUSolver::setup()
     p.local_collision_iters = _obj->config().collisions ? _obj->config().num_local_collision_iters : 0;
     const int rc = xpbd_create(&d, &p, &_h);
     if (rc != XPBD_OK) throw Sim::SimError(rc, std::string("xpbd_create: ") + xpbd_last_error(nullptr));
-    if (_policy != XPBDSolverResidualPolicyEnum::NEVER) check(xpbd_set_residual_policy(_h, 1));
+    if (_policy != XPBDSolverResidualPolicyEnum::NEVER) {
+        // EVERY_SUBSTEP records both residuals (XPBDSolver.hpp:91-96)
+        check(xpbd_set_residual_policy(_h, 1));
+        check(xpbd_set_primary_residual(_h, 1));
+    }
 }
 
 void XPBDGPUSolver::solve() { check(xpbd_step(_h, 1)); }
@@ -244,6 +248,21 @@ double XPBDGPUSolver::lastConstraintResidualNorm() const
 {
     double r = 0;
     check(xpbd_last_step_residual(_h, &r));
+    return r;
+}
+
+std::vector<double> XPBDGPUSolver::primaryResidual() const
+{
+    std::vector<double> r(3 * (size_t)_obj->numVertices());
+    double norm = 0;
+    check(xpbd_last_step_primary_residual(_h, &norm, r.data()));
+    return r;
+}
+
+double XPBDGPUSolver::lastPrimaryResidualNorm() const
+{
+    double r = 0;
+    check(xpbd_last_step_primary_residual(_h, &r, nullptr));
     return r;
 }
 
@@ -533,6 +552,8 @@ Vec3r XPBDMeshObject::vertex(int index) const { return { _x[3 * index], _x[3 * i
 std::vector<double> XPBDMeshObject::lastConstraintResidual() const { return _solver->constraintResidual(); }
 
 double XPBDMeshObject::lastConstraintResidualNorm() const { return _solver->lastConstraintResidualNorm(); }
+std::vector<double> XPBDMeshObject::lastPrimaryResidual() const { return _solver->primaryResidual(); }
+double XPBDMeshObject::lastPrimaryResidualNorm() const { return _solver->lastPrimaryResidualNorm(); }
 
 double XPBDMeshObject::totalStrainEnergy() const { return _solver->totalStrainEnergy(); }
 
@@ -668,8 +689,10 @@ void BeamSimulation::setup()
         const int tip = p->getClosestVertex({ c[0], b.max[1], c[2] });
         const double z0 = p->vertex(tip)[2];
         addOutput(p->name() + "_Z-deflection[m]", [p, tip, z0]() { return z0 - p->vertex(tip)[2]; });
-        if (p->config().residual_policy != Config::XPBDSolverResidualPolicyEnum::NEVER)
+        if (p->config().residual_policy != Config::XPBDSolverResidualPolicyEnum::NEVER) {
             addOutput(p->name() + "_Constraint-Residual", [p]() { return p->lastConstraintResidualNorm(); });
+            addOutput(p->name() + "_Primary-Residual", [p]() { return p->lastPrimaryResidualNorm(); });
+        }
     }
 }
 

@@ -131,6 +131,10 @@ public:
     int numIterations() const { return _num_iter; }
     std::vector<double> constraintResidual() const; // r = C + alpha~ lambda, 2 per element
     double lastConstraintResidualNorm() const;      // EVERY_SUBSTEP value of the last step
+    // primary residual (x - xt) / w - sum grad C^T lambda, 3 per vertex, of the
+    // last step (XPBDSolver::primaryResidual, XPBDSolver.hpp:127-130), and its norm
+    std::vector<double> primaryResidual() const;
+    double lastPrimaryResidualNorm() const;
     double totalStrainEnergy() const;
     void setCollider(int32_t kind, const Sim::Vec3r& pos, const std::array<double, 4>& quat,
                      const Sim::Vec3r& params, const Sim::Vec3r& vel);
@@ -174,6 +178,8 @@ public:
     Vec3r vertex(int index) const;
     std::vector<double> lastConstraintResidual() const;
     double lastConstraintResidualNorm() const;
+    std::vector<double> lastPrimaryResidual() const;
+    double lastPrimaryResidualNorm() const;
     double totalStrainEnergy() const;
     AABB boundingBox() const;
     // Attachment (grasp) constraints, XPBDMeshObject.cpp:254-275: vertex v_ind is

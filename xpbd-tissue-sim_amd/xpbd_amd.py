@@ -81,6 +81,7 @@ _sig("xpbd_constraint_residual", [_vp, _dp, _dp])
 _sig("xpbd_strain_energy", [_vp, _dp])
 _sig("xpbd_set_residual_policy", [_vp, C.c_int32])
 _sig("xpbd_last_step_residual", [_vp, _dp])
+_sig("xpbd_set_primary_residual", [_vp, C.c_int32])
 _sig("xpbd_last_step_primary_residual", [_vp, _dp, _dp])
 _sig("xpbd_num_colors", [_vp, _ip])
 _sig("xpbd_get_tet_order", [_vp, _ip])
@@ -362,10 +363,13 @@ class Solver:
         self._c(lib.xpbd_last_step_residual(self.h, C.byref(r)))
         return r.value
 
+    def set_primary_residual(self, on: bool):
+        self._c(lib.xpbd_set_primary_residual(self.h, int(on)))
+
     def last_step_primary_residual(self, vec=False):
         """Norm (and, with vec, the V x 3 vector) of the primary residual
         (x - xt) / w - sum grad C^T lambda recorded at the end of the last
-        step's solve (residual policy on)."""
+        step's solve (set_primary_residual on)."""
         n = C.c_double()
         out = np.empty((self.nv, 3), np.float64) if vec else None
         self._c(lib.xpbd_last_step_primary_residual(self.h, C.byref(n), ptr(out, C.c_double)))
