@@ -186,8 +186,8 @@ int xpbd_last_step_residual(xpbd_handle* h, double* out);
  * (XPBDSolver.hpp:91-96); here it has its own switch because it costs an FP64
  * pass over every vertex's elements, which the constraint residual that the
  * learned step's fallback reads does not need.  Switching it on allocates 4
- * floats and 3 doubles per vertex plus the vertex -> element and vertex ->
- * face lists.  xpbd_last_step_primary_residual returns, per vertex
+ * floats and 3 doubles per vertex, 12 doubles per element, and the vertex ->
+ * element and vertex -> face lists.  xpbd_last_step_primary_residual returns, per vertex
  *     R_v = (x_v - xt_v) / w_v - sum grad_v C^T lambda
  * over both constraints of every element and every active contact around v,
  * with xt the step's inertial prediction and w the inverse mass (first order:

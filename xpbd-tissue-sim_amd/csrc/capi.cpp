@@ -1282,7 +1282,7 @@ int enqueue_step_residual(xpbd_handle* h)
         A.vel = h->d_vel;
         A.pred = h->d_pred;
         A.vt_off = h->d_vt_off;
-        A.vt_pos = h->d_vt_pos;
+        A.vt_slot = h->d_vt_slot;
         A.vf_off = h->nf > 0 ? h->d_vf_off : nullptr;
         A.vf = h->d_vf;
         A.flag = h->d_flag;
@@ -1292,6 +1292,8 @@ int enqueue_step_residual(xpbd_handle* h)
         A.att_v = h->d_att_v;
         A.natt = h->natt;
         A.nv = h->nv;
+        A.nt = h->nt;
+        A.ef = h->d_eforce;
         A.R = h->d_pres;
         A.part = h->d_part;
         A.nblocks = h->red_blocks;
@@ -1301,7 +1303,8 @@ int enqueue_step_residual(xpbd_handle* h)
 }
 
 // Buffers of the primary residual: the vertex lists it gathers over, the kept
-// prediction, the residual vector and its norm (zero until the first step).
+// prediction, the element forces, the residual vector and its norm (zero
+// until the first step).
 static int ensure_primary_residual(xpbd_handle* h)
 {
     hipError_t e = hipStreamSynchronize(h->stream);
@@ -1309,12 +1312,31 @@ static int ensure_primary_residual(xpbd_handle* h)
     int rc = ensure_vertex_elements(h, false);
     if (!rc) rc = ensure_vertex_faces(h);
     if (rc) return rc;
+    if (!h->d_vt_slot) {
+        // the slot of (solver position i, corner j) in the vertex -> element list,
+        // filled in the order that list was built (positions ascending, corners in order)
+        const int nv = h->nv, nt = h->nt;
+        std::vector<int32_t> fill((size_t)nv + 1, 0), slot(4 * (size_t)nt);
+        for (int e = 0; e < nt; e++)
+            for (int j = 0; j < 4; j++) fill[h->tets_host[4 * (size_t)e + j] + 1]++;
+        for (int v = 0; v < nv; v++) fill[v + 1] += fill[v];
+        for (int i = 0; i < nt; i++) {
+            const int e = h->order[i];
+            for (int j = 0; j < 4; j++) slot[4 * (size_t)i + j] = fill[h->tets_host[4 * (size_t)e + j]]++;
+        }
+        int32_t* d = nullptr;
+        if ((rc = dalloc(h, &d, slot.size()))) return rc;
+        e = hipMemcpy(d, slot.data(), sizeof(int32_t) * slot.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_err(h, e, "vertex-element slots");
+        h->d_vt_slot = d;
+    }
     if (!h->d_pred) {
         float4* pred = nullptr;
-        double *pres = nullptr, *norm = nullptr;
+        double *pres = nullptr, *norm = nullptr, *ef = nullptr;
         rc = dalloc(h, &pred, (size_t)h->nv);
         if (!rc) rc = dalloc(h, &pres, 3 * (size_t)h->nv);
         if (!rc) rc = dalloc(h, &norm, 1);
+        if (!rc) rc = dalloc(h, &ef, 3 * 4 * (size_t)h->nt);
         if (rc) return rc;
         e = launch_fill_zero(h->stream, pred, sizeof(float4) * (size_t)h->nv);
         if (e == hipSuccess) e = launch_fill_zero(h->stream, pres, sizeof(double) * 3 * (size_t)h->nv);
@@ -1324,6 +1346,7 @@ static int ensure_primary_residual(xpbd_handle* h)
         h->d_pred = pred;
         h->d_pres = pres;
         h->d_step_pres = norm;
+        h->d_eforce = ef;
     }
     return XPBD_OK;
 }

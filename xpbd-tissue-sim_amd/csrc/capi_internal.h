@@ -121,9 +121,12 @@ struct xpbd_handle {
     int residual_policy = 0;     // 1: record ||r|| at the end of every solve (reference EVERY_SUBSTEP)
     double* d_step_res = nullptr;
     // primary residual, recorded at the same point when on: the prediction's
-    // in-step motion, the per-vertex residual (3 nv) and its squared norm
+    // in-step motion, the constraint forces of every (element, vertex) pair in
+    // vertex order, the per-vertex residual (3 nv) and its squared norm
     int primary_residual = 0;
     float4* d_pred = nullptr;
+    int32_t* d_vt_slot = nullptr;   // 4 nt: slot of (solver position, corner) in the vertex -> element list
+    double* d_eforce = nullptr;     // 3 per slot
     double* d_pres = nullptr;
     double* d_step_pres = nullptr;
     hipGraphExec_t graph_exec[2] = { nullptr, nullptr };  // [0] plain step, [1] step with detection

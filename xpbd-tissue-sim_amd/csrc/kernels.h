@@ -147,8 +147,8 @@ struct PrimaryArgs {
     const float4* lo;
     const float4* vel;    // .w != 0 marks pins
     const float4* pred;
-    const int* vt_off;    // vertex -> solver positions of its elements
-    const int* vt_pos;
+    const int* vt_off;    // vertex -> its range of element-corner slots
+    const int* vt_slot;   // 4 * nt: slot of (element in solver position e, corner c) at 4 e + c
     const int* vf_off;    // vertex -> face slot * 3 + corner (null without faces)
     const int* vf;
     const int* flag;      // per face slot: contact active
@@ -158,6 +158,8 @@ struct PrimaryArgs {
     const int* att_v;
     int natt;
     int nv;
+    int nt;
+    double* ef;           // 3 per slot: grad C^T lambda of one element at one vertex, in vertex order
     double* R;            // 3 * nv
     double* part;         // nblocks partial sums of R^2
     int nblocks;

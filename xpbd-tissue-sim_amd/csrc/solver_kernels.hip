@@ -1041,14 +1041,46 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restric
 // in-step motion minus the motion the inertial prediction set, so it is formed
 // exactly.  Pinned vertices read zero (their force is the support's), and so
 // do attached ones: a zero-compliance attachment holds its vertex wherever the
-// balance needs.  One thread per vertex walks its elements and faces in the
-// fixed order of its lists, so the result is reproducible.
+// balance needs.  One thread per element writes grad C^T lambda at each of its
+// four vertices into that vertex's slot of a vertex-ordered list; one thread
+// per vertex then sums its contiguous slots and its faces' terms in a fixed
+// order, so the result is reproducible.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_keep_prediction(const float4* __restrict__ pos, float4* __restrict__ pred,
                                                            int nv)
 {
     const int v = blockIdx.x * kBlock + threadIdx.x;
     if (v < nv) pred[v] = pos[2 * (size_t)v + 1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_element_forces(TetArrays ta, PrimaryArgs A)
+{
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= A.nt) return;
+    const int4 t = ta.tet[e];
+    double u0[3], u1[3], u2[3], u3[3];
+    disp_f64(A.pos, A.lo, t.x, true, u0);
+    disp_f64(A.pos, A.lo, t.y, true, u1);
+    disp_f64(A.pos, A.lo, t.z, true, u2);
+    disp_f64(A.pos, A.lo, t.w, true, u3);
+    double q[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) q[i] = ta.Q[(size_t)i * ta.qstride + e];
+    double F[9], ss = 0.0, gD[12], gH[12];
+    deformation_gradient<double>(u0, u1, u2, u3, q, F);
+#pragma unroll
+    for (int i = 0; i < 9; i++) ss += F[i] * F[i];
+    // the gradients need C_D = |F| and J, not C_H
+    constraint_gradients<double>(F, q, sqrt(ss), det3cm<double>(F), gD, gH);
+    const float2 l = ta.lam[e];
+    const int4 slot = reinterpret_cast<const int4*>(A.vt_slot)[e];
+    const int sl[4] = { slot.x, slot.y, slot.z, slot.w };
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        double* f = A.ef + 3 * (size_t)sl[c];
+#pragma unroll
+        for (int i = 0; i < 3; i++) f[i] = gD[3 * c + i] * (double)l.x + gH[3 * c + i] * (double)l.y;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_primary_residual(TetArrays ta, PrimaryArgs A)
@@ -1065,28 +1097,9 @@ __global__ __launch_bounds__(kBlock) void k_primary_residual(TetArrays ta, Prima
         r[1] = ((double)d.y - (double)p.y) / w;
         r[2] = ((double)d.z - (double)p.z) / w;
         for (int k = A.vt_off[v]; k < A.vt_off[v + 1]; k++) {
-            const int e = A.vt_pos[k];
-            const int4 t = ta.tet[e];
-            double u0[3], u1[3], u2[3], u3[3];
-            disp_f64(A.pos, A.lo, t.x, true, u0);
-            disp_f64(A.pos, A.lo, t.y, true, u1);
-            disp_f64(A.pos, A.lo, t.z, true, u2);
-            disp_f64(A.pos, A.lo, t.w, true, u3);
-            double q[9];
+            const double* f = A.ef + 3 * (size_t)k;
 #pragma unroll
-            for (int i = 0; i < 9; i++) q[i] = ta.Q[(size_t)i * ta.qstride + e];
-            double F[9], CD, CH, J, gD[12], gH[12];
-            deformation_gradient<double>(u0, u1, u2, u3, q, F);
-            element_constraints<double>(F, (double)ta.gamma[e], &CD, &CH, &J);
-            constraint_gradients<double>(F, q, CD, J, gD, gH);
-            const float2 l = ta.lam[e];
-            const int c = t.x == v ? 0 : t.y == v ? 1 : t.z == v ? 2 : 3;
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const double gd = c == 0 ? gD[i] : c == 1 ? gD[3 + i] : c == 2 ? gD[6 + i] : gD[9 + i];
-                const double gh = c == 0 ? gH[i] : c == 1 ? gH[3 + i] : c == 2 ? gH[6 + i] : gH[9 + i];
-                r[i] -= gd * (double)l.x + gh * (double)l.y;
-            }
+            for (int i = 0; i < 3; i++) r[i] -= f[i];
         }
         if (A.vf_off) {
             for (int k = A.vf_off[v]; k < A.vf_off[v + 1]; k++) {
@@ -2414,6 +2427,7 @@ hipError_t launch_keep_prediction(hipStream_t s, const float4* pos, float4* pred
 hipError_t launch_primary_residual(hipStream_t s, const TetArrays& ta, const PrimaryArgs& A, double* out)
 {
     if (A.nv <= 0 || A.nblocks <= 0) return hipSuccess;
+    if (A.nt > 0) hipLaunchKernelGGL(k_element_forces, dim3(nblk(A.nt, kBlock)), dim3(kBlock), 0, s, ta, A);
     hipLaunchKernelGGL(k_primary_residual, dim3(nblk(A.nv, kBlock)), dim3(kBlock), 0, s, ta, A);
     if (A.natt > 0)
         hipLaunchKernelGGL(k_zero_attached, dim3(nblk(A.natt, kBlock)), dim3(kBlock), 0, s, A.att_v, A.natt, A.nv,
