@@ -167,3 +167,97 @@ def test_primary_residual_switch_and_reproducible():
     gpu.set_primary_residual(False)
     with pytest.raises(X.XpbdError):
         gpu.last_step_primary_residual()
+
+
+def test_primary_residual_of_the_learned_step_matches_oracle():
+    """The learned step (network correction, multiplier start, 3 iterations,
+    no fallback) against the oracle's orc_step_warm from the same state and
+    correction: the prediction kept is the inertial one, before the
+    correction.  The multiplier start breaks the momentum balance, so its
+    residual is larger than the plain solve's from the same state."""
+    import train_warmstart as TW
+    mesh, sph = _block(seed=17, depth=0.004)
+    gpu, o = _pair(mesh, "gs", "quad", 20)
+    plain, _ = _pair(mesh, "gs", "quad", 20)
+    net = TW.load(TW.DEFAULT_WEIGHTS)
+    gpu.attach(net)
+    _collider(gpu, sph)
+    _collider(plain, sph)
+    w = o.inverse_inertia()
+    pinned = mesh.pinned.astype(bool)
+    errs, ratios, over_plain = [], [], []
+    for k in range(15):
+        x0, v0 = gpu.positions(), gpu.velocities()
+        corr = net.forward(gpu.features()).astype(np.float64)
+        corr[pinned] = 0.0
+        plain.set_state(x0, v0)
+        plain.step(1)
+        gpu.step_learned(3, ref_residual=0.0)
+        face, cp, cn, cb = gpu.contacts()
+        o.set_state(x0, v0)
+        o.set_contacts(face, cp, cn, cb)
+        o.step_warm(corr, 3, detect=False)
+        ng, Rg = gpu.last_step_primary_residual(vec=True)
+        nr, Rr = o.last_primary_residual(vec=True)
+        scale = np.abs((o.x() - o.last_prediction()) / w[:, None])[~pinned].max()
+        errs.append(np.abs(Rg - Rr).max() / scale)
+        ratios.append(ng / nr)
+        over_plain.append(ng / plain.last_step_primary_residual())
+    errs, ratios = np.array(errs), np.array(ratios)
+    print(f"primary residual, learned step: max err/inertia {errs.max():.2e}, norm ratio "
+          f"{ratios.min():.6f}..{ratios.max():.6f}, learned / plain-20 median {np.median(over_plain):.1f}")
+    assert errs.max() <= TOL
+    assert np.all(np.abs(ratios - 1) <= 1e-2)
+    assert np.median(over_plain) > 3
+
+
+def test_primary_residual_of_a_batch_matches_each_instance():
+    """Four uneven cubes in one batched handle, each with its own sphere;
+    every instance's slice of the residual against its own oracle from the
+    shared state and the device's contacts."""
+    rng = np.random.default_rng(23)
+    meshes, sdfs = [], []
+    for _ in range(4):
+        nx, ny, nz = (int(v) for v in rng.integers(3, 8, size=3))
+        m = W.grid_object(nx, ny, nz, 0.01, E=float(rng.uniform(2e3, 2e4)), nu=0.47, pin_axis=2, noise=0.0,
+                          faces=True, rng=rng)
+        ext = np.array([nx, ny, nz]) * 0.01
+        r = 0.6 * ext.min()
+        meshes.append(m)
+        sdfs.append((X.SDF_SPHERE, (ext[0] * 0.43, ext[1] * 0.57, ext[2] + r - 0.08 * ext[2]), (0, 0, 0, 1),
+                     (r, 0, 0)))
+    gpu = X.Solver(meshes, dt=DT, g=G, iters=10, solver="gs", variant="quad", collision_every=1)
+    for i, (kind, pos, quat, prm) in enumerate(sdfs):
+        gpu.set_sdf(kind, pos, quat, prm, instance=i)
+    gpu.set_primary_residual(True)
+    t_off = np.concatenate([[0], np.cumsum([m.nt for m in meshes])])
+    v_off = np.concatenate([[0], np.cumsum([m.nv for m in meshes])])
+    f_off = np.concatenate([[0], np.cumsum([len(m.faces) for m in meshes])])
+    order, rank = gpu.tet_order(), gpu.face_rank()
+    orcs = []
+    for i, m in enumerate(meshes):
+        o = orc.Sim(m, dt=DT, g=G, iters=10, mode="gs")
+        o.set_order(order[(order >= t_off[i]) & (order < t_off[i + 1])] - t_off[i])
+        o.set_contact_order(rank[f_off[i]:f_off[i + 1]])
+        o.set_primary_residual(True)
+        orcs.append(o)
+    worst, ncon = 0.0, 0
+    for _ in range(15):
+        x0, v0 = gpu.positions(), gpu.velocities()
+        gpu.step(1)
+        face, cp, cn, cb = gpu.contacts()
+        ncon = max(ncon, len(face))
+        _, Rg = gpu.last_step_primary_residual(vec=True)
+        for i, o in enumerate(orcs):
+            vs = slice(v_off[i], v_off[i + 1])
+            o.set_state(x0[vs], v0[vs])
+            sel = (face >= f_off[i]) & (face < f_off[i + 1])
+            o.set_contacts(face[sel] - f_off[i], cp[sel], cn[sel], cb[sel])
+            o.step(1, detect=False)
+            _, Rr = o.last_primary_residual(vec=True)
+            free = ~meshes[i].pinned.astype(bool)
+            scale = np.abs((o.x() - o.last_prediction()) / o.inverse_inertia()[:, None])[free].max()
+            worst = max(worst, np.abs(Rg[vs] - Rr).max() / scale)
+    print(f"primary residual, batch of {len(meshes)}: contacts up to {ncon}, max err/inertia {worst:.2e}")
+    assert ncon > 0
+    assert worst <= TOL
