@@ -237,8 +237,34 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// FP32 master -> BF16 MFMA layouts: W1 [128][32] (k >= 24 zero), W2 [128][128],
-// W3 [16][128] (rows >= 3 zero)
+// The hidden feature at MFMA K position pos (0..31) of K block m in the
+// inference kernel: the order in which a 16 x 16 output tile pair leaves the
+// features in a lane (position 8q + i holds 4q + i of the block's first tile for
+// i < 4, 4q + i - 4 of its second tile for i >= 4).
+__host__ __device__ constexpr int mlp_k_feature(int m, int pos)
+{
+    return 32 * m + ((pos & 7) < 4 ? 4 * (pos >> 3) + (pos & 7) : 16 + 4 * (pos >> 3) + (pos & 7) - 4);
+}
+
+constexpr bool mlp_k_order_is_a_permutation()
+{
+    for (int m = 0; m < kH / 32; m++)
+        for (int f = 32 * m; f < 32 * m + 32; f++) {
+            int hits = 0;
+            for (int pos = 0; pos < 32; pos++) hits += mlp_k_feature(m, pos) == f;
+            if (hits != 1) return false;
+        }
+    return true;
+}
+static_assert(mlp_k_order_is_a_permutation(), "every hidden feature at exactly one K position");
+// the inference kernel's shapes: one 32-wide K step for the inputs (8 per lane,
+// lanes q < 3 full), 16-wide output tiles in pairs, at most 4 outputs (lanes q = 0)
+static_assert(kInP == 32 && kIn % 8 == 0 && kIn <= kInP && kH % 32 == 0 && kOut <= 4 && kOutP == 16,
+              "shapes of k_mlp_fused");
+
+// FP32 master -> BF16 MFMA layouts of the inference kernel: W1 [128][32] (k >= 24
+// zero), W2 [128][128] and W3 [16][128] (rows >= 3 zero) with their hidden K index
+// in mlp_k_feature order
 __global__ void k_pack(const float* __restrict__ p, __bf16* __restrict__ w1, __bf16* __restrict__ w2,
                        __bf16* __restrict__ w3)
 {
@@ -247,19 +273,28 @@ __global__ void k_pack(const float* __restrict__ p, __bf16* __restrict__ w1, __b
         const int o = i / kInP, k = i % kInP;
         w1[i] = (__bf16)(k < kIn ? p[oW1 + o * kIn + k] : 0.f);
     }
-    if (i < kH * kH) w2[i] = (__bf16)p[oW2 + i];
+    if (i < kH * kH) {
+        const int o = i / kH, k = i % kH;
+        w2[i] = (__bf16)p[oW2 + o * kH + mlp_k_feature(k / 32, k % 32)];
+    }
     if (i < kOutP * kH) {
         const int o = i / kH, k = i % kH;
-        w3[i] = (__bf16)(o < kOut ? p[oW3 + o * kH + k] : 0.f);
+        w3[i] = (__bf16)(o < kOut ? p[oW3 + o * kH + mlp_k_feature(k / 32, k % 32)] : 0.f);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: 4 waves per workgroup, 16 rows per wave per tile, weights
-// staged once per workgroup in LDS (BF16), activations re-laid out through a
-// per-wave LDS tile between layers (the D layout holds 4 rows x 1 column per
-// lane, the A layout 1 row x 8 consecutive k).
-// out[r] (float4) = scale * MLP((x_r - mean) * istd), w = 0.
+// Fused inference, 4 waves per workgroup, 16 rows per wave per tile.  Every
+// layer forms Z^T = W H^T: the weights (BF16, staged once per workgroup in LDS)
+// are the MFMA's A operand and the activations its B operand.  A 16 x 16 output
+// tile leaves lane (q = lane / 16, c = lane % 16) holding features 4q..4q+3 of
+// row c, so the two tiles of a 32-wide K block hold, per lane, features
+// 4q + {0..3} and 16 + 4q + {0..3} of the block: after bias and SiLU, rounded to
+// BF16, they are the next layer's B operand as they lie, with that layer's
+// weights packed in the same K order (mlp_k_feature).  The activations stay in
+// registers from input to output; the first layer's B operand is read from the
+// input rows directly (lane (q, c): features 8q..8q+7 of row c, zero past 24).
+// out[r] (float4) = scale * MLP((x_r - mean) * istd) (* mask[r]), w = 0.
 // ---------------------------------------------------------------------------
 constexpr int kFwdRows = 64;
 
@@ -272,9 +307,7 @@ __global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, 
     __shared__ alignas(16) __bf16 sw1[kH][kInP + 8];
     __shared__ alignas(16) __bf16 sw2[kH][kH + 8];
     __shared__ alignas(16) __bf16 sw3[kOutP][kH + 8];
-    __shared__ alignas(16) __bf16 sx[4][16][kInP + 8];
-    __shared__ alignas(16) __bf16 sh[4][16][kH + 8];
-    __shared__ float sb[2 * kH + kOut];
+    __shared__ alignas(16) float sb[2 * kH + kOutP];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // weights in 16-byte pieces (8 BF16 values; rows are multiples of 8 long and
     // the padded LDS rows start on 16-byte boundaries)
@@ -288,99 +321,86 @@ __global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, 
         sb[i] = params[oB1 + i];
         sb[kH + i] = params[oB2 + i];
     }
-    if (tid < kOut) sb[2 * kH + tid] = params[oB3 + tid];
+    if (tid < kOutP) sb[2 * kH + tid] = tid < kOut ? params[oB3 + tid] : 0.f;
     __syncthreads();
-    // from here on each wave works on its own rows and its own LDS tiles (sx[wv],
-    // sh[wv]): LDS operations of one wave complete in program order, so a wave
-    // barrier (which also keeps the compiler from moving LDS accesses across it)
-    // orders the lanes' writes before other lanes' reads, and the four waves run
-    // independently, overlapping one wave's MFMA with another's activations
-    const int ntile = (n + kFwdRows - 1) / kFwdRows;
-    const int col = lane & 15, kq = 8 * (lane >> 4), rq = 4 * (lane >> 4);
-    // input staging: lane holds feature k = lane % 32 of rows lane / 32 + 2 * it;
-    // the next tile's rows are loaded while the current tile is computed
-    constexpr int kXPer = 16 * kInP / 64;
-    const int xk = lane % kInP, xr = lane / kInP;
-    const bool kin = xk < kIn;
-    const float xmean = kin ? mean[xk] : 0.f, xistd = kin ? istd[xk] : 0.f;
-    float xn[kXPer];
-    auto load_rows = [&](int t) {
-        const int r0 = t * kFwdRows + wv * 16;
+    // from here on the waves share only the weights and biases, which they read
+    const int c = lane & 15, q = lane >> 4, kq = 8 * q;
+    const bool xin = kq < kIn;   // lanes q < 3 hold input features, q = 3 the zero padding
+    float xm[8], xs[8];
 #pragma unroll
-        for (int it = 0; it < kXPer; it++) {
-            const int gr = r0 + xr + 2 * it;
-            xn[it] = (gr < n && kin) ? X[(size_t)gr * kIn + xk] : 0.f;
+    for (int i = 0; i < 8; i++) {
+        xm[i] = xin ? mean[kq + i] : 0.f;
+        xs[i] = xin ? istd[kq + i] : 0.f;
+    }
+    const int ntile = (n + kFwdRows - 1) / kFwdRows;
+    // the next tile's input features are loaded while the current tile is computed
+    float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), xb = xa;
+    auto load_row = [&](int t) {
+        const int r = t * kFwdRows + wv * 16 + c;
+        xa = xb = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (xin && r < n) {
+            const float4* x4 = reinterpret_cast<const float4*>(X + (size_t)r * kIn + kq);
+            xa = x4[0];
+            xb = x4[1];
         }
     };
-    if (blockIdx.x < ntile) load_rows(blockIdx.x);
+    if ((int)blockIdx.x < ntile) load_row(blockIdx.x);
     for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
-        const int r0 = t * kFwdRows + wv * 16;
-        // standardised input rows -> sx (bf16)
+        const int r = t * kFwdRows + wv * 16 + c;
+        const float xv[8] = { xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w };
+        bf16x8 h0;
 #pragma unroll
-        for (int it = 0; it < kXPer; it++) {
-            const int gr = r0 + xr + 2 * it;
-            const float v = (gr < n && kin) ? (xn[it] - xmean) * xistd : 0.f;
-            sx[wv][xr + 2 * it][xk] = (__bf16)v;
+        for (int i = 0; i < 8; i++) h0[i] = (__bf16)((xv[i] - xm[i]) * xs[i]);
+        if (t + (int)gridDim.x < ntile) load_row(t + gridDim.x);
+        // layer 1: 8 output tiles, one K step; K block m of layer 2 from tiles 2m, 2m + 1
+        bf16x8 h1[kH / 32];
+#pragma unroll
+        for (int m = 0; m < kH / 32; m++) {
+            const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&sw1[32 * m + c][kq]);
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&sw1[32 * m + 16 + c][kq]);
+            const f32x4 z0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, h0, f32x4{ 0.f, 0.f, 0.f, 0.f }, 0, 0, 0);
+            const f32x4 z1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, h0, f32x4{ 0.f, 0.f, 0.f, 0.f }, 0, 0, 0);
+            const float4 b0 = *reinterpret_cast<const float4*>(&sb[32 * m + 4 * q]);
+            const float4 b1 = *reinterpret_cast<const float4*>(&sb[32 * m + 16 + 4 * q]);
+            h1[m] = bf16x8{ (__bf16)silu_fast(z0[0] + b0.x), (__bf16)silu_fast(z0[1] + b0.y),
+                            (__bf16)silu_fast(z0[2] + b0.z), (__bf16)silu_fast(z0[3] + b0.w),
+                            (__bf16)silu_fast(z1[0] + b1.x), (__bf16)silu_fast(z1[1] + b1.y),
+                            (__bf16)silu_fast(z1[2] + b1.z), (__bf16)silu_fast(z1[3] + b1.w) };
         }
-        if (t + (int)gridDim.x < ntile) load_rows(t + gridDim.x);
-        __builtin_amdgcn_wave_barrier();
-        // layer 1: 8 column tiles, one K step
-        const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&sx[wv][col][kq]);
-#pragma unroll
-        for (int nt = 0; nt < kH / 16; nt++) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sw1[nt * 16 + col][kq]);
-            f32x4 z = { 0.f, 0.f, 0.f, 0.f };
-            z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, z, 0, 0, 0);
-#pragma unroll
-            for (int j = 0; j < 4; j++) sh[wv][rq + j][nt * 16 + col] = (__bf16)silu_fast(z[j] + sb[nt * 16 + col]);
-        }
-        __builtin_amdgcn_wave_barrier();
-        // layer 2: 8 column tiles x 4 K steps
+        // layer 2: 8 output tiles x 4 K steps
         f32x4 z2[kH / 16];
 #pragma unroll
         for (int nt = 0; nt < kH / 16; nt++) z2[nt] = f32x4{ 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
-        for (int kk = 0; kk < kH / 32; kk++) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sh[wv][col][kk * 32 + kq]);
+        for (int m = 0; m < kH / 32; m++)
 #pragma unroll
             for (int nt = 0; nt < kH / 16; nt++) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sw2[nt * 16 + col][kk * 32 + kq]);
-                z2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, z2[nt], 0, 0, 0);
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sw2[nt * 16 + c][32 * m + kq]);
+                z2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h1[m], z2[nt], 0, 0, 0);
             }
+        bf16x8 h2[kH / 32];
+#pragma unroll
+        for (int m = 0; m < kH / 32; m++) {
+            const float4 b0 = *reinterpret_cast<const float4*>(&sb[kH + 32 * m + 4 * q]);
+            const float4 b1 = *reinterpret_cast<const float4*>(&sb[kH + 32 * m + 16 + 4 * q]);
+            const f32x4 z0 = z2[2 * m], z1 = z2[2 * m + 1];
+            h2[m] = bf16x8{ (__bf16)silu_fast(z0[0] + b0.x), (__bf16)silu_fast(z0[1] + b0.y),
+                            (__bf16)silu_fast(z0[2] + b0.z), (__bf16)silu_fast(z0[3] + b0.w),
+                            (__bf16)silu_fast(z1[0] + b1.x), (__bf16)silu_fast(z1[1] + b1.y),
+                            (__bf16)silu_fast(z1[2] + b1.z), (__bf16)silu_fast(z1[3] + b1.w) };
         }
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int nt = 0; nt < kH / 16; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) sh[wv][rq + j][nt * 16 + col] = (__bf16)silu_fast(z2[nt][j] + sb[kH + nt * 16 + col]);
-        __builtin_amdgcn_wave_barrier();
-        // layer 3: one padded column tile x 4 K steps
+        // layer 3: one padded output tile x 4 K steps; lanes q = 0 hold outputs 0..3 of row c
         f32x4 y = { 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
-        for (int kk = 0; kk < kH / 32; kk++) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sh[wv][col][kk * 32 + kq]);
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sw3[col][kk * 32 + kq]);
-            y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, y, 0, 0, 0);
+        for (int m = 0; m < kH / 32; m++) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sw3[c][32 * m + kq]);
+            y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h2[m], y, 0, 0, 0);
         }
-        // lanes with col < 3 hold output component `col` of rows rq..rq+3
-        if (col < kOut) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int gr = r0 + rq + j;
-                if (gr < n) {
-                    const float v = scale * (y[j] + sb[2 * kH + col]) * (mask ? mask[gr] : 1.f);
-                    reinterpret_cast<float*>(out + gr)[col] = v;
-                }
-            }
+        if (q == 0 && r < n) {
+            const float mk = mask ? mask[r] : 1.f;
+            out[r] = make_float4(scale * (y[0] + sb[2 * kH]) * mk, scale * (y[1] + sb[2 * kH + 1]) * mk,
+                                 scale * (y[2] + sb[2 * kH + 2]) * mk, 0.f);
         }
-        if (col == kOut) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int gr = r0 + rq + j;
-                if (gr < n) reinterpret_cast<float*>(out + gr)[3] = 0.f;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
