@@ -423,9 +423,9 @@ int xpbd_attach_mlp(xpbd_handle* h, xpbd_mlp* m);
  * *residual the final ||r||_2.  ref_residual <= 0 disables the fallback.
  * Everything up to the fallback decision (and the rest of the step, which a
  * fallback redoes) replays one captured graph per network, iteration count,
- * start mode and detection, and the fallback replays one more; in kernel
- * timing mode both launch eagerly and the network is timed with events
- * (xpbd_learned_stats).  The same bits either way. */
+ * start mode and detection; in kernel timing mode the step launches eagerly
+ * and times the network with events (xpbd_learned_stats).  The same bits
+ * either way. */
 int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_residual, double fallback_ratio,
                       int32_t* fell_back, double* residual);
 

@@ -1021,11 +1021,6 @@ struct Learned {
         hipGraphExec_t exec = nullptr;
     };
     std::vector<Graph> graphs;
-    // the captured fallback (snapshot restore and the plain step), for the
-    // handle's graph generation and iteration count it was captured at
-    hipGraphExec_t fb_exec = nullptr;
-    uint64_t fb_gen = 0;
-    int fb_iters = -1;
     // inference timing (kernel timing mode of the handle)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double mlp_ms = 0;
@@ -1040,7 +1035,6 @@ static void free_learned(Learned* L)
     if (!L) return;
     for (auto& g : L->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (L->fb_exec) (void)hipGraphExecDestroy(L->fb_exec);
     void* bufs[] = { L->d_nb_off, L->d_nb, L->d_vmat, L->d_pinfrac, L->d_feat, L->d_pred,
                      L->d_cprev,  L->d_corr, L->d_save, L->d_step_save, L->d_res };
     for (void* b : bufs)
@@ -1340,65 +1334,6 @@ static int launch_learned_attempt(xpbd_handle* h, int32_t iters, bool det)
     hipError_t e = hipGraphLaunch(exec, h->stream);
     return e == hipSuccess ? XPBD_OK : hip_err(h, e, "hipGraphLaunch");
 }
-
-// The fallback: the snapshot restored, then the step as the plain solver takes
-// it, without a new detection (the attempt detected from the same start, so its
-// contacts are the step's), its end-of-solve squared residual into L->d_res.
-static int enqueue_learned_fallback(xpbd_handle* h)
-{
-    Learned* L = h->learned;
-    const int nv = h->nv;
-    hipStream_t s = h->stream;
-    hipError_t e = hipMemcpyAsync(h->d_pos, L->d_save, sizeof(float4) * 2 * (size_t)nv, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(h->d_lo, L->d_save + 2 * (size_t)nv, sizeof(float4) * nv, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(h->d_vel, L->d_save + 3 * (size_t)nv, sizeof(float4) * nv, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(h->d_step, L->d_step_save, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return hip_err(h, e, "learned restore");
-    int rc;
-    if ((rc = enqueue_predict(h)) || (rc = enqueue_copy_pred(h))) return rc;
-    if ((rc = enqueue_iterations(h, h->params.iters, false))) return rc;
-    e = launch_residual(s, tet_arrays(h), h->d_pos, h->d_lo, true, h->nt, h->d_part, h->red_blocks, nullptr);
-    if (e == hipSuccess) e = launch_sum_partials(s, h->d_part, h->red_blocks, L->d_res);
-    if (e != hipSuccess) return hip_err(h, e, "fallback residual");
-    if (records_residuals(h) && (rc = enqueue_step_residual(h))) return rc;
-    if ((rc = enqueue_local_collision(h))) return rc;
-    return (rc = enqueue_store_cprev(h)) ? rc : enqueue_finish(h);
-}
-
-// The fallback from its captured graph (captured again when the handle's
-// captured steps or the iteration count changed).
-static int launch_learned_fallback(xpbd_handle* h)
-{
-    Learned* L = h->learned;
-    if (L->fb_exec && (L->fb_gen != h->graph_gen || L->fb_iters != h->params.iters)) {
-        (void)hipGraphExecDestroy(L->fb_exec);
-        L->fb_exec = nullptr;
-    }
-    if (!L->fb_exec) {
-        hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return hip_err(h, e, "hipStreamBeginCapture");
-        int rc = enqueue_learned_fallback(h);
-        hipGraph_t g = nullptr;
-        e = hipStreamEndCapture(h->stream, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) return hip_err(h, e, "hipStreamEndCapture");
-        hipGraphExec_t exec = nullptr;
-        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return hip_err(h, e, "hipGraphInstantiate");
-        L->fb_exec = exec;
-        L->fb_gen = h->graph_gen;
-        L->fb_iters = h->params.iters;
-    }
-    hipError_t e = hipGraphLaunch(L->fb_exec, h->stream);
-    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "hipGraphLaunch");
-}
 }  // namespace xpbd
 
 extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_residual, double fallback_ratio,
@@ -1409,6 +1344,7 @@ extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_resid
         return set_err(h, XPBD_E_INVALID_ARG, "step_learned: iters >= 0 and finite thresholds required");
     if (!h->learned || !h->learned->mlp) return set_err(h, XPBD_E_STATE, "no network attached");
     Learned* L = h->learned;
+    const int nv = h->nv;
     hipStream_t s = h->stream;
     const bool det = detect_this_step(h);
     // kernel timing mode launches eagerly and times the inference with events
@@ -1427,13 +1363,23 @@ extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_resid
     }
     const bool fb = ref_residual > 0 && fallback_ratio > 0 && !(r <= fallback_ratio * ref_residual);
     if (fb) {
-        // redo the step from the snapshot as the plain solver does; its residual is the step's
-        rc = h->timing ? enqueue_learned_fallback(h) : launch_learned_fallback(h);
-        if (rc) return rc;
-        e = hipMemcpyAsync(&r2, L->d_res, sizeof(double), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        // redo the step from the snapshot as the plain solver does
+        e = hipMemcpyAsync(h->d_pos, L->d_save, sizeof(float4) * 2 * (size_t)nv, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h->d_lo, L->d_save + 2 * (size_t)nv, sizeof(float4) * nv, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h->d_vel, L->d_save + 3 * (size_t)nv, sizeof(float4) * nv, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h->d_step, L->d_step_save, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_err(h, e, "learned restore");
+        if ((rc = enqueue_predict(h)) || (rc = enqueue_copy_pred(h))) return rc;
+        if ((rc = enqueue_iterations(h, h->params.iters, false))) return rc;
+        if ((rc = device_residual(h, &r, nullptr))) return rc;
+        if (records_residuals(h) && (rc = enqueue_step_residual(h))) return rc;
+        if ((rc = enqueue_local_collision(h))) return rc;
+        if ((rc = enqueue_store_cprev(h)) || (rc = enqueue_finish(h))) return rc;
+        e = hipStreamSynchronize(s);
         if (e != hipSuccess) return hip_err(h, e, "learned fallback");
-        r = std::sqrt(r2);
         L->fallbacks++;
     }
     h->host_step++;
