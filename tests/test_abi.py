@@ -65,3 +65,57 @@ def test_integration_stub_matches_the_header(tmp_path, struct):
     names = [n for n, _ in cls._fields_]
     assert names == [n for n, _ in BINDINGS[struct]._fields_]
     assert _c_layout(tmp_path, struct, names) == _ctypes_layout(cls)
+
+
+# C parameter types of the header -> the ctypes type that passes them
+_CTYPES = {
+    "int32_t": C.c_int32, "int64_t": C.c_int64, "uint64_t": C.c_uint64, "double": C.c_double, "float": C.c_float,
+    "int": C.c_int,
+    "int32_t*": C.POINTER(C.c_int32), "int64_t*": C.POINTER(C.c_int64), "double*": C.POINTER(C.c_double),
+    "float*": C.POINTER(C.c_float), "uint8_t*": C.POINTER(C.c_uint8), "char*": C.c_char_p,
+    "xpbd_handle*": C.c_void_p, "xpbd_batch*": C.c_void_p, "xpbd_mlp*": C.c_void_p,
+    "xpbd_handle**": C.POINTER(C.c_void_p), "xpbd_batch**": C.POINTER(C.c_void_p), "xpbd_mlp**": C.POINTER(C.c_void_p),
+    "xpbd_mesh_desc*": C.POINTER(X.MeshDesc), "xpbd_params*": C.POINTER(X.Params),
+}
+
+
+def _header_prototypes():
+    text = open(os.path.join(ROOT, "include", "xpbd_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    protos = {}
+    for ret, name, args in re.findall(r"([A-Za-z_][\w\s\*]*?)\b(xpbd_\w+)\s*\(([^;{]*?)\)\s*;", text):
+        def canon(decl):
+            decl = re.sub(r"\bconst\b", " ", decl)
+            stars = decl.count("*")
+            words = decl.replace("*", " ").split()
+            return words[0] + "*" * stars if words else ""
+        # each parameter: its type, then its name
+        params = [] if args.strip() in ("", "void") else [canon(re.match(r"(.*?)\w+\s*$", a.strip()).group(1))
+                                                           for a in args.split(",")]
+        # the return type: the stars and the last word before the name
+        toks = ret.replace("*", " * ").split()
+        stars, word = 0, ""
+        for t in reversed(toks):
+            if t == "*":
+                stars += 1
+            elif t != "const":
+                word = t
+                break
+        protos[name] = (word + "*" * stars, params)
+    return protos
+
+
+def test_binding_signatures_match_the_header():
+    """Every function the binding registers has the header's parameter list
+    (count and C types) and return type."""
+    protos = _header_prototypes()
+    checked = 0
+    for name, (ret, params) in sorted(protos.items()):
+        fn = getattr(X.lib, name, None)
+        if fn is None or fn.argtypes is None:
+            continue
+        assert [_CTYPES[p] for p in params] == list(fn.argtypes), name
+        assert fn.restype == (C.c_char_p if ret == "char*" else _CTYPES[ret]), name
+        checked += 1
+    assert checked >= 60, checked
