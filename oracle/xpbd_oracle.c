@@ -1342,6 +1342,7 @@ ORC_API void orc_step_dual_warm(orc_sim* s, int detect, const double* lam0, int 
         memset(s->lam, 0, sizeof(double) * 2 * (size_t)s->nt);
     }
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
+    for (int a = 0; a < s->na; a++) s->att_lam[a] = 0;
     for (int it = 0; it < iters; it++) iterate(s);
     s->last_res = orc_residual(s, NULL); record_primary(s);
     local_collision_iterations(s);
@@ -1389,6 +1390,7 @@ ORC_API void orc_step_learned(orc_sim* s, int detect, const double* corr, int it
     for (int i = 0; i < 3 * nv; i++) s->x[i] += dx[i];
     free(dx);
     for (int c = 0; c < s->nc; c++) s->c_lam[c] = 0;
+    for (int a = 0; a < s->na; a++) s->att_lam[a] = 0;
     for (int it = 0; it < iters; it++) iterate(s);
     s->last_res = orc_residual(s, NULL); record_primary(s);
     local_collision_iterations(s);

@@ -1012,6 +1012,84 @@ def test_primary_residual_stiff_block_and_pins():
     assert n > 0 and np.all(R[_mesh(3).pinned.astype(bool)] == 0)
 
 
+def _moving_state(m, steps=3):
+    """Positions and velocities after a few plain steps of m."""
+    s = orc.Sim(m, iters=5, mode="gs")
+    s.step(steps)
+    return s.x(), s.v()
+
+
+def test_consistent_learned_start_without_correction_is_the_dual_start_from_its_multipliers():
+    """orc_step_learned with a zero correction starts from lambda0 = -C(xt) / alpha~
+    and x0 = xt + M^-1 grad C(xt)^T lambda0: exactly orc_step_dual_warm given
+    those multipliers (computed here from orc_eval_tet at the same predicted
+    positions), bit for bit, with or without iterations."""
+    m = _mesh(3, E=3e3)
+    x, v = _moving_state(m)
+    dt, g = 1e-3, 9.81
+    xt = x + dt * v
+    xt[:, 2] += -g * dt * dt
+    ref = orc.Sim(m, iters=4, mode="gs")
+    Q9, _ = ref.rest()
+    aD, aH, gam = ref.compliance()
+    dt2 = dt * dt
+    lam0 = np.empty((m.nt, 2))
+    for e, t in enumerate(m.tets):
+        C2, _ = orc.eval_tet(xt[t].reshape(12), Q9[e].reshape(3, 3).T, gam[e])
+        lam0[e] = (-C2[0] / (aD[e] / dt2), -C2[1] / (aH[e] / dt2))
+    for iters in (0, 4):
+        a, b = orc.Sim(m, iters=4, mode="gs"), orc.Sim(m, iters=4, mode="gs")
+        for s in (a, b):
+            s.set_state(x, v)
+        a.step_learned(None, iters)
+        b.step_dual_warm(lam0, iters)
+        assert np.array_equal(a.x(), b.x()) and np.array_equal(a.lambdas(), b.lambdas()), iters
+
+
+def test_consistent_learned_start_from_the_exact_correction_is_the_solution():
+    """Given the exact solution (x*, lambda*) of the step's equations
+        M (x - xt) = grad C(x)^T lambda,   C(x) + alpha~ lambda = 0
+    for a single free element (solved here by scipy's root finder on
+    orc_eval_tet), the consistent start from the correction x* - xt with no
+    iterations lands on x*: lambda0 = -C(x*) / alpha~ = lambda* and
+    xt + M^-1 grad C(x*)^T lambda0 = x*.  Gradients taken at xt instead of
+    xt + corr, or the shift added to xt + corr instead of xt, miss x* by the
+    size of the correction."""
+    from scipy.optimize import root
+    import xpbd_amd as X
+    x0 = np.array([[0.0, 0.0, 0.0], [0.01, 0.0, 0.0], [0.0, 0.01, 0.0], [0.0, 0.0, 0.01]])
+    v0 = np.array([[0.0, 0.0, 0.0], [0.9, 0.0, 0.0], [0.0, -0.6, 0.0], [0.3, 0.2, 0.7]])
+    mesh = X.Mesh(x0=x0, tets=np.array([[0, 1, 2, 3]], np.int32), E=np.array([3e3]), nu=np.array([0.45]),
+                  rho=np.array([1000.0]), v0=v0)
+    dt = 1e-3
+    s = orc.Sim(mesh, dt=dt, g=0.0, iters=1, mode="gs")
+    m = s.masses()
+    aD, aH, gam = s.compliance()
+    Q = s.rest()[0][0].reshape(3, 3).T
+    at = np.array([aD[0], aH[0]]) / (dt * dt)
+    xt = x0 + dt * v0
+
+    def eqs(z):
+        x, lam = z[:12].reshape(4, 3), z[12:]
+        C2, g = orc.eval_tet(x.reshape(12), Q, gam[0])
+        grad = g.reshape(2, 12)
+        r1 = (m[:, None] * (x - xt)).reshape(12) - grad.T @ lam
+        r2 = C2 + at * lam
+        return np.concatenate([r1 / m.max(), r2])
+
+    sol = root(eqs, np.concatenate([xt.reshape(12), np.zeros(2)]), method="hybr", tol=1e-14)
+    assert sol.success and np.abs(eqs(sol.x)).max() < 1e-12
+    xstar = sol.x[:12].reshape(4, 3)
+    corr = xstar - xt
+    s.step_learned(corr, 0)
+    err = np.abs(s.x() - xstar).max()
+    size = np.abs(corr).max()
+    print(f"consistent start from the exact correction: |x0 - x*| {err:.2e} against |x* - xt| {size:.2e}")
+    assert size > 1e-6
+    assert err < 1e-9 * np.abs(xstar).max()
+    np.testing.assert_allclose(s.lambdas()[0], sol.x[12:], rtol=1e-6)
+
+
 def test_even_the_exact_multipliers_save_few_iterations():
     """An upper bound on any warm start that keeps the momentum balance: from
     the exact multipliers of the plain 20-iteration solve, with positions
