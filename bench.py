@@ -28,6 +28,7 @@ the plain solver on the same workload.
 """
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -280,11 +281,14 @@ def learned_vs_plain(a, net_path):
       positions_start  network correction, multipliers from zero;
       positions_start_no_fallback  the same, never falling back (its residual
                        is reported, not judged);
+      consistent_start  network correction, multipliers -C / alpha~ there and
+                       positions that keep the momentum balance;
+      consistent_start_no_fallback  the same, never falling back;
     and plain_same_iterations is the plain solver with a.learned_iters
     iterations (no network, no fallback), for scale.  Each run reports its
     trajectory error against the plain run after the last step, as
     max |x - x_plain| / max |x_plain| and in units of the plain run's largest
-    displacement from rest.  primary_residual repeats the plain run and three
+    displacement from rest.  primary_residual repeats the plain run and four
     of the others untimed with the primary residual (the momentum balance
     (x - xt) / w - sum grad C^T lambda) recorded every step, and reports its
     norm against the plain run's, step by step.  The learned runs replay captured steps; the
@@ -326,24 +330,32 @@ def learned_vs_plain(a, net_path):
     for mode, network, start, judged in (("with_fallback", net, "multipliers", True),
                                          ("no_correction", zero, "multipliers", True),
                                          ("positions_start", net, "positions", True),
-                                         ("positions_start_no_fallback", net, "positions", False)):
+                                         ("positions_start_no_fallback", net, "positions", False),
+                                         ("consistent_start", net, "consistent", True),
+                                         ("consistent_start_no_fallback", net, "consistent", False)):
         s, _, _ = make(a.iters)
         s.attach(network)
         s.set_learned_start(start)
-        ratios, fb, t = [], 0, 0.0
+        ratios, fb, t, diverged = [], 0, 0.0, None
         for n in range(a.learned_steps):
             t0 = time.perf_counter()
             f, r = s.step_learned(a.learned_iters, r_plain[n] if judged else 0.0, 1.1)
             t += time.perf_counter() - t0
+            if not np.isfinite(r):   # the run left the finite numbers: report where, stop
+                diverged = n
+                break
             fb += int(f)
             ratios.append(r / max(r_plain[n], 1e-300))
-        x = s.positions()
-        ratios = np.array(ratios)
-        dx = float(np.max(np.abs(x - x_plain)))
-        out[mode] = {"start": start, "steps_per_s": a.learned_steps / t, "fallback_rate": fb / a.learned_steps,
-                     "residual_ratio_median": float(np.median(ratios)), "residual_ratio_max": float(ratios.max()),
-                     "rel_pos_err_vs_plain_end": dx / float(np.max(np.abs(x_plain))),
-                     "traj_err_vs_plain_end_over_displacement": dx / disp}
+        taken = len(ratios)
+        ratios = np.array(ratios) if taken else np.full(1, np.nan)
+        out[mode] = {"start": start, "steps_per_s": max(taken, 1) / t, "fallback_rate": fb / max(taken, 1),
+                     "residual_ratio_median": float(np.median(ratios)), "residual_ratio_max": float(ratios.max())}
+        if diverged is None:
+            dx = float(np.max(np.abs(s.positions() - x_plain)))
+            out[mode]["rel_pos_err_vs_plain_end"] = dx / float(np.max(np.abs(x_plain)))
+            out[mode]["traj_err_vs_plain_end_over_displacement"] = dx / disp
+        else:
+            out[mode]["diverged_at_step"] = diverged
         if mode == "with_fallback":
             # inference time: a few more steps, launched eagerly with HIP events around the network
             s.set_kernel_timing(True)
@@ -389,7 +401,10 @@ def learned_vs_plain(a, net_path):
                 s_.step(1)
             else:
                 s_.step_learned(a.learned_iters, r_plain[n] if judged else 0.0, 1.1)
-            p.append(s_.last_step_primary_residual())
+            v = s_.last_step_primary_residual()
+            if not np.isfinite(v):
+                break
+            p.append(v)
         s_.close()
         return np.array(p)
 
@@ -397,12 +412,27 @@ def learned_vs_plain(a, net_path):
     pr = {"plain_median": float(np.median(p_plain))}
     for key, args in (("with_fallback", (a.iters, net, "multipliers", True)),
                       ("positions_start_no_fallback", (a.iters, net, "positions", False)),
+                      ("consistent_start_no_fallback", (a.iters, net, "consistent", False)),
                       ("plain_same_iterations", (a.learned_iters,))):
-        q = primary_norms(*args) / np.maximum(p_plain, 1e-300)
-        pr[key + "_over_plain_median"] = float(np.median(q))
-        pr[key + "_over_plain_max"] = float(np.max(q))
+        q = primary_norms(*args)
+        q = q / np.maximum(p_plain[:len(q)], 1e-300)
+        if len(q) < len(p_plain):
+            pr[key + "_diverged_at_step"] = len(q)
+        pr[key + "_over_plain_median"] = float(np.median(q)) if len(q) else None
+        pr[key + "_over_plain_max"] = float(np.max(q)) if len(q) else None
     out["primary_residual"] = pr
     return out
+
+
+def _finite(o):
+    """The record with every non-finite number replaced by null (strict JSON)."""
+    if isinstance(o, dict):
+        return {k: _finite(v) for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [_finite(v) for v in o]
+    if isinstance(o, float) and not math.isfinite(o):
+        return None
+    return o
 
 
 def main():
@@ -462,7 +492,7 @@ def main():
         if a.workload != "batch":
             # second reported baseline: the same oracle on all host cores
             out["cpu_baseline_parallel"] = cpu_baseline_parallel(meshes, cfg, sdf, 4 * n, start)
-    print(json.dumps(out))
+    print(json.dumps(_finite(out), allow_nan=False))
 
 
 if __name__ == "__main__":

@@ -440,9 +440,18 @@ int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_residual, double
  *   rounding of the FP32 multipliers whatever the correction, so the
  *   residual test no longer judges the correction: the trajectory is as
  *   accurate as the network (the positions are not moved by
- *   M^-1 grad C^T lambda, the implicit-Euler momentum balance). */
+ *   M^-1 grad C^T lambda, the implicit-Euler momentum balance).
+ * CONSISTENT: every element's multipliers start at lambda0 = -C / alpha~ at
+ *   the corrected positions xt + corr (C and its gradients in FP64), and the
+ *   positions at xt + M^-1 sum grad C(xt + corr)^T lambda0, so the momentum
+ *   balance M (x - xt) = grad C^T lambda that the XPBD updates preserve holds
+ *   from the first iteration; contact and attachment multipliers start at
+ *   zero.  From the exact correction this start is the step's solution.  The
+ *   first call with this mode builds the vertex -> element slot lists (16
+ *   bytes and 12 doubles per element). */
 #define XPBD_LEARNED_START_POSITIONS 0
 #define XPBD_LEARNED_START_MULTIPLIERS 1
+#define XPBD_LEARNED_START_CONSISTENT 2
 int xpbd_set_learned_start(xpbd_handle* h, int32_t mode);
 /* Counters of the learned steps on h: steps taken, fallbacks, and (with kernel
  * timing on) the summed milliseconds and number of network inferences. */

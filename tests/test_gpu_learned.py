@@ -139,11 +139,15 @@ def test_features_match_oracle():
 
 @pytest.mark.parametrize("solver,variant,start", [("gs", "cluster", "positions"), ("jacobi", "patch", "positions"),
                                                   ("gs", "cluster", "multipliers"), ("gs", "quad", "multipliers"),
-                                                  ("jacobi", "patch", "multipliers")])
+                                                  ("jacobi", "patch", "multipliers"), ("gs", "quad", "consistent"),
+                                                  ("gs", "cluster", "consistent"), ("jacobi", "patch", "consistent")])
 def test_learned_step_matches_oracle_from_the_same_corrected_start(solver, variant, start):
     """The device's learned step against the oracle's from the same network
     correction: 'positions' starts every multiplier at zero (orc_step_corrected),
-    'multipliers' at -C / alpha~ of the corrected positions (orc_step_warm)."""
+    'multipliers' at -C / alpha~ of the corrected positions (orc_step_warm),
+    'consistent' at those multipliers with the positions moved by
+    M^-1 grad C^T lambda0 from the prediction (orc_step_learned), whose primary
+    residual (momentum balance) is compared too."""
     mesh = W.cube(6, seed=8, E=3e3 if solver == "gs" else 1e3)
     iters = 10
     gpu = X.Solver(mesh, dt=1e-3, iters=iters, solver=solver, variant=variant)
@@ -154,6 +158,9 @@ def test_learned_step_matches_oracle_from_the_same_corrected_start(solver, varia
     net.set_scaling(np.zeros(24), np.full(24, 1e-3), 2e-6)
     gpu.attach(net)
     gpu.set_learned_start(start)
+    if start == "consistent":
+        gpu.set_primary_residual(True)
+        o.set_primary_residual(True)
     for _ in range(3):
         gpu.step(1)
         o.step(1)
@@ -165,13 +172,20 @@ def test_learned_step_matches_oracle_from_the_same_corrected_start(solver, varia
         assert not fb
         if start == "positions":
             o.step_corrected(corr, 3)
+        elif start == "consistent":
+            o.step_learned(corr, 3)
         else:
             o.step_warm(corr, 3)
         err = rel_err(gpu.positions(), o.x())
         ro = o.last_step_residual()
         print(f"{solver} {start} learned step {k}: rel err {err:.2e}, residual {r:.3e} (oracle {ro:.3e})")
         assert err <= 1e-5
-        if start == "positions":
+        if start == "consistent":
+            pg, po = gpu.last_step_primary_residual(), o.last_primary_residual()
+            print(f"  primary residual {pg:.4e} (oracle {po:.4e})")
+            assert abs(pg - po) <= 1e-2 * po
+            assert abs(r - ro) <= 1e-2 * ro + 1e-9
+        elif start == "positions":
             assert abs(r - ro) <= 1e-3 * ro + 1e-9
         else:
             # both start at zero elastic residual; what remains is rounding of the

@@ -1276,13 +1276,9 @@ int enqueue_step_residual(xpbd_handle* h)
         if (e == hipSuccess) e = launch_sum_partials(h->stream, h->d_part, h->red_blocks, h->d_step_res);
     }
     if (e == hipSuccess && h->primary_residual) {
-        PrimaryArgs A;
-        A.pos = h->d_pos;
-        A.lo = h->d_lo;
+        PrimaryArgs A = element_force_args(h);
         A.vel = h->d_vel;
         A.pred = h->d_pred;
-        A.vt_off = h->d_vt_off;
-        A.vt_slot = h->d_vt_slot;
         A.vf_off = h->nf > 0 ? h->d_vf_off : nullptr;
         A.vf = h->d_vf;
         A.flag = h->d_flag;
@@ -1291,9 +1287,6 @@ int enqueue_step_residual(xpbd_handle* h)
         A.clam = h->d_clam;
         A.att_v = h->d_att_v;
         A.natt = h->natt;
-        A.nv = h->nv;
-        A.nt = h->nt;
-        A.ef = h->d_eforce;
         A.R = h->d_pres;
         A.part = h->d_part;
         A.nblocks = h->red_blocks;
@@ -1302,27 +1295,26 @@ int enqueue_step_residual(xpbd_handle* h)
     return e == hipSuccess ? XPBD_OK : hip_err(h, e, "step residual");
 }
 
-// Buffers of the primary residual: the vertex lists it gathers over, the kept
-// prediction, the element forces, the residual vector and its norm (zero
-// until the first step).
-static int ensure_primary_residual(xpbd_handle* h)
+// The vertex -> element list with a slot per (element, vertex) pair and the
+// forces buffer over those slots: what k_element_forces writes and the
+// per-vertex gathers (primary residual, consistent learned start) read.
+int ensure_vertex_slots(xpbd_handle* h)
 {
     hipError_t e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return hip_err(h, e, "residual policy");
+    if (e != hipSuccess) return hip_err(h, e, "vertex slots");
     int rc = ensure_vertex_elements(h, false);
-    if (!rc) rc = ensure_vertex_faces(h);
     if (rc) return rc;
     if (!h->d_vt_slot) {
         // the slot of (solver position i, corner j) in the vertex -> element list,
         // filled in the order that list was built (positions ascending, corners in order)
         const int nv = h->nv, nt = h->nt;
         std::vector<int32_t> fill((size_t)nv + 1, 0), slot(4 * (size_t)nt);
-        for (int e = 0; e < nt; e++)
-            for (int j = 0; j < 4; j++) fill[h->tets_host[4 * (size_t)e + j] + 1]++;
+        for (int t = 0; t < nt; t++)
+            for (int j = 0; j < 4; j++) fill[h->tets_host[4 * (size_t)t + j] + 1]++;
         for (int v = 0; v < nv; v++) fill[v + 1] += fill[v];
         for (int i = 0; i < nt; i++) {
-            const int e = h->order[i];
-            for (int j = 0; j < 4; j++) slot[4 * (size_t)i + j] = fill[h->tets_host[4 * (size_t)e + j]]++;
+            const int t = h->order[i];
+            for (int j = 0; j < 4; j++) slot[4 * (size_t)i + j] = fill[h->tets_host[4 * (size_t)t + j]]++;
         }
         int32_t* d = nullptr;
         if ((rc = dalloc(h, &d, slot.size()))) return rc;
@@ -1330,15 +1322,38 @@ static int ensure_primary_residual(xpbd_handle* h)
         if (e != hipSuccess) return hip_err(h, e, "vertex-element slots");
         h->d_vt_slot = d;
     }
+    if (!h->d_eforce) return dalloc(h, &h->d_eforce, 3 * 4 * (size_t)h->nt);
+    return XPBD_OK;
+}
+
+PrimaryArgs element_force_args(xpbd_handle* h)
+{
+    PrimaryArgs A{};
+    A.pos = h->d_pos;
+    A.lo = h->d_lo;
+    A.vt_off = h->d_vt_off;
+    A.vt_slot = h->d_vt_slot;
+    A.nt = h->nt;
+    A.nv = h->nv;
+    A.ef = h->d_eforce;
+    return A;
+}
+
+// Buffers of the primary residual: the vertex lists it gathers over, the kept
+// prediction, the residual vector and its norm (zero until the first step).
+static int ensure_primary_residual(xpbd_handle* h)
+{
+    int rc = ensure_vertex_slots(h);
+    if (!rc) rc = ensure_vertex_faces(h);
+    if (rc) return rc;
     if (!h->d_pred) {
         float4* pred = nullptr;
-        double *pres = nullptr, *norm = nullptr, *ef = nullptr;
+        double *pres = nullptr, *norm = nullptr;
         rc = dalloc(h, &pred, (size_t)h->nv);
         if (!rc) rc = dalloc(h, &pres, 3 * (size_t)h->nv);
         if (!rc) rc = dalloc(h, &norm, 1);
-        if (!rc) rc = dalloc(h, &ef, 3 * 4 * (size_t)h->nt);
         if (rc) return rc;
-        e = launch_fill_zero(h->stream, pred, sizeof(float4) * (size_t)h->nv);
+        hipError_t e = launch_fill_zero(h->stream, pred, sizeof(float4) * (size_t)h->nv);
         if (e == hipSuccess) e = launch_fill_zero(h->stream, pres, sizeof(double) * 3 * (size_t)h->nv);
         if (e == hipSuccess) e = launch_fill_zero(h->stream, norm, sizeof(double));
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1346,7 +1361,6 @@ static int ensure_primary_residual(xpbd_handle* h)
         h->d_pred = pred;
         h->d_pres = pres;
         h->d_step_pres = norm;
-        h->d_eforce = ef;
     }
     return XPBD_OK;
 }

@@ -171,6 +171,9 @@ int device_residual(xpbd_handle* h, double* norm, double* vec_solver_order_dev);
 // residual policy, primary residual when switched on)
 int enqueue_step_residual(xpbd_handle* h);
 inline bool records_residuals(const xpbd_handle* h) { return h->residual_policy || h->primary_residual; }
+// the vertex -> element slots and the forces buffer over them (built on first use)
+int ensure_vertex_slots(xpbd_handle* h);
+PrimaryArgs element_force_args(xpbd_handle* h);
 void destroy_learned(xpbd_handle* h);
 // Clears the previous-step correction the learned warm start reads (state was replaced).
 int learned_reset_history(xpbd_handle* h);

@@ -168,6 +168,12 @@ struct PrimaryArgs {
 hipError_t launch_keep_prediction(hipStream_t s, const float4* pos, float4* pred, int nv);
 // R into A.R and ||R||^2 into *out
 hipError_t launch_primary_residual(hipStream_t s, const TetArrays& ta, const PrimaryArgs& A, double* out);
+// grad C^T lambda of every element at its four vertices into A.ef (the
+// vertex-ordered slots of A.vt_slot), at the current positions and multipliers
+hipError_t launch_element_forces(hipStream_t s, const TetArrays& ta, const PrimaryArgs& A);
+// consistent start: in-step motion = pred + w * (sum of the vertex's slots of ef)
+hipError_t launch_consistent_shift(hipStream_t s, float4* pos, const float4* pred, const int* vt_off,
+                                   const double* ef, int nv);
 
 // Contacts.  faces: (a, b, c, instance) in colour-sorted order.
 hipError_t launch_detect(hipStream_t s, const int4* faces, int nf, const double* rest, const float4* pos,

@@ -1173,6 +1173,25 @@ static int enqueue_multiplier_start(xpbd_handle* h)
     return e == hipSuccess ? XPBD_OK : hip_err(h, e, "multiplier start");
 }
 
+// Consistent start (XPBD_LEARNED_START_CONSISTENT, the oracle's orc_step_learned):
+// the multipliers lambda0 = -C / alpha~ at the corrected positions xl = xt + corr,
+// then the positions x0 = xt + M^-1 sum grad C(xl)^T lambda0 from the kept
+// prediction, so that the momentum balance M (x - xt) = grad C^T lambda, which
+// the XPBD updates preserve, holds at the first iteration.  Contact and
+// attachment multipliers start at zero, as in a plain solve.
+static int enqueue_consistent_start(xpbd_handle* h)
+{
+    hipStream_t s = h->stream;
+    hipError_t e = hipSuccess;
+    if (h->natt > 0) e = launch_fill_zero(s, h->d_att_lam, sizeof(double) * (size_t)h->natt);
+    if (e == hipSuccess && h->nf > 0) e = launch_fill_zero(s, h->d_clam, sizeof(double) * (size_t)h->nf);
+    const TetArrays ta = tet_arrays(h);
+    if (e == hipSuccess) e = launch_lambda_start(s, ta, h->d_pos, h->d_lo, h->nt);
+    if (e == hipSuccess) e = launch_element_forces(s, ta, element_force_args(h));
+    if (e == hipSuccess) e = launch_consistent_shift(s, h->d_pos, h->learned->d_pred, h->d_vt_off, h->d_eforce, h->nv);
+    return e == hipSuccess ? XPBD_OK : hip_err(h, e, "consistent start");
+}
+
 }  // namespace xpbd
 
 using namespace xpbd;
@@ -1277,8 +1296,10 @@ static int enqueue_learned_attempt(xpbd_handle* h, int32_t iters, bool det, bool
     hipLaunchKernelGGL(k_apply_corr, dim3(nb(nv)), dim3(256), 0, s, h->d_pos, h->d_vel, L->d_corr, nv);
     if ((e = hipGetLastError()) != hipSuccess) return hip_err(h, e, "apply launch");
     const bool warm = L->start == XPBD_LEARNED_START_MULTIPLIERS;
+    const bool consistent = L->start == XPBD_LEARNED_START_CONSISTENT;
     if (warm && (rc = enqueue_multiplier_start(h))) return rc;
-    if ((rc = enqueue_iterations(h, iters, false, !warm))) return rc;
+    if (consistent && (rc = enqueue_consistent_start(h))) return rc;
+    if ((rc = enqueue_iterations(h, iters, false, !(warm || consistent)))) return rc;
     e = launch_residual(s, tet_arrays(h), h->d_pos, h->d_lo, true, h->nt, h->d_part, h->red_blocks, nullptr);
     if (e == hipSuccess) e = launch_sum_partials(s, h->d_part, h->red_blocks, L->d_res);
     if (e != hipSuccess) return hip_err(h, e, "learned residual");
@@ -1392,10 +1413,12 @@ extern "C" int xpbd_step_learned(xpbd_handle* h, int32_t iters, double ref_resid
 extern "C" int xpbd_set_learned_start(xpbd_handle* h, int32_t mode)
 {
     if (!h) return XPBD_E_INVALID_ARG;
-    if (mode != XPBD_LEARNED_START_POSITIONS && mode != XPBD_LEARNED_START_MULTIPLIERS)
+    if (mode != XPBD_LEARNED_START_POSITIONS && mode != XPBD_LEARNED_START_MULTIPLIERS &&
+        mode != XPBD_LEARNED_START_CONSISTENT)
         return set_err(h, XPBD_E_INVALID_ARG, "unknown learned start mode");
     int rc = ensure_learned(h);
     if (rc) return rc;
+    if (mode == XPBD_LEARNED_START_CONSISTENT && (rc = ensure_vertex_slots(h))) return rc;
     h->learned->start = mode;
     return XPBD_OK;
 }

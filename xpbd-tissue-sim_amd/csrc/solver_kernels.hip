@@ -1115,6 +1115,28 @@ __global__ __launch_bounds__(kBlock) void k_primary_residual(TetArrays ta, Prima
     for (int i = 0; i < 3; i++) A.R[3 * (size_t)v + i] = r[i];
 }
 
+// Consistent start of the learned step: the solve begins at
+//     x0 = xt + M^-1 sum_e grad C_e^T lambda0_e
+// from the kept prediction, with the forces of the multipliers lambda0 that
+// k_element_forces left in the vertex's slots, summed in slot order.  The
+// momentum balance M (x - xt) = grad C^T lambda then holds at the start.
+__global__ __launch_bounds__(kBlock) void k_consistent_shift(float4* __restrict__ pos, const float4* __restrict__ pred,
+                                                            const int* __restrict__ vt_off,
+                                                            const double* __restrict__ ef, int nv)
+{
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nv) return;
+    double f[3] = { 0.0, 0.0, 0.0 };
+    for (int k = vt_off[v]; k < vt_off[v + 1]; k++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) f[i] += ef[3 * (size_t)k + i];
+    }
+    const double w = (double)pos[2 * (size_t)v].w;
+    const float4 p = pred[v];
+    pos[2 * (size_t)v + 1] = make_float4((float)((double)p.x + w * f[0]), (float)((double)p.y + w * f[1]),
+                                         (float)((double)p.z + w * f[2]), 0.f);
+}
+
 __global__ __launch_bounds__(kBlock) void k_zero_attached(const int* __restrict__ att_v, int natt, int nv,
                                                          double* __restrict__ R)
 {
@@ -2421,6 +2443,21 @@ hipError_t launch_keep_prediction(hipStream_t s, const float4* pos, float4* pred
 {
     if (nv <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_keep_prediction, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos, pred, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_element_forces(hipStream_t s, const TetArrays& ta, const PrimaryArgs& A)
+{
+    if (A.nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_element_forces, dim3(nblk(A.nt, kBlock)), dim3(kBlock), 0, s, ta, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_consistent_shift(hipStream_t s, float4* pos, const float4* pred, const int* vt_off,
+                                   const double* ef, int nv)
+{
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_consistent_shift, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos, pred, vt_off, ef, nv);
     return hipGetLastError();
 }
 

@@ -503,8 +503,10 @@ class Solver:
 
     def set_learned_start(self, mode: str):
         """'multipliers' (default: element multipliers at -C / alpha~ at the
-        corrected positions) or 'positions' (multipliers from zero)."""
-        self._c(lib.xpbd_set_learned_start(self.h, {"positions": 0, "multipliers": 1}[mode]))
+        corrected positions), 'positions' (multipliers from zero) or
+        'consistent' (those multipliers with the positions that keep the
+        momentum balance, xt + M^-1 grad C^T lambda0)."""
+        self._c(lib.xpbd_set_learned_start(self.h, {"positions": 0, "multipliers": 1, "consistent": 2}[mode]))
 
     def step_learned(self, iters: int, ref_residual: float = 0.0, ratio: float = 1.1):
         """One learned step; returns (fell_back, residual)."""
