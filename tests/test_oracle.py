@@ -1090,6 +1090,52 @@ def test_consistent_learned_start_from_the_exact_correction_is_the_solution():
     np.testing.assert_allclose(s.lambdas()[0], sol.x[12:], rtol=1e-6)
 
 
+def test_consistent_start_from_the_plain_solution_saves_no_iterations():
+    """The consistent start keeps the momentum balance M (x - xt) = grad C^T lambda
+    exactly; the plain Gauss-Seidel solution does not (its primary residual is
+    the gradient change along the iterates, 17% of the inertial term here).  So
+    even from the plain solve's own correction, the consistent start lands off
+    that solution, stiff constraints turn the offset into a constraint residual
+    ~1e6x the plain one, and it needs at least 18 iterations to come back to
+    1.1x the plain 20-iteration residual (19-20 measured on this 8^3 block
+    pressed by a sphere, 23 on a 10^3 block).  No predictor can do better than
+    the plain solution's own correction."""
+    import workloads as W
+    n = 8
+    m = W.block(n)
+    L = 0.01 * n
+    sph = (orc.SDF_SPHERE, (L / 2, L / 2, L + 0.05 - 0.002), (0, 0, 0, 1), (0.05, 0, 0))
+    base = orc.Sim(m, iters=20, mode="gs")
+    base.set_sdf(*sph)
+    base.step(10, detect=True)
+    x, v = base.x(), base.v()
+    plain = orc.Sim(m, iters=20, mode="gs")
+    plain.set_sdf(*sph)
+    plain.set_state(x, v)
+    plain.set_primary_residual(True)
+    plain.step(1, detect=True)
+    rp, xs = plain.last_step_residual(), plain.x()
+    inertia = np.linalg.norm((plain.x() - plain.last_prediction()) / plain.inverse_inertia()[:, None])
+    dt, g = 1e-3, 9.81
+    xt = x + dt * v
+    xt[:, 2] += -g * dt * dt
+    corr = np.where(m.pinned.astype(bool)[:, None], 0.0, xs - xt)
+
+    def ratio(k):
+        s = orc.Sim(m, iters=20, mode="gs")
+        s.set_sdf(*sph)
+        s.set_state(x, v)
+        s.step_learned(corr, k, detect=True)
+        return s.last_step_residual() / rp
+
+    need = next((k for k in range(0, 26) if ratio(k) <= 1.1), None)
+    print(f"plain primary residual / inertial term {plain.last_primary_residual() / inertia:.3f}; consistent start "
+          f"from the plain correction: residual / plain {ratio(0):.3g} at 0 iterations, 1.1x reached at {need}")
+    assert plain.last_primary_residual() > 1e-2 * inertia
+    assert ratio(0) > 1e3
+    assert need is None or need >= 18
+
+
 def test_even_the_exact_multipliers_save_few_iterations():
     """An upper bound on any warm start that keeps the momentum balance: from
     the exact multipliers of the plain 20-iteration solve, with positions
