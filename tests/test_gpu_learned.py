@@ -261,6 +261,31 @@ def test_captured_learned_step_equals_the_eager_one(solver, variant):
     assert eager.learned_stats()["mlp_calls"] == 12
 
 
+def test_switching_learned_start_modes_replays_the_right_graph():
+    """One captured graph per start mode: a handle that switches modes from
+    step to step (positions, multipliers, consistent, in a cycle that returns
+    to each) takes the same bits as the same sequence launched eagerly."""
+    import train_warmstart as TW
+    mesh, sph = _held_out_block(seed=81, n=6)
+    graph, eager = (X.Solver(mesh, dt=1e-3, iters=6, solver="gs", variant="quad") for _ in range(2))
+    net = TW.load(TW.DEFAULT_WEIGHTS)
+    for s in (graph, eager):
+        s.set_sdf(*sph[:4])
+        s.set_sdf_velocity(sph[4])
+        s.attach(net)
+    eager.set_kernel_timing(True)
+    modes = ["positions", "multipliers", "consistent", "multipliers", "positions", "consistent", "consistent",
+             "positions", "multipliers"]
+    for k, mode in enumerate(modes):
+        for s in (graph, eager):
+            s.set_learned_start(mode)
+        fg, rg = graph.step_learned(3, ref_residual=0.0)
+        fe, re = eager.step_learned(3, ref_residual=0.0)
+        assert rg == re, (k, mode)
+        assert np.array_equal(graph.positions(), eager.positions()), (k, mode)
+        assert np.array_equal(graph.velocities(), eager.velocities()), (k, mode)
+
+
 def test_learned_step_requires_a_network():
     mesh = W.cube(4, seed=1)
     gpu = X.Solver(mesh, dt=1e-3, iters=5, solver="gs")
