@@ -1317,12 +1317,7 @@ ORC_API void orc_step_dual_warm(orc_sim* s, int detect, const double* lam0, int 
     if (detect) orc_detect(s);
     const int nv = s->nv;
     const double dt = s->dt;
-    memcpy(s->xp, s->x, sizeof(double) * 3 * nv);
-    for (int i = 0; i < nv; i++)
-        for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
-    const double dz = -s->g * dt * dt;
-    for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
-    keep_prediction(s);
+    move_inertially(s);
     if (lam0) {
         memcpy(s->lam, lam0, sizeof(double) * 2 * (size_t)s->nt);
         double* dx = (double*)calloc(3 * (size_t)nv, sizeof(double));
@@ -1366,13 +1361,8 @@ ORC_API void orc_step_learned(orc_sim* s, int detect, const double* corr, int it
 {
     if (detect) orc_detect(s);
     const int nv = s->nv;
-    const double dt = s->dt, dt2 = dt * dt;
-    memcpy(s->xp, s->x, sizeof(double) * 3 * nv);
-    for (int i = 0; i < nv; i++)
-        for (int k = 0; k < 3; k++) s->x[3 * i + k] += dt * s->v[3 * i + k];
-    const double dz = -s->g * dt * dt;
-    for (int i = 0; i < nv; i++) s->x[3 * i + 2] += dz;
-    keep_prediction(s);
+    const double dt = s->dt, dt2 = s->ts;   /* alpha~ = alpha / dt^2, or alpha / dt for first order */
+    move_inertially(s);
     double* dx = (double*)calloc(3 * (size_t)nv, sizeof(double));
     for (int e = 0; e < s->nt; e++) {
         const int* t = s->tets + 4 * e;

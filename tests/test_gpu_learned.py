@@ -261,6 +261,42 @@ def test_captured_learned_step_equals_the_eager_one(solver, variant):
     assert eager.learned_stats()["mlp_calls"] == 12
 
 
+@pytest.mark.parametrize("start", ["multipliers", "consistent"])
+def test_first_order_learned_step_matches_oracle(start):
+    """First-order XPBD (damping inertia, alpha~ = alpha / dt, the damped
+    gravity move) under the learned step's warm starts, against the oracle's
+    from the same correction."""
+    rng = np.random.default_rng(41)
+    n = 6
+    E, nu = W.heterogeneous(6 * n ** 3, rng, E_range=(1e3, 5e4))
+    mesh = W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.0, rng=rng)
+    gpu = X.Solver(mesh, dt=1e-3, iters=10, solver="gs", variant="quad", first_order=True, damping_multiplier=20.0,
+                   adjust_damping=True)
+    o = orc.Sim(mesh, dt=1e-3, iters=10, mode="gs")
+    o.set_first_order(20.0, True)
+    o.set_order(gpu.tet_order())
+    net = X.Network(glorot(13))
+    net.set_scaling(np.zeros(24), np.full(24, 1e-3), 2e-6)
+    gpu.attach(net)
+    gpu.set_learned_start(start)
+    for _ in range(3):
+        gpu.step(1)
+        o.step(1)
+    for k in range(3):
+        corr = net.forward(gpu.features()).astype(np.float64)
+        corr[mesh.pinned.astype(bool)] = 0.0
+        fb, r = gpu.step_learned(3, ref_residual=0.0)
+        assert not fb
+        if start == "consistent":
+            o.step_learned(corr, 3)
+        else:
+            o.step_warm(corr, 3)
+        err = rel_err(gpu.positions(), o.x())
+        print(f"first order {start} learned step {k}: rel err {err:.2e}, residual {r:.3e} "
+              f"(oracle {o.last_step_residual():.3e})")
+        assert err <= 1e-5
+
+
 def test_switching_learned_start_modes_replays_the_right_graph():
     """One captured graph per start mode: a handle that switches modes from
     step to step (positions, multipliers, consistent, in a cycle that returns
