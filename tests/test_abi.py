@@ -119,3 +119,66 @@ def test_binding_signatures_match_the_header():
         assert fn.restype == (C.c_char_p if ret == "char*" else _CTYPES[ret]), name
         checked += 1
     assert checked >= 60, checked
+
+
+def _split_args(s):
+    """Top-level comma split of a call's argument text (template argument
+    lists are not tracked: no call in the checked code has one)."""
+    out, depth, cur = [], 0, ""
+    for ch in s:
+        if ch in "([{":
+            depth += 1
+        if ch in ")]}":
+            depth -= 1
+        if ch == "," and depth == 0:
+            out.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        out.append(cur)
+    return out
+
+
+def _calls(code, prefix="xpbd_"):
+    """(name, argument count) of every call of a function named prefix* in code."""
+    found = []
+    for m in re.finditer(r"\b(%s\w+)\s*\(" % prefix, code):
+        i, depth = m.end(), 1
+        while depth:
+            depth += {"(": 1, ")": -1}.get(code[i], 0)
+            i += 1
+        found.append((m.group(1), len(_split_args(code[m.end():i - 1]))))
+    return found
+
+
+def test_integration_shim_calls_match_the_header():
+    """The C++ shim for the reference's tree and the
+    ctypes stub call only functions the header declares, each with the
+    header's number of arguments."""
+    protos = _header_prototypes()
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```(?:cpp|c\+\+|python)\n(.*?)```", text, re.S)
+    # comments dropped: they name functions without calling them
+    calls = [c for b in blocks for c in _calls(re.sub(r"(//|#)[^\n]*", " ", b))]
+    assert len(calls) >= 15, calls
+    for name, nargs in calls:
+        assert name in protos, f"{name} is not declared in xpbd_hip.h"
+        assert nargs == len(protos[name][1]), (name, nargs, protos[name][1])
+
+
+def test_integration_shim_aggregates_fit_the_structures():
+    """The shim's brace initialisers of xpbd_mesh_desc and xpbd_params give no
+    more values than the structures have fields, in the header's order of
+    kinds (pointer fields get pointers or nullptr, counts get counts)."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    shim = re.search(r"```cpp\n(.*?)```", text, re.S).group(1)
+    for struct, cls in BINDINGS.items():
+        m = re.search(r"\b%s\s+\w+\s*\{(.*?)\};" % struct, shim, re.S)
+        assert m, struct
+        vals = [v.strip() for v in _split_args(re.sub(r"/\*.*?\*/", "", m.group(1)))]
+        assert 0 < len(vals) <= len(cls._fields_), (struct, vals)
+        for v, (fname, ftype) in zip(vals, cls._fields_):
+            is_ptr = issubclass(ftype, (C._Pointer, C.c_void_p, C.c_char_p))
+            looks_ptr = v == "nullptr" or v.endswith(".data()")
+            assert is_ptr == looks_ptr, (struct, fname, v)
