@@ -326,74 +326,6 @@ __device__ __forceinline__ float quad_lane3(float x)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xFF, 0xF, 0xF, true));
 }
 
-// One element's projection in a quad, lane j holding element vertex j: from the
-// fixed part of this lane's row of D (Db), its vertex's delta dd, its column of
-// Q (qj), the element's gamma, alpha tilde and multipliers and the vertex's
-// inverse mass w.  Returns the multiplier updates d0, d1 (the same bits in all
-// four lanes) and leaves the vertex's new delta in dd.
-__device__ __forceinline__ void quad_project(const float Db[3], float dd[3], const float qj[3], float gamma,
-                                             float atD, float atH, float l0, float l1, float w, float& d0_out,
-                                             float& d1_out)
-{
-    // this lane's row of D (zero for vertex 3)
-    float D[3];
-#pragma unroll
-    for (int kk = 0; kk < 3; kk++) D[kk] = Db[kk] + (dd[kk] - quad_lane3(dd[kk]));
-    float F[9];
-#pragma unroll
-    for (int cc = 0; cc < 3; cc++)
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            const float sum = quad_sum_of_product(D[r], qj[cc]);
-            F[3 * cc + r] = r == cc ? sum + 1.f : sum;
-        }
-    // C_D, C_H as element_constraints, with both branches of C_H evaluated
-    float ss = 0.f;
-#pragma unroll
-    for (int i = 0; i < 9; i++) ss += F[i] * F[i];
-    const float CD = xsqrt<float>(ss);
-    float Fc[9];
-    Fc[0] = F[4] * F[8] - F[5] * F[7];
-    Fc[1] = F[5] * F[6] - F[3] * F[8];
-    Fc[2] = F[3] * F[7] - F[4] * F[6];
-    Fc[3] = F[7] * F[2] - F[8] * F[1];
-    Fc[4] = F[8] * F[0] - F[6] * F[2];
-    Fc[5] = F[6] * F[1] - F[7] * F[0];
-    Fc[6] = F[1] * F[5] - F[2] * F[4];
-    Fc[7] = F[2] * F[3] - F[0] * F[5];
-    Fc[8] = F[0] * F[4] - F[1] * F[3];
-    // J = det F by cofactors along column 0 (Fc[0..2] are its cofactors)
-    const float J = F[0] * Fc[0] + F[1] * Fc[1] + F[2] * Fc[2];
-    const float a = J - 1.f;
-    const float ch_log = -gamma + xlog<float>(J);
-    const float ch_series = -gamma + a - a * a * 0.5f + a * a * a * (1.f / 3.f);
-    const float CH = J >= 1.f ? ch_log : ch_series;
-    const float inv_C = xrcp<float>(CD);
-    const float fac = (J >= 1.f) ? xrcp<float>(J) : 1.f - a + a * a;
-    float gD[3], gH[3];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        gD[r] = inv_C * (F[r] * qj[0] + F[3 + r] * qj[1] + F[6 + r] * qj[2]);
-        gH[r] = fac * (Fc[r] * qj[0] + Fc[3 + r] * qj[1] + Fc[6 + r] * qj[2]);
-    }
-    const float s00 = quad_sum_of_product(w, gD[0] * gD[0] + gD[1] * gD[1] + gD[2] * gD[2]);
-    const float s01 = quad_sum_of_product(w, gD[0] * gH[0] + gD[1] * gH[1] + gD[2] * gH[2]);
-    const float s11 = quad_sum_of_product(w, gH[0] * gH[0] + gH[1] * gH[1] + gH[2] * gH[2]);
-    const float A00 = atD + s00, A11 = atH + s11, A01 = s01;
-    const float b0 = -CD - atD * l0;
-    const float b1 = -CH - atH * l1;
-    // SPD in exact arithmetic; a non-positive FP32 determinant (slivers) skips
-    // the projection, as in project_tet_F
-    const float det = A00 * A11 - A01 * A01;
-    const float inv_det = det > 0.f ? xrcp<float>(det) : 0.f;
-    const float d0 = (b0 * A11 - b1 * A01) * inv_det;
-    const float d1 = (b1 * A00 - b0 * A01) * inv_det;
-    d0_out = d0;
-    d1_out = d1;
-#pragma unroll
-    for (int kk = 0; kk < 3; kk++) dd[kk] = dd[kk] + w * (gD[kk] * d0 + gH[kk] * d1);
-}
-
 // BLOCK threads per workgroup = BLOCK / 4 clusters; the launch shape is a tuning choice.
 // MT bounds the elements of a cluster: 8 in general, 6 where every cluster is
 // one hexahedral cell, which leaves a wave's element planes a quarter smaller
@@ -612,13 +544,64 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT, LEAN>& L, const Te
             const float gamma = sq[kQR + 2][e];
             const float atD = sq[kQR][e], atH = sq[kQR + 1][e];
             const float l0 = FIRST ? 0.f : sl[0][e], l1 = FIRST ? 0.f : sl[1][e];
-            float d0, d1;
-            quad_project(Db[k], dd, qj, gamma, atD, atH, l0, l1, w, d0, d1);
+            // this lane's row of D (zero for vertex 3)
+            float D[3];
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) D[kk] = Db[k][kk] + (dd[kk] - quad_lane3(dd[kk]));
+            float F[9];
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    const float sum = quad_sum_of_product(D[r], qj[cc]);
+                    F[3 * cc + r] = r == cc ? sum + 1.f : sum;
+                }
+            // C_D, C_H as element_constraints, with both branches of C_H evaluated
+            float ss = 0.f;
+#pragma unroll
+            for (int i = 0; i < 9; i++) ss += F[i] * F[i];
+            const float CD = xsqrt<float>(ss);
+            float Fc[9];
+            Fc[0] = F[4] * F[8] - F[5] * F[7];
+            Fc[1] = F[5] * F[6] - F[3] * F[8];
+            Fc[2] = F[3] * F[7] - F[4] * F[6];
+            Fc[3] = F[7] * F[2] - F[8] * F[1];
+            Fc[4] = F[8] * F[0] - F[6] * F[2];
+            Fc[5] = F[6] * F[1] - F[7] * F[0];
+            Fc[6] = F[1] * F[5] - F[2] * F[4];
+            Fc[7] = F[2] * F[3] - F[0] * F[5];
+            Fc[8] = F[0] * F[4] - F[1] * F[3];
+            // J = det F by cofactors along column 0 (Fc[0..2] are its cofactors)
+            const float J = F[0] * Fc[0] + F[1] * Fc[1] + F[2] * Fc[2];
+            const float a = J - 1.f;
+            const float ch_log = -gamma + xlog<float>(J);
+            const float ch_series = -gamma + a - a * a * 0.5f + a * a * a * (1.f / 3.f);
+            const float CH = J >= 1.f ? ch_log : ch_series;
+            const float inv_C = xrcp<float>(CD);
+            const float fac = (J >= 1.f) ? xrcp<float>(J) : 1.f - a + a * a;
+            float gD[3], gH[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                gD[r] = inv_C * (F[r] * qj[0] + F[3 + r] * qj[1] + F[6 + r] * qj[2]);
+                gH[r] = fac * (Fc[r] * qj[0] + Fc[3 + r] * qj[1] + Fc[6 + r] * qj[2]);
+            }
+            const float s00 = quad_sum_of_product(w, gD[0] * gD[0] + gD[1] * gD[1] + gD[2] * gD[2]);
+            const float s01 = quad_sum_of_product(w, gD[0] * gH[0] + gD[1] * gH[1] + gD[2] * gH[2]);
+            const float s11 = quad_sum_of_product(w, gH[0] * gH[0] + gH[1] * gH[1] + gH[2] * gH[2]);
+            const float A00 = atD + s00, A11 = atH + s11, A01 = s01;
+            const float b0 = -CD - atD * l0;
+            const float b1 = -CH - atH * l1;
+            // SPD in exact arithmetic; a non-positive FP32 determinant (slivers) skips
+            // the projection, as in project_tet_F
+            const float det = A00 * A11 - A01 * A01;
+            const float inv_det = det > 0.f ? xrcp<float>(det) : 0.f;
+            const float d0 = (b0 * A11 - b1 * A01) * inv_det;
+            const float d1 = (b1 * A00 - b0 * A01) * inv_det;
             // the four lanes hold the same multipliers and store the same value
             sl[0][e] = l0 + d0;
             sl[1][e] = l1 + d1;
 #pragma unroll
-            for (int kk = 0; kk < 3; kk++) sd[so + kk * kQuadCol] = dd[kk];
+            for (int kk = 0; kk < 3; kk++) sd[so + kk * kQuadCol] = dd[kk] + w * (gD[kk] * d0 + gH[kk] * d1);
             // the next element's reads of other lanes' slots follow these writes in
             // program order (LDS operations of one wave complete in order)
             __builtin_amdgcn_wave_barrier();
