@@ -316,7 +316,8 @@ int xpbd_batch_step(xpbd_batch* b, int32_t nsteps);
  * layout on its own problem (Gauss-Seidel with clusters: the one-thread
  * cluster kernel and the quad kernel at 64/128/256 threads per workgroup, each
  * in its full and its LDS-lean form (vertex 3's weights summed in the quad and
- * vertex bases read across lanes: more instructions, more waves per CU);
+ * vertex bases read across lanes: more instructions, more waves per CU), each
+ * with the deltas interleaved and split (XPBD_QUAD_SPLIT);
  * the patch Jacobi kernel with and without the fused one-launch iteration;
  * cluster and patch kernels each with and without the XCD-contiguous
  * workgroup mapping) as replays of a captured graph of the handle's
@@ -326,10 +327,18 @@ int xpbd_batch_step(xpbd_batch* b, int32_t nsteps);
  * same shape.  No reference counterpart: the reference has no kernel variants. */
 int xpbd_autotune(xpbd_batch* b, const char* cache_path);
 /* The kernel variant in use (XPBD_VARIANT_*); block: for CLUSTER_QUAD the
- * workgroup size (negative for the LDS-lean form), for PATCH 1 when the fused one-launch iteration is in use
+ * workgroup size (negative for the LDS-lean form; magnitude XPBD_QUAD_SPLIT +
+ * size when the iterations keep the vertex deltas in their own array), for
+ * PATCH 1 when the fused one-launch iteration is in use
  * (deterministic mode without attachments), else 0; xcd: whether the
  * XCD-contiguous workgroup mapping is on (block and xcd may be NULL). */
 int xpbd_get_variant(xpbd_handle* h, int32_t* variant, int32_t* block, int32_t* xcd);
+/* Added to a quad block size: the Gauss-Seidel iterations copy the vertex
+ * deltas out of the interleaved vertex records into an array of their own
+ * before the loop and back after it, so that the deltas of neighbouring
+ * vertices share 32-byte write sectors (same results).  Handles with
+ * attachments keep the interleaved layout. */
+#define XPBD_QUAD_SPLIT 1000
 /* Batch handles are also plain handles: xpbd_step, xpbd_destroy, ... apply. */
 
 /* ---- host-side mesh utilities (no GPU needed) ------------------------------ */

@@ -688,13 +688,16 @@ def _forced(mesh_or_meshes, tmp_path, variant, block, xcd, **kw):
 @pytest.mark.parametrize("xcd", [0, 1])
 def test_lds_lean_quad_form_gives_the_bits_of_the_full_form(tmp_path, mesh_kind, block, xcd):
     """Both forms of the quad kernel at every workgroup size, with and without
-    the XCD-contiguous mapping, each forced through the tuning cache: the lean
-    form sums vertex 3's weights in the quad, in the order the full form stages
-    them, and moves vertex bases between lanes unchanged, so the two give the
-    same bits, and both match the oracle."""
+    the XCD-contiguous mapping, each with the deltas interleaved in the vertex
+    records and split into their own array, each forced through the tuning
+    cache: the lean form sums vertex 3's weights in the quad, in the order the
+    full form stages them, and moves vertex bases between lanes unchanged, and
+    the split layout only moves where the deltas are stored, so all four give
+    the same bits, and each matches the oracle."""
     mesh = W.cube(8, seed=21) if mesh_kind == "hexahedral cells" else _eight_element_cells(3, 4, 5, seed=23)
     out = {}
-    for b in (block, -block):
+    S = X.QUAD_SPLIT
+    for b in (block, -block, S + block, -(S + block)):
         gpu = _forced(mesh, tmp_path, "quad", b, xcd, iters=10)
         o = orc.Sim(mesh, dt=1e-3, g=9.81, iters=10, mode="gs")
         o.set_order(gpu.tet_order())
@@ -703,14 +706,18 @@ def test_lds_lean_quad_form_gives_the_bits_of_the_full_form(tmp_path, mesh_kind,
         assert errs.max() <= TOL
         out[b] = gpu.positions()
     assert np.array_equal(out[block], out[-block])
+    assert np.array_equal(out[block], out[S + block])
+    assert np.array_equal(out[block], out[-(S + block)])
 
 
-def test_uneven_batch_on_the_benchmarked_kernel_shape(tmp_path):
+@pytest.mark.parametrize("block", [-256, -(X.QUAD_SPLIT + 256)])
+def test_uneven_batch_on_the_benchmarked_kernel_shape(tmp_path, block):
     """The shape the autotuner picks for the 256-cube batch (LDS-lean quad form,
-    256 threads, XCD mapping), forced on an uneven batch with box colliders and
-    contacts every step: every instance against its own oracle step."""
+    256 threads, XCD mapping; deltas interleaved or split), forced on an uneven
+    batch with box colliders and contacts every step: every instance against
+    its own oracle step."""
     meshes, sdfs = _uneven_batch(6, (1e3, 5e4), seed=9, kind=X.SDF_BOX)
-    gpu = _forced(meshes, tmp_path, "quad", -256, 1, iters=10, collision_every=1)
+    gpu = _forced(meshes, tmp_path, "quad", block, 1, iters=10, collision_every=1)
     for i, (kind, pos, quat, prm) in enumerate(sdfs):
         gpu.set_sdf(kind, pos, quat, prm, instance=i)
     t_off = np.concatenate([[0], np.cumsum([m.nt for m in meshes])])

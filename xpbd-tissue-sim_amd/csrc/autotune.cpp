@@ -109,8 +109,12 @@ extern "C" int xpbd_autotune(xpbd_batch* h, const char* cache_path)
                                         Candidate{ XPBD_VARIANT_CLUSTER_QUAD, 256, x },
                                         Candidate{ XPBD_VARIANT_CLUSTER_QUAD, -64, x },    // LDS-lean forms
                                         Candidate{ XPBD_VARIANT_CLUSTER_QUAD, -128, x },
-                                        Candidate{ XPBD_VARIANT_CLUSTER_QUAD, -256, x } })
+                                        Candidate{ XPBD_VARIANT_CLUSTER_QUAD, -256, x } }) {
                 cand.push_back(b);
+                // the same quad shape with the split delta layout (not with attachments)
+                if (b.variant == XPBD_VARIANT_CLUSTER_QUAD && h->natt == 0)
+                    cand.push_back({ b.variant, b.block < 0 ? b.block - kQuadSplit : b.block + kQuadSplit, x });
+            }
     } else if (h->variant == XPBD_VARIANT_PATCH) {
         // block 1: the fused one-launch iteration (when the layout allows it)
         for (int f = h->fused_ok ? 1 : 0; f >= 0; f--)

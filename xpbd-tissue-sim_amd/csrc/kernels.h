@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "xpbd_hip.h"
+
 namespace xpbd {
 
 // Kinematic signed-distance field of one instance.  Pose at step n is
@@ -62,12 +64,21 @@ hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cve
                              const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, bool xcd);
 // Same clusters and order, four lanes per cluster (one per vertex of the element
 // being projected); block = threads per workgroup (64, 128 or 256; negative:
-// the same shape in the LDS-lean form, which fits more waves per CU); max_tets =
-// the most elements any cluster holds (at most kClusterTets).
+// the same shape in the LDS-lean form, which fits more waves per CU; plus
+// kQuadSplit in magnitude: the shape run with the split delta layout); max_tets
+// = the most elements any cluster holds (at most kClusterTets).  dsep: the
+// split array of the deltas (one float4 per vertex) while an iteration loop
+// runs in that layout, else null (deltas in the vertex records).
+constexpr int kQuadSplit = XPBD_QUAD_SPLIT;
 hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                                   const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block,
-                                  bool xcd, int max_tets);
+                                  bool xcd, int max_tets, float4* dsep = nullptr);
 bool quad_block_supported(int block);
+int quad_shape(int block);   // the block value without the split flag
+bool quad_split(int block);  // whether the block value asks for the split layout
+// the deltas out of the vertex records into dsep, and back
+hipError_t launch_split_deltas(hipStream_t s, const float4* pos, float4* dsep, int nv);
+hipError_t launch_join_deltas(hipStream_t s, float4* pos, const float4* dsep, int nv);
 hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float4* pos, float* acc, int nt,
                                  bool first);
 hipError_t launch_jacobi_apply(hipStream_t s, float4* pos, float* acc, int nv);
@@ -187,7 +198,8 @@ hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int
 // where instance i's faces of colour c start.
 hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fci_off, int ncol, int ninst,
                               const int* cpos, const int* clist, const double* rest, float4* pos, const float4* lo,
-                              const double* cp, const double* cn, const double* cb, double* clam, bool first);
+                              const double* cp, const double* cn, const double* cb, double* clam, bool first,
+                              float4* dsep = nullptr);
 hipError_t launch_contacts_jacobi(hipStream_t s, const int4* faces, int nf, const int* cpos, const int* clist,
                                   const double* rest, const float4* pos, const float4* lo, float* acc,
                                   const double* cp, const double* cn, const double* cb, double* clam, bool first);

@@ -354,15 +354,25 @@ struct QuadLds {
 // The unrolled step loop covers every element of a cluster.
 static_assert(kClusterTets == 8 && kClusterVerts == 8, "cluster shape of build_clusters");
 
+// Where the in-step motion (delta) of vertex v lies: after its base in the
+// interleaved vertex record, or, while a Gauss-Seidel iteration loop runs in
+// the split layout, in its own array (dsep), where the deltas of neighbouring
+// vertices share 32-byte write sectors.
+template <bool SPLIT>
+__device__ __forceinline__ float4* delta_at(float4* pos, float4* dsep, int v)
+{
+    return SPLIT ? dsep + v : pos + 2 * (size_t)v + 1;
+}
+
 // One chunk of BLOCK / 4 consecutive clusters starting at cluster cb (clusters
 // below cend), projected by one workgroup and written back.  Each wave owns 16
 // consecutive clusters and everything it stages in LDS (their vertex columns
 // and, in its own slice of the element planes, their elements), so the waves
 // need no workgroup barrier: a wave's LDS operations complete in order.
-template <bool FIRST, int BLOCK, int MT, bool LEAN>
+template <bool FIRST, int BLOCK, int MT, bool LEAN, bool SPLIT>
 __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT, LEAN>& L, const TetArrays& ta, const int4* __restrict__ cverts,
                                            const int* __restrict__ cstart, const uint32_t* __restrict__ tloc,
-                                           float4* __restrict__ pos, int cb, int cend)
+                                           float4* __restrict__ pos, float4* __restrict__ dsep, int cb, int cend)
 {
     constexpr int kQuadBlock = BLOCK;
     using Lds = QuadLds<BLOCK, MT, LEAN>;
@@ -425,13 +435,17 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT, LEAN>& L, const Te
     const int v_lo = j == 0 ? va.x : j == 1 ? va.y : j == 2 ? va.z : va.w;
     const int v_hi = j == 0 ? vb.x : j == 1 ? vb.y : j == 2 ? vb.z : vb.w;
     float4 blo = make_float4(0.f, 0.f, 0.f, 0.f), dlo = blo, bhi = blo, dhi = blo;
+    // (the interleaved delta is written as an offset from its base's address,
+    // which the loads fold into one address)
     if (v_lo >= 0) {
         blo = pos[2 * (size_t)v_lo];
-        dlo = pos[2 * (size_t)v_lo + 1];
+        if constexpr (SPLIT) dlo = dsep[v_lo];
+        else dlo = pos[2 * (size_t)v_lo + 1];
     }
     if (v_hi >= 0) {
         bhi = pos[2 * (size_t)v_hi];
-        dhi = pos[2 * (size_t)v_hi + 1];
+        if constexpr (SPLIT) dhi = dsep[v_hi];
+        else dhi = pos[2 * (size_t)v_hi + 1];
     }
 #pragma unroll
     for (int r = 0; r < kPer; r++) {
@@ -607,13 +621,18 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT, LEAN>& L, const Te
             __builtin_amdgcn_wave_barrier();
         }
     }
-    if (v_lo >= 0)
-        pos[2 * (size_t)v_lo + 1] = make_float4(sd[(j * 3 + 0) * kQuadCol + cl], sd[(j * 3 + 1) * kQuadCol + cl],
-                                                sd[(j * 3 + 2) * kQuadCol + cl], 0.f);
+    if (v_lo >= 0) {
+        const float4 d = make_float4(sd[(j * 3 + 0) * kQuadCol + cl], sd[(j * 3 + 1) * kQuadCol + cl],
+                                     sd[(j * 3 + 2) * kQuadCol + cl], 0.f);
+        if constexpr (SPLIT) dsep[v_lo] = d;
+        else pos[2 * (size_t)v_lo + 1] = d;
+    }
     if (v_hi >= 0) {
         const int qs = j + 4;
-        pos[2 * (size_t)v_hi + 1] = make_float4(sd[(qs * 3 + 0) * kQuadCol + cl], sd[(qs * 3 + 1) * kQuadCol + cl],
-                                                sd[(qs * 3 + 2) * kQuadCol + cl], 0.f);
+        const float4 d = make_float4(sd[(qs * 3 + 0) * kQuadCol + cl], sd[(qs * 3 + 1) * kQuadCol + cl],
+                                     sd[(qs * 3 + 2) * kQuadCol + cl], 0.f);
+        if constexpr (SPLIT) dsep[v_hi] = d;
+        else pos[2 * (size_t)v_hi + 1] = d;
     }
     // multipliers of the wave's tet range, coalesced
     __builtin_amdgcn_wave_barrier();
@@ -624,15 +643,31 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT, LEAN>& L, const Te
     }
 }
 
-template <bool FIRST, int BLOCK, int MT, bool LEAN>
+template <bool FIRST, int BLOCK, int MT, bool LEAN, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const int4* __restrict__ cverts,
                                                           const int* __restrict__ cstart,
                                                           const uint32_t* __restrict__ tloc,
-                                                          float4* __restrict__ pos, int cbeg, int cend, int xcd)
+                                                          float4* __restrict__ pos, float4* __restrict__ dsep,
+                                                          int cbeg, int cend, int xcd)
 {
     __shared__ QuadLds<BLOCK, MT, LEAN> L;
     const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * QuadLds<BLOCK, MT, LEAN>::kClusters;
-    quad_chunk<FIRST, BLOCK, MT, LEAN>(L, ta, cverts, cstart, tloc, pos, cb, cend);
+    quad_chunk<FIRST, BLOCK, MT, LEAN, SPLIT>(L, ta, cverts, cstart, tloc, pos, dsep, cb, cend);
+}
+
+// The split layout of the deltas for a Gauss-Seidel iteration loop: copied out
+// of the vertex records before it and back after it.
+__global__ __launch_bounds__(kBlock) void k_split_deltas(const float4* __restrict__ pos, float4* __restrict__ dsep,
+                                                         int nv)
+{
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < nv) dsep[v] = pos[2 * (size_t)v + 1];
+}
+__global__ __launch_bounds__(kBlock) void k_join_deltas(float4* __restrict__ pos, const float4* __restrict__ dsep,
+                                                        int nv)
+{
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < nv) pos[2 * (size_t)v + 1] = dsep[v];
 }
 
 // ---------------------------------------------------------------------------
@@ -1436,17 +1471,36 @@ __device__ __forceinline__ bool contact_dx(const double* const x[3], const float
 
 // One contact of the Gauss-Seidel pass: project it from the current positions
 // and apply its correction to the face's three vertices.
+template <bool SPLIT>
 __device__ __forceinline__ void contact_gs_one(bool first, int j, const int4* __restrict__ faces,
                                                const double* __restrict__ rest, float4* __restrict__ pos,
+                                               float4* __restrict__ dsep,
                                                const float4* __restrict__ lo, const double* __restrict__ cp,
                                                const double* __restrict__ cn, const double* __restrict__ cb,
                                                double* __restrict__ clam)
 {
     const int4 f = faces[j];
     double x0[3], x1[3], x2[3];
-    world_pos(rest, pos, lo, f.x, true, x0);
-    world_pos(rest, pos, lo, f.y, true, x1);
-    world_pos(rest, pos, lo, f.z, true, x2);
+    if constexpr (SPLIT) {
+        // world_pos with the delta read from the split array, in the same order
+        const int vv[3] = { f.x, f.y, f.z };
+        double* xx[3] = { x0, x1, x2 };
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            double u[3];
+            disp_f64(pos, lo, vv[m], false, u);
+            const float4 d = dsep[vv[m]];
+            u[0] += (double)d.x;
+            u[1] += (double)d.y;
+            u[2] += (double)d.z;
+#pragma unroll
+            for (int k = 0; k < 3; k++) xx[m][k] = rest[3 * (size_t)vv[m] + k] + u[k];
+        }
+    } else {
+        world_pos(rest, pos, lo, f.x, true, x0);
+        world_pos(rest, pos, lo, f.y, true, x1);
+        world_pos(rest, pos, lo, f.z, true, x2);
+    }
     const double* xs[3] = { x0, x1, x2 };
     const float w[3] = { pos[2 * (size_t)f.x].w, pos[2 * (size_t)f.y].w, pos[2 * (size_t)f.z].w };
     double lam = first ? 0.0 : clam[j];
@@ -1455,11 +1509,11 @@ __device__ __forceinline__ void contact_gs_one(bool first, int j, const int4* __
         const int vid[3] = { f.x, f.y, f.z };
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            float4 d = pos[2 * (size_t)vid[i] + 1];
+            float4 d = *delta_at<SPLIT>(pos, dsep, vid[i]);
             d.x = (float)((double)d.x + dx[3 * i]);
             d.y = (float)((double)d.y + dx[3 * i + 1]);
             d.z = (float)((double)d.z + dx[3 * i + 2]);
-            pos[2 * (size_t)vid[i] + 1] = d;
+            *delta_at<SPLIT>(pos, dsep, vid[i]) = d;
         }
     }
     clam[j] = lam;
@@ -1475,12 +1529,12 @@ __device__ __forceinline__ void contact_gs_one(bool first, int j, const int4* __
 // masses, contact point, normal, barycentrics, multiplier) is loaded once, so a
 // colour costs one round trip for the in-step motion of its vertices.  Contacts
 // past the first kBlock of an instance are loaded as they are projected.
-template <bool FIRST>
+template <bool FIRST, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__ faces,
                                                        const int* __restrict__ fci_off, int ncol, int ninst,
                                                        const int* __restrict__ cpos, const int* __restrict__ clist,
                                                        const double* __restrict__ rest, float4* __restrict__ pos,
-                                                       const float4* __restrict__ lo,
+                                                       float4* __restrict__ dsep, const float4* __restrict__ lo,
                                                        const double* __restrict__ cp, const double* __restrict__ cn,
                                                        const double* __restrict__ cb, double* __restrict__ clam)
 {
@@ -1550,7 +1604,7 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
                 double x[3][3];
 #pragma unroll
                 for (int m = 0; m < 3; m++) {
-                    d[m] = pos[2 * (size_t)vid[m] + 1];
+                    d[m] = *delta_at<SPLIT>(pos, dsep, vid[m]);
                     const double u[3] = { sbl[m][0] + (double)d[m].x, sbl[m][1] + (double)d[m].y,
                                           sbl[m][2] + (double)d[m].z };
 #pragma unroll
@@ -1565,12 +1619,14 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
                         dn.x = (float)((double)dn.x + dx[3 * m]);
                         dn.y = (float)((double)dn.y + dx[3 * m + 1]);
                         dn.z = (float)((double)dn.z + dx[3 * m + 2]);
-                        pos[2 * (size_t)vid[m] + 1] = dn;
+                        *delta_at<SPLIT>(pos, dsep, vid[m]) = dn;
                     }
                 }
             }
             for (int q = loc[c] + tid; q < loc[c + 1]; q += kBlock)
-                if (q >= kBlock) contact_gs_one(FIRST, clist[cs + (q - loc[c])], faces, rest, pos, lo, cp, cn, cb, clam);
+                if (q >= kBlock)
+                    contact_gs_one<SPLIT>(FIRST, clist[cs + (q - loc[c])], faces, rest, pos, dsep, lo, cp, cn, cb,
+                                          clam);
             __syncthreads();
         }
         if (mc >= 0) clam[j] = lam;
@@ -2276,55 +2332,95 @@ hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cve
     return hipGetLastError();
 }
 
-template <int BLOCK, int MT, bool LEAN>
+template <int BLOCK, int MT, bool LEAN, bool SPLIT>
 static void launch_quad_shape(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                              const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, bool xcd)
+                              const uint32_t* tloc, float4* pos, float4* dsep, int cbeg, int cend, bool first,
+                              bool xcd)
 {
     const dim3 g(nblk(cend - cbeg, BLOCK / 4));
     if (first)
-        hipLaunchKernelGGL((k_gs_cluster_quad<true, BLOCK, MT, LEAN>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc,
-                           pos, cbeg, cend, (int)xcd);
+        hipLaunchKernelGGL((k_gs_cluster_quad<true, BLOCK, MT, LEAN, SPLIT>), g, dim3(BLOCK), 0, s, ta, cverts, cstart,
+                           tloc, pos, dsep, cbeg, cend, (int)xcd);
     else
-        hipLaunchKernelGGL((k_gs_cluster_quad<false, BLOCK, MT, LEAN>), g, dim3(BLOCK), 0, s, ta, cverts, cstart, tloc,
-                           pos, cbeg, cend, (int)xcd);
+        hipLaunchKernelGGL((k_gs_cluster_quad<false, BLOCK, MT, LEAN, SPLIT>), g, dim3(BLOCK), 0, s, ta, cverts,
+                           cstart, tloc, pos, dsep, cbeg, cend, (int)xcd);
 }
 
-template <int MT, bool LEAN>
+template <int MT, bool LEAN, bool SPLIT>
 static bool launch_quad_mt(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                           const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block, bool xcd)
+                           const uint32_t* tloc, float4* pos, float4* dsep, int cbeg, int cend, bool first, int block,
+                           bool xcd)
 {
-    if (block == 64) launch_quad_shape<64, MT, LEAN>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
-    else if (block == 128) launch_quad_shape<128, MT, LEAN>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
-    else if (block == 256) launch_quad_shape<256, MT, LEAN>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, xcd);
+    if (block == 64)
+        launch_quad_shape<64, MT, LEAN, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, xcd);
+    else if (block == 128)
+        launch_quad_shape<128, MT, LEAN, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, xcd);
+    else if (block == 256)
+        launch_quad_shape<256, MT, LEAN, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, xcd);
     else return false;
     return true;
 }
 
-template <int MT>
+template <int MT, bool SPLIT>
 static bool launch_quad_lean(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                             const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block, bool xcd)
+                             const uint32_t* tloc, float4* pos, float4* dsep, int cbeg, int cend, bool first,
+                             int block, bool xcd)
 {
     // a negative shape selects the LDS-lean form
-    return block < 0 ? launch_quad_mt<MT, true>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, -block, xcd)
-                     : launch_quad_mt<MT, false>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd);
+    return block < 0
+               ? launch_quad_mt<MT, true, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, -block, xcd)
+               : launch_quad_mt<MT, false, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, block, xcd);
+}
+
+int quad_shape(int block)
+{
+    const int b = block < 0 ? -block : block;
+    const int shape = b > kQuadSplit ? b - kQuadSplit : b;
+    return block < 0 ? -shape : shape;
+}
+
+bool quad_split(int block)
+{
+    return (block < 0 ? -block : block) > kQuadSplit;
 }
 
 bool quad_block_supported(int block)
 {
-    const int b = block < 0 ? -block : block;
+    const int s = quad_shape(block);
+    const int b = s < 0 ? -s : s;
     return b == 64 || b == 128 || b == 256;
 }
 
 hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                                   const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block,
-                                  bool xcd, int max_tets)
+                                  bool xcd, int max_tets, float4* dsep)
 {
     if (cend <= cbeg) return hipSuccess;
     if (max_tets < 1 || max_tets > kClusterTets) return hipErrorInvalidValue;
-    const bool ok = max_tets <= 6
-                        ? launch_quad_lean<6>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd)
-                        : launch_quad_lean<8>(s, ta, cverts, cstart, tloc, pos, cbeg, cend, first, block, xcd);
+    const int b = quad_shape(block);
+    bool ok;
+    if (dsep)
+        ok = max_tets <= 6 ? launch_quad_lean<6, true>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, b, xcd)
+                           : launch_quad_lean<8, true>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, b, xcd);
+    else
+        ok = max_tets <= 6
+                 ? launch_quad_lean<6, false>(s, ta, cverts, cstart, tloc, pos, nullptr, cbeg, cend, first, b, xcd)
+                 : launch_quad_lean<8, false>(s, ta, cverts, cstart, tloc, pos, nullptr, cbeg, cend, first, b, xcd);
     if (!ok) return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_split_deltas(hipStream_t s, const float4* pos, float4* dsep, int nv)
+{
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_split_deltas, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos, dsep, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_join_deltas(hipStream_t s, float4* pos, const float4* dsep, int nv)
+{
+    if (nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_join_deltas, dim3(nblk(nv, kBlock)), dim3(kBlock), 0, s, pos, dsep, nv);
     return hipGetLastError();
 }
 
@@ -2497,15 +2593,23 @@ hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int
 
 hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fci_off, int ncol, int ninst,
                               const int* cpos, const int* clist, const double* rest, float4* pos, const float4* lo,
-                              const double* cp, const double* cn, const double* cb, double* clam, bool first)
+                              const double* cp, const double* cn, const double* cb, double* clam, bool first,
+                              float4* dsep)
 {
     if (ninst <= 0 || ncol <= 0) return hipSuccess;
-    if (first)
-        hipLaunchKernelGGL(k_contacts_gs<true>, dim3(ninst), dim3(kBlock), 0, s, faces, fci_off, ncol, ninst, cpos,
-                           clist, rest, pos, lo, cp, cn, cb, clam);
+    const dim3 g(ninst), b(kBlock);
+    if (first && dsep)
+        hipLaunchKernelGGL((k_contacts_gs<true, true>), g, b, 0, s, faces, fci_off, ncol, ninst, cpos, clist, rest, pos,
+                           dsep, lo, cp, cn, cb, clam);
+    else if (first)
+        hipLaunchKernelGGL((k_contacts_gs<true, false>), g, b, 0, s, faces, fci_off, ncol, ninst, cpos, clist, rest,
+                           pos, nullptr, lo, cp, cn, cb, clam);
+    else if (dsep)
+        hipLaunchKernelGGL((k_contacts_gs<false, true>), g, b, 0, s, faces, fci_off, ncol, ninst, cpos, clist, rest,
+                           pos, dsep, lo, cp, cn, cb, clam);
     else
-        hipLaunchKernelGGL(k_contacts_gs<false>, dim3(ninst), dim3(kBlock), 0, s, faces, fci_off, ncol, ninst, cpos,
-                           clist, rest, pos, lo, cp, cn, cb, clam);
+        hipLaunchKernelGGL((k_contacts_gs<false, false>), g, b, 0, s, faces, fci_off, ncol, ninst, cpos, clist, rest,
+                           pos, nullptr, lo, cp, cn, cb, clam);
     return hipGetLastError();
 }
 

@@ -259,6 +259,7 @@ SOLVERS = {"jacobi": XPBD_SOLVER_JACOBI, "gs": XPBD_SOLVER_GS_COLORED}
 
 
 VARIANTS = {"auto": 0, "simple": 1, "cluster": 2, "patch": 3, "quad": 4}
+QUAD_SPLIT = 1000  # XPBD_QUAD_SPLIT: added to a quad block size, the iterations keep the deltas in their own array
 
 
 def make_params(dt=1e-3, g=9.81, iters=10, solver="gs", deterministic=1, collision_every=1,
