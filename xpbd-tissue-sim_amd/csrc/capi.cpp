@@ -451,7 +451,7 @@ static int build(int32_t n, const xpbd_mesh_desc* meshes, const xpbd_params* P, 
     }
     if (h->nclusters > 0) {
         rc |= dalloc(h, &h->d_cverts, 2 * (size_t)h->nclusters);
-        rc |= dalloc(h, &h->d_dsep, (size_t)nv);
+        rc |= dalloc(h, &h->d_dsep, 2 * (size_t)nv);   // deltas, then bases
         rc |= dalloc(h, &h->d_cstart, (size_t)h->nclusters + 1);
         rc |= dalloc(h, &h->d_tloc, nt);
     }
@@ -853,8 +853,9 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed, bool reset_lambda)
         return XPBD_OK;
     }
     // Quad Gauss-Seidel in the split layout: the deltas leave the vertex records
-    // for their own array before the loop and return after it (the attachment
-    // pass reads the records, so handles with attachments stay interleaved).
+    // for their own array before the loop and return after it, and the bases,
+    // which the loop only reads, are copied into one (the attachment pass reads
+    // the records, so handles with attachments stay interleaved).
     float4* dsep = h->params.solver == XPBD_SOLVER_GS_COLORED && h->variant == XPBD_VARIANT_CLUSTER_QUAD &&
                            quad_split(h->quad_block) && h->natt == 0 && iters > 0
                        ? h->d_dsep
@@ -875,7 +876,7 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed, bool reset_lambda)
                 else if (h->variant == XPBD_VARIANT_CLUSTER_QUAD)
                     e = launch_gs_cluster_quad(s, ta, h->d_cverts, h->d_cstart, h->d_tloc, h->d_pos,
                                                h->color_off[c], h->color_off[c + 1], first, h->quad_block,
-                                               h->xcd_map, h->cluster_max_tets, dsep);
+                                               h->xcd_map, h->cluster_max_tets, dsep, dsep ? dsep + h->nv : nullptr);
                 else
                     e = launch_gs_color(s, ta, h->d_pos, h->color_off[c], h->color_off[c + 1], first);
                 if (e != hipSuccess) return hip_err(h, e, "gs_color launch");
@@ -883,7 +884,8 @@ int enqueue_iterations(xpbd_handle* h, int iters, bool timed, bool reset_lambda)
             if (timed) timed_end(h);
             if (con) {
                 e = launch_contacts_gs(s, h->d_faces, h->d_fci_off, h->nfcolors, h->ninst, h->d_cpos, h->d_clist, h->d_rest,
-                                       h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first, dsep);
+                                       h->d_pos, h->d_lo, h->d_cp, h->d_cn, h->d_cb, h->d_clam, first, dsep,
+                                       dsep ? dsep + h->nv : nullptr);
                 if (e != hipSuccess) return hip_err(h, e, "contacts_gs launch");
             }
             // attachments follow the collision projectors (type-list order, XPBDTypedefs.hpp:56)

@@ -65,7 +65,7 @@ struct xpbd_handle {
     float* d_gamma = nullptr;
     float2* d_lam = nullptr;
     int4* d_cverts = nullptr;    // cluster variant: 8 vertex ids per cluster (2 x int4)
-    float4* d_dsep = nullptr;    // quad variant, split layout: the vertex deltas during an iteration loop
+    float4* d_dsep = nullptr;    // quad variant, split layout: vertex deltas, then bases, during an iteration loop
     int32_t* d_cstart = nullptr;
     uint32_t* d_tloc = nullptr;
     uint32_t* d_ptloc = nullptr; // patch variant (see PatchArrays)

@@ -65,18 +65,20 @@ hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cve
 // Same clusters and order, four lanes per cluster (one per vertex of the element
 // being projected); block = threads per workgroup (64, 128 or 256; negative:
 // the same shape in the LDS-lean form, which fits more waves per CU; plus
-// kQuadSplit in magnitude: the shape run with the split delta layout); max_tets
-// = the most elements any cluster holds (at most kClusterTets).  dsep: the
-// split array of the deltas (one float4 per vertex) while an iteration loop
-// runs in that layout, else null (deltas in the vertex records).
+// kQuadSplit in magnitude: the shape run with the split layout); max_tets = the
+// most elements any cluster holds (at most kClusterTets).  dsep, dbase: the
+// deltas and the bases in arrays of their own (one float4 per vertex each)
+// while an iteration loop runs in the split layout, else null (both in the
+// interleaved vertex records).
 constexpr int kQuadSplit = XPBD_QUAD_SPLIT;
 hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                                   const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block,
-                                  bool xcd, int max_tets, float4* dsep = nullptr);
+                                  bool xcd, int max_tets, float4* dsep = nullptr, const float4* dbase = nullptr);
 bool quad_block_supported(int block);
 int quad_shape(int block);   // the block value without the split flag
 bool quad_split(int block);  // whether the block value asks for the split layout
-// the deltas out of the vertex records into dsep, and back
+// the deltas out of the vertex records into dsep[0, nv) and the bases into
+// dsep[nv, 2 nv); the deltas back
 hipError_t launch_split_deltas(hipStream_t s, const float4* pos, float4* dsep, int nv);
 hipError_t launch_join_deltas(hipStream_t s, float4* pos, const float4* dsep, int nv);
 hipError_t launch_jacobi_project(hipStream_t s, const TetArrays& ta, const float4* pos, float* acc, int nt,
@@ -199,7 +201,7 @@ hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int
 hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fci_off, int ncol, int ninst,
                               const int* cpos, const int* clist, const double* rest, float4* pos, const float4* lo,
                               const double* cp, const double* cn, const double* cb, double* clam, bool first,
-                              float4* dsep = nullptr);
+                              float4* dsep = nullptr, const float4* dbase = nullptr);
 hipError_t launch_contacts_jacobi(hipStream_t s, const int4* faces, int nf, const int* cpos, const int* clist,
                                   const double* rest, const float4* pos, const float4* lo, float* acc,
                                   const double* cp, const double* cn, const double* cb, double* clam, bool first);

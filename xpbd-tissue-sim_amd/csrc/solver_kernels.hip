@@ -372,7 +372,8 @@ __device__ __forceinline__ float4* delta_at(float4* pos, float4* dsep, int v)
 template <bool FIRST, int BLOCK, int MT, bool LEAN, bool SPLIT>
 __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT, LEAN>& L, const TetArrays& ta, const int4* __restrict__ cverts,
                                            const int* __restrict__ cstart, const uint32_t* __restrict__ tloc,
-                                           float4* __restrict__ pos, float4* __restrict__ dsep, int cb, int cend)
+                                           float4* __restrict__ pos, float4* __restrict__ dsep,
+                                           const float4* __restrict__ dbase, int cb, int cend)
 {
     constexpr int kQuadBlock = BLOCK;
     using Lds = QuadLds<BLOCK, MT, LEAN>;
@@ -438,14 +439,22 @@ __device__ __forceinline__ void quad_chunk(QuadLds<BLOCK, MT, LEAN>& L, const Te
     // (the interleaved delta is written as an offset from its base's address,
     // which the loads fold into one address)
     if (v_lo >= 0) {
-        blo = pos[2 * (size_t)v_lo];
-        if constexpr (SPLIT) dlo = dsep[v_lo];
-        else dlo = pos[2 * (size_t)v_lo + 1];
+        if constexpr (SPLIT) {
+            blo = dbase[v_lo];
+            dlo = dsep[v_lo];
+        } else {
+            blo = pos[2 * (size_t)v_lo];
+            dlo = pos[2 * (size_t)v_lo + 1];
+        }
     }
     if (v_hi >= 0) {
-        bhi = pos[2 * (size_t)v_hi];
-        if constexpr (SPLIT) dhi = dsep[v_hi];
-        else dhi = pos[2 * (size_t)v_hi + 1];
+        if constexpr (SPLIT) {
+            bhi = dbase[v_hi];
+            dhi = dsep[v_hi];
+        } else {
+            bhi = pos[2 * (size_t)v_hi];
+            dhi = pos[2 * (size_t)v_hi + 1];
+        }
     }
 #pragma unroll
     for (int r = 0; r < kPer; r++) {
@@ -648,20 +657,25 @@ __global__ __launch_bounds__(BLOCK) void k_gs_cluster_quad(TetArrays ta, const i
                                                           const int* __restrict__ cstart,
                                                           const uint32_t* __restrict__ tloc,
                                                           float4* __restrict__ pos, float4* __restrict__ dsep,
-                                                          int cbeg, int cend, int xcd)
+                                                          const float4* __restrict__ dbase, int cbeg, int cend,
+                                                          int xcd)
 {
     __shared__ QuadLds<BLOCK, MT, LEAN> L;
     const int cb = cbeg + xcd_item(blockIdx.x, gridDim.x, xcd != 0) * QuadLds<BLOCK, MT, LEAN>::kClusters;
-    quad_chunk<FIRST, BLOCK, MT, LEAN, SPLIT>(L, ta, cverts, cstart, tloc, pos, dsep, cb, cend);
+    quad_chunk<FIRST, BLOCK, MT, LEAN, SPLIT>(L, ta, cverts, cstart, tloc, pos, dsep, dbase, cb, cend);
 }
 
-// The split layout of the deltas for a Gauss-Seidel iteration loop: copied out
-// of the vertex records before it and back after it.
+// The split layout for a Gauss-Seidel iteration loop: the deltas copied out of
+// the vertex records before it (dsep[0, nv)) and back after it, and the bases,
+// which the loop only reads, copied once into dsep[nv, 2 nv).
 __global__ __launch_bounds__(kBlock) void k_split_deltas(const float4* __restrict__ pos, float4* __restrict__ dsep,
                                                          int nv)
 {
     const int v = blockIdx.x * kBlock + threadIdx.x;
-    if (v < nv) dsep[v] = pos[2 * (size_t)v + 1];
+    if (v < nv) {
+        dsep[v] = pos[2 * (size_t)v + 1];
+        dsep[(size_t)nv + v] = pos[2 * (size_t)v];
+    }
 }
 __global__ __launch_bounds__(kBlock) void k_join_deltas(float4* __restrict__ pos, const float4* __restrict__ dsep,
                                                         int nv)
@@ -1474,7 +1488,7 @@ __device__ __forceinline__ bool contact_dx(const double* const x[3], const float
 template <bool SPLIT>
 __device__ __forceinline__ void contact_gs_one(bool first, int j, const int4* __restrict__ faces,
                                                const double* __restrict__ rest, float4* __restrict__ pos,
-                                               float4* __restrict__ dsep,
+                                               float4* __restrict__ dsep, const float4* __restrict__ dbase,
                                                const float4* __restrict__ lo, const double* __restrict__ cp,
                                                const double* __restrict__ cn, const double* __restrict__ cb,
                                                double* __restrict__ clam)
@@ -1487,8 +1501,9 @@ __device__ __forceinline__ void contact_gs_one(bool first, int j, const int4* __
         double* xx[3] = { x0, x1, x2 };
 #pragma unroll
         for (int m = 0; m < 3; m++) {
-            double u[3];
-            disp_f64(pos, lo, vv[m], false, u);
+            const float4 b = dbase[vv[m]];
+            const float4 l = lo[vv[m]];
+            double u[3] = { (double)b.x + (double)l.x, (double)b.y + (double)l.y, (double)b.z + (double)l.z };
             const float4 d = dsep[vv[m]];
             u[0] += (double)d.x;
             u[1] += (double)d.y;
@@ -1502,7 +1517,16 @@ __device__ __forceinline__ void contact_gs_one(bool first, int j, const int4* __
         world_pos(rest, pos, lo, f.z, true, x2);
     }
     const double* xs[3] = { x0, x1, x2 };
-    const float w[3] = { pos[2 * (size_t)f.x].w, pos[2 * (size_t)f.y].w, pos[2 * (size_t)f.z].w };
+    float w[3];
+    if constexpr (SPLIT) {
+        w[0] = dbase[f.x].w;
+        w[1] = dbase[f.y].w;
+        w[2] = dbase[f.z].w;
+    } else {
+        w[0] = pos[2 * (size_t)f.x].w;
+        w[1] = pos[2 * (size_t)f.y].w;
+        w[2] = pos[2 * (size_t)f.z].w;
+    }
     double lam = first ? 0.0 : clam[j];
     double dx[9];
     if (contact_dx(xs, w, cp + 3 * (size_t)j, cn + 3 * (size_t)j, cb + 3 * (size_t)j, &lam, dx)) {
@@ -1534,7 +1558,8 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
                                                        const int* __restrict__ fci_off, int ncol, int ninst,
                                                        const int* __restrict__ cpos, const int* __restrict__ clist,
                                                        const double* __restrict__ rest, float4* __restrict__ pos,
-                                                       float4* __restrict__ dsep, const float4* __restrict__ lo,
+                                                       float4* __restrict__ dsep, const float4* __restrict__ dbase,
+                                                       const float4* __restrict__ lo,
                                                        const double* __restrict__ cp, const double* __restrict__ cn,
                                                        const double* __restrict__ cb, double* __restrict__ clam)
 {
@@ -1578,7 +1603,7 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
             vid[2] = f.z;
 #pragma unroll
             for (int m = 0; m < 3; m++) {
-                const float4 bv = pos[2 * (size_t)vid[m]];
+                const float4 bv = SPLIT ? dbase[vid[m]] : pos[2 * (size_t)vid[m]];
                 const float4 l = lo[vid[m]];
                 sbl[m][0] = (double)bv.x + (double)l.x;
                 sbl[m][1] = (double)bv.y + (double)l.y;
@@ -1625,8 +1650,8 @@ __global__ __launch_bounds__(kBlock) void k_contacts_gs(const int4* __restrict__
             }
             for (int q = loc[c] + tid; q < loc[c + 1]; q += kBlock)
                 if (q >= kBlock)
-                    contact_gs_one<SPLIT>(FIRST, clist[cs + (q - loc[c])], faces, rest, pos, dsep, lo, cp, cn, cb,
-                                          clam);
+                    contact_gs_one<SPLIT>(FIRST, clist[cs + (q - loc[c])], faces, rest, pos, dsep, dbase, lo, cp, cn,
+                                          cb, clam);
             __syncthreads();
         }
         if (mc >= 0) clam[j] = lam;
@@ -2334,42 +2359,43 @@ hipError_t launch_gs_cluster(hipStream_t s, const TetArrays& ta, const int4* cve
 
 template <int BLOCK, int MT, bool LEAN, bool SPLIT>
 static void launch_quad_shape(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                              const uint32_t* tloc, float4* pos, float4* dsep, int cbeg, int cend, bool first,
-                              bool xcd)
+                              const uint32_t* tloc, float4* pos, float4* dsep, const float4* dbase, int cbeg,
+                              int cend, bool first, bool xcd)
 {
     const dim3 g(nblk(cend - cbeg, BLOCK / 4));
     if (first)
         hipLaunchKernelGGL((k_gs_cluster_quad<true, BLOCK, MT, LEAN, SPLIT>), g, dim3(BLOCK), 0, s, ta, cverts, cstart,
-                           tloc, pos, dsep, cbeg, cend, (int)xcd);
+                           tloc, pos, dsep, dbase, cbeg, cend, (int)xcd);
     else
         hipLaunchKernelGGL((k_gs_cluster_quad<false, BLOCK, MT, LEAN, SPLIT>), g, dim3(BLOCK), 0, s, ta, cverts,
-                           cstart, tloc, pos, dsep, cbeg, cend, (int)xcd);
+                           cstart, tloc, pos, dsep, dbase, cbeg, cend, (int)xcd);
 }
 
 template <int MT, bool LEAN, bool SPLIT>
 static bool launch_quad_mt(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                           const uint32_t* tloc, float4* pos, float4* dsep, int cbeg, int cend, bool first, int block,
-                           bool xcd)
+                           const uint32_t* tloc, float4* pos, float4* dsep, const float4* dbase, int cbeg, int cend,
+                           bool first, int block, bool xcd)
 {
     if (block == 64)
-        launch_quad_shape<64, MT, LEAN, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, xcd);
+        launch_quad_shape<64, MT, LEAN, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, dbase, cbeg, cend, first, xcd);
     else if (block == 128)
-        launch_quad_shape<128, MT, LEAN, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, xcd);
+        launch_quad_shape<128, MT, LEAN, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, dbase, cbeg, cend, first, xcd);
     else if (block == 256)
-        launch_quad_shape<256, MT, LEAN, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, xcd);
+        launch_quad_shape<256, MT, LEAN, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, dbase, cbeg, cend, first, xcd);
     else return false;
     return true;
 }
 
 template <int MT, bool SPLIT>
 static bool launch_quad_lean(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
-                             const uint32_t* tloc, float4* pos, float4* dsep, int cbeg, int cend, bool first,
-                             int block, bool xcd)
+                             const uint32_t* tloc, float4* pos, float4* dsep, const float4* dbase, int cbeg, int cend,
+                             bool first, int block, bool xcd)
 {
     // a negative shape selects the LDS-lean form
-    return block < 0
-               ? launch_quad_mt<MT, true, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, -block, xcd)
-               : launch_quad_mt<MT, false, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, block, xcd);
+    return block < 0 ? launch_quad_mt<MT, true, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, dbase, cbeg, cend,
+                                                       first, -block, xcd)
+                     : launch_quad_mt<MT, false, SPLIT>(s, ta, cverts, cstart, tloc, pos, dsep, dbase, cbeg, cend,
+                                                        first, block, xcd);
 }
 
 int quad_shape(int block)
@@ -2393,19 +2419,21 @@ bool quad_block_supported(int block)
 
 hipError_t launch_gs_cluster_quad(hipStream_t s, const TetArrays& ta, const int4* cverts, const int* cstart,
                                   const uint32_t* tloc, float4* pos, int cbeg, int cend, bool first, int block,
-                                  bool xcd, int max_tets, float4* dsep)
+                                  bool xcd, int max_tets, float4* dsep, const float4* dbase)
 {
     if (cend <= cbeg) return hipSuccess;
     if (max_tets < 1 || max_tets > kClusterTets) return hipErrorInvalidValue;
     const int b = quad_shape(block);
     bool ok;
-    if (dsep)
-        ok = max_tets <= 6 ? launch_quad_lean<6, true>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, b, xcd)
-                           : launch_quad_lean<8, true>(s, ta, cverts, cstart, tloc, pos, dsep, cbeg, cend, first, b, xcd);
-    else
+    if (dsep && dbase)
         ok = max_tets <= 6
-                 ? launch_quad_lean<6, false>(s, ta, cverts, cstart, tloc, pos, nullptr, cbeg, cend, first, b, xcd)
-                 : launch_quad_lean<8, false>(s, ta, cverts, cstart, tloc, pos, nullptr, cbeg, cend, first, b, xcd);
+                 ? launch_quad_lean<6, true>(s, ta, cverts, cstart, tloc, pos, dsep, dbase, cbeg, cend, first, b, xcd)
+                 : launch_quad_lean<8, true>(s, ta, cverts, cstart, tloc, pos, dsep, dbase, cbeg, cend, first, b, xcd);
+    else
+        ok = max_tets <= 6 ? launch_quad_lean<6, false>(s, ta, cverts, cstart, tloc, pos, nullptr, nullptr, cbeg, cend,
+                                                        first, b, xcd)
+                           : launch_quad_lean<8, false>(s, ta, cverts, cstart, tloc, pos, nullptr, nullptr, cbeg, cend,
+                                                        first, b, xcd);
     if (!ok) return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -2594,22 +2622,23 @@ hipError_t launch_compact(hipStream_t s, const int* flag, int nf, int* cpos, int
 hipError_t launch_contacts_gs(hipStream_t s, const int4* faces, const int* fci_off, int ncol, int ninst,
                               const int* cpos, const int* clist, const double* rest, float4* pos, const float4* lo,
                               const double* cp, const double* cn, const double* cb, double* clam, bool first,
-                              float4* dsep)
+                              float4* dsep, const float4* dbase)
 {
     if (ninst <= 0 || ncol <= 0) return hipSuccess;
     const dim3 g(ninst), b(kBlock);
-    if (first && dsep)
+    const bool split = dsep && dbase;
+    if (first && split)
         hipLaunchKernelGGL((k_contacts_gs<true, true>), g, b, 0, s, faces, fci_off, ncol, ninst, cpos, clist, rest, pos,
-                           dsep, lo, cp, cn, cb, clam);
+                           dsep, dbase, lo, cp, cn, cb, clam);
     else if (first)
         hipLaunchKernelGGL((k_contacts_gs<true, false>), g, b, 0, s, faces, fci_off, ncol, ninst, cpos, clist, rest,
-                           pos, nullptr, lo, cp, cn, cb, clam);
-    else if (dsep)
+                           pos, nullptr, nullptr, lo, cp, cn, cb, clam);
+    else if (split)
         hipLaunchKernelGGL((k_contacts_gs<false, true>), g, b, 0, s, faces, fci_off, ncol, ninst, cpos, clist, rest,
-                           pos, dsep, lo, cp, cn, cb, clam);
+                           pos, dsep, dbase, lo, cp, cn, cb, clam);
     else
         hipLaunchKernelGGL((k_contacts_gs<false, false>), g, b, 0, s, faces, fci_off, ncol, ninst, cpos, clist, rest,
-                           pos, nullptr, lo, cp, cn, cb, clam);
+                           pos, nullptr, nullptr, lo, cp, cn, cb, clam);
     return hipGetLastError();
 }
 
