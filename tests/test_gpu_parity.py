@@ -884,3 +884,31 @@ def test_deterministic_jacobi_subset_solve_is_bitwise_reproducible():
         gpu.close()
     for x, lam in out[1:]:
         assert np.array_equal(x, out[0][0]) and np.array_equal(lam, out[0][1])
+
+
+def test_split_layout_with_attachments_falls_back_to_the_interleaved_records(tmp_path):
+    """A handle tuned to a split-layout quad shape runs its iterations in the
+    interleaved records while attachments are set (their pass reads the
+    records) and in the split arrays otherwise: grasping, releasing and
+    grasping again, it gives the bits of the same shape with interleaved
+    records at every step."""
+    rng = np.random.default_rng(13)
+    n = 6
+    E, nu = W.heterogeneous(6 * n ** 3, rng, E_range=(1e3, 5e3))
+    mesh = W.grid_object(n, n, n, 0.01, E=E, nu=nu, pin_axis=2, noise=0.0, rng=rng)
+    top = np.where(np.abs(mesh.x0[:, 2] - n * 0.01) < 1e-9)[0]
+    grasp = top[[0, len(top) // 2, len(top) - 1]]
+    split = _forced(mesh, tmp_path, "quad", X.QUAD_SPLIT + 128, 1, iters=5)
+    inter = _forced(mesh, tmp_path, "quad", 128, 1, iters=5)
+    for k in range(30):
+        if k in (0, 20):
+            tgt = mesh.x0[grasp] + np.array([0.003, -0.002, 0.008])
+            for s in (split, inter):
+                s.set_attachments(grasp, tgt)
+        if k == 10:
+            for s in (split, inter):
+                s.set_attachments([], np.zeros((0, 3)))
+        split.step(1)
+        inter.step(1)
+        assert np.array_equal(split.positions(), inter.positions()), k
+    assert np.max(np.abs(split.positions() - mesh.x0)) > 1e-3
